@@ -1,0 +1,229 @@
+/* mrirecon_hip.h -- C ABI of the MI355X (gfx950) NUFFT + NLCG hot path.
+ *
+ * Plain pointers and sizes only.  "device" pointers are HIP device memory
+ * (e.g. torch.Tensor.data_ptr() of a CUDA tensor on ROCm), "host" pointers are
+ * ordinary CPU memory.  complex64 is interleaved (re, im) float pairs.  Every
+ * kernel is enqueued on the `stream` argument (a hipStream_t; pass the stream
+ * that is current in the caller, never a cached one) and nothing blocks the
+ * host except plan / engine creation and the explicit *_read calls.
+ *
+ * Every entry point returns 0 on success or an MRH_E* code; mrh_last_error()
+ * returns the message of the last failure on the calling thread.
+ *
+ * Reference file:line citations name the function of timodelgado/mrirecon
+ * (package graspcg) that each entry point replaces.
+ */
+#ifndef MRIRECON_HIP_H
+#define MRIRECON_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MRH_DCF_NONE 0
+#define MRH_DCF_BALANCED 1
+#define MRH_DCF_STANDARD 2
+
+/* ------------------------------------------------------------------ errors */
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mrh_last_error(void);
+
+/* ABI version of the library (bumped on any signature change). */
+int mrh_abi_version(void);
+
+/* ------------------------------------------------------------------ NUFFT */
+
+typedef struct mrh_nufft_plan mrh_nufft_plan;
+
+typedef struct {
+  int ndim;                 /* 2 (3-D is not built)                                   */
+  int S0, S1;               /* image size (maps.shape[1:])                            */
+  int G0, G1;               /* oversampled grid size                                  */
+  int W;                    /* Kaiser-Bessel width in grid cells (2..8)                */
+  double beta;              /* KB shape; <= 0 selects Beatty's formula                 */
+  int C;                    /* coils                                                   */
+  int B;                    /* frames (batch of trajectories)                          */
+  int like;                 /* largest Like-fold used at call time (>= 1)              */
+  const int64_t* k_offsets; /* host (B+1): frame b owns samples [k_offsets[b], k_offsets[b+1]) */
+  const float* traj;        /* host: per frame a (2, K_b) block in radians, blocks in frame order */
+  const float* dcf;         /* host (sum K_b) or NULL                                  */
+  int dcf_mode;             /* MRH_DCF_*                                               */
+  int dcf_in_fwd;           /* 'standard' mode: multiply by dcf in A                   */
+  int dcf_in_adj;           /* 'standard' mode: multiply by dcf in A^H                 */
+  const void* maps;         /* device complex64 (C, S0, S1); copied into the plan      */
+  int strip_cols;           /* grid columns per strip workgroup: 8, 16 or 0 (auto)     */
+} mrh_nufft_desc;
+
+typedef struct {
+  int G0, G1, W, strip_cols, n_strips, kb_terms;
+  double beta;
+  int64_t n_samples, n_entries, max_strip_entries, workspace_bytes;
+} mrh_nufft_info;
+
+/* Build a plan: footprints, strip binning, KB coefficients, deapodisation,
+ * twiddles and the intermediate workspace.  Replaces
+ * TorchKbNUFFTAdapter.prepare (graspcg/nufft/backends/torchkb.py:98-144). */
+int mrh_nufft_plan_create(const mrh_nufft_desc* desc, mrh_nufft_plan** plan);
+int mrh_nufft_plan_destroy(mrh_nufft_plan* plan);
+int mrh_nufft_plan_info(const mrh_nufft_plan* plan, mrh_nufft_info* info);
+
+/* Forward A: x (B*like, S0, S1) coil-combined image (or (B*like, C, S0, S1)
+ * coil images when x_has_coils) -> y, frame b / like l block at
+ * C*(k_offsets[b]*like + l*K_b) laid out (C, K_b); for uniform K this is
+ * (B*like, C, K).  Replaces TorchKbNUFFTAdapter.A
+ * (graspcg/nufft/backends/torchkb.py:203-270) and NUFFT.A (graspcg/nufft/api.py:167-220). */
+int mrh_nufft_forward(mrh_nufft_plan* plan, const void* x, void* y, int like, int x_has_coils,
+                      void* stream);
+
+/* Adjoint A^H: y -> x (B*like, S0, S1), coil-combined with conj(maps).
+ * Replaces TorchKbNUFFTAdapter.AH (graspcg/nufft/backends/torchkb.py:273-323)
+ * and NUFFT.AH (graspcg/nufft/api.py:223-270). */
+int mrh_nufft_adjoint(mrh_nufft_plan* plan, const void* y, void* x, int like, void* stream);
+
+/* The two kernels of each direction, launched separately so that a caller can
+ * time them with events on `stream`.  Forward: stage 0 = row pass
+ * (maps * deapod * zero-pad * row FFT, clears y), stage 1 = strip pass (column
+ * FFT + interpolation); stage 0 must precede stage 1.  Adjoint: stage 0 =
+ * strip pass (spreading + inverse column FFT), stage 1 = row pass (inverse row
+ * FFT, crop, deapod, conj(maps) coil sum).  Running both stages in order is
+ * exactly mrh_nufft_forward / mrh_nufft_adjoint.  No counterpart in the
+ * reference (whose NUFFT is a single third-party call, torchkb.py:249,309). */
+int mrh_nufft_forward_stage(mrh_nufft_plan* plan, int stage, const void* x, void* y, int like,
+                            int x_has_coils, void* stream);
+int mrh_nufft_adjoint_stage(mrh_nufft_plan* plan, int stage, const void* y, void* x, int like,
+                            void* stream);
+
+/* ------------------------------------------------------------------ reductions / objective */
+
+/* Number of float64 workspace slots the reduction entry points below need. */
+int64_t mrh_reduce_workspace_len(void);
+
+/* r = t*Ad + Ax - y (in place into r, that order) and out[0] = sum |r|^2 in
+ * float64.  Replaces _ker_residual_energy_ (graspcg/ops/objective.py:31-45).
+ * `work`: device float64[mrh_reduce_workspace_len()]; `out`: device float64. */
+int mrh_residual_energy(void* r, const void* Ax, const void* Ad, const void* y, double t,
+                        int64_t n, double* work, double* out, void* stream);
+
+/* out[0] = Re sum conj(a) * b over n complex64 elements, float64 accumulation.
+ * Replaces dot_real0 / dot_chunked (graspcg/numerics/utils.py:94-101,
+ * graspcg/ops/dot.py:5-71). */
+int mrh_dot_real(const void* a, const void* b, int64_t n, double* work, double* out, void* stream);
+
+/* ------------------------------------------------------------------ TV */
+
+#define MRH_TV_AXIS_T 1  /* temporal (frame) axis, within each problem */
+#define MRH_TV_AXIS_X 2  /* first image axis (S0)                       */
+#define MRH_TV_AXIS_Y 4  /* second image axis (S1)                      */
+#define MRH_TV_MAX_TERMS 3
+
+typedef struct {
+  float weight;      /* lambda                        */
+  float eps;         /* Huber knee                    */
+  int axes;          /* OR of MRH_TV_AXIS_*           */
+  int isotropic;     /* 1: l2 over axes, 0: l1        */
+  float axis_w[3];   /* per-axis weights (t, x, y)    */
+} mrh_tv_term;
+
+typedef struct {
+  int P;             /* independent problems (slices)              */
+  int F;             /* frames per problem                         */
+  int S0, S1;        /* image size                                 */
+  int n_terms;
+  mrh_tv_term term[MRH_TV_MAX_TERMS];
+} mrh_tv_desc;
+
+/* g += sum_terms -lambda * div(flux) at x (when g != NULL) and
+ * energy[p] = sum_terms lambda * sum e for every problem p (device float64[P]).
+ * Image layout (P*F, S0, S1); temporal differences never cross problems.
+ * Replaces TVND.energy_grad_fixed_axes (graspcg/regularization/tv_nd.py:54-88)
+ * with the stencils of graspcg/ops/ndops.py:28-149. */
+int mrh_tv_energy_grad(const mrh_tv_desc* desc, const void* x, void* g, double* work,
+                       double* energy, void* stream);
+
+/* ------------------------------------------------------------------ device NLCG engine */
+
+typedef struct mrh_cg mrh_cg;
+
+#define MRH_CG_DIR_FR 0
+#define MRH_CG_DIR_PRPLUS 1
+
+#define MRH_CG_ACTIVE 0
+#define MRH_CG_LS_FAILED 1
+#define MRH_CG_NOT_DESCENT 2
+#define MRH_CG_REL_TOL 3
+
+typedef struct {
+  mrh_tv_desc tv;           /* image layout + regularisers                      */
+  const int64_t* k_prob;    /* host (P+1): complex64 offsets of each problem's k-space */
+  int direction;            /* MRH_CG_DIR_*                                     */
+  double c1;                /* Armijo constant (1e-4)                           */
+  double ls_beta;           /* backtracking factor (0.5)                        */
+  int ls_max_iter;          /* trials (20)                                      */
+  double rel_tol;           /* stop when |f_k - f_{k-1}| <= rel_tol |f_{k-1}|; 0 = never */
+  int max_hist;             /* iterations of history kept on the device          */
+} mrh_cg_desc;
+
+/* Per-problem, per-iteration record copied out by mrh_cg_read_history. */
+typedef struct {
+  double f0;        /* objective at the start of the iteration (f_base)      */
+  double g0d;       /* Re<g, d> at t = 0                                     */
+  double t;         /* accepted step                                         */
+  double ft;        /* objective at the accepted step                        */
+  double rho;       /* |g|^2 at the new iterate                              */
+  double beta;      /* CG beta used for the next direction                   */
+  double freg;      /* regulariser energy at the new iterate                 */
+  double gdot_step; /* Re<g_new, d> (slope at the accepted step)              */
+  double xnorm2;    /* |x_new|^2                                             */
+  double dnorm2;    /* |d|^2 of the direction that was searched               */
+} mrh_cg_hist;
+
+/* Allocate the engine's per-problem scalars, partial sums and history.
+ * Replaces the solver state of CGSolver.__init__ (graspcg/solvers/cg.py:65-157). */
+int mrh_cg_create(const mrh_cg_desc* desc, mrh_cg** eng);
+int mrh_cg_destroy(mrh_cg* eng);
+/* Reset iteration counters / status (before a new run on the same shapes). */
+int mrh_cg_reset(mrh_cg* eng, void* stream);
+
+/* r = Ax - y over all problems (prime step, t = 0). */
+int mrh_cg_residual(mrh_cg* eng, const void* Ax, const void* y, void* r, void* stream);
+
+/* Device-side Armijo line search for every active problem:
+ * f(t) = 0.5 (a + 2 b t + c t^2) + TV(x + t d) with a = |Ax - y|^2,
+ * b = Re<Ax - y, Ad>, c = |Ad|^2, TV at t in {0, 1, beta, beta^2, ...} from one
+ * pass over x and d, first t with f(t) <= f(0) + c1 t Re<g, d> accepted.
+ * Accepts the same t as sequential backtracking
+ * (graspcg/numerics/line_search.py:24-79 with Objective.f_g_tensor,
+ * graspcg/ops/objective.py:184-291) without evaluating gradients at trials. */
+int mrh_cg_linesearch(mrh_cg* eng, const void* x, const void* d, const void* g, const void* Ax,
+                      const void* Ad, const void* y, void* stream);
+
+/* x += t d, Ax += t Ad, r = Ax - y with each problem's accepted t
+ * (graspcg/solvers/cg.py:205-207; objective residual of the accepted step). */
+int mrh_cg_accept(mrh_cg* eng, void* x, const void* d, void* Ax, const void* Ad, const void* y,
+                  void* r, void* stream);
+
+/* g += TV gradient at x (g holds A^H r on entry); reduces |g|^2, Re<g_prev, g>,
+ * Re<g, d> and the TV energy, then computes beta (FR with the first beta = 1,
+ * or PR+) and the relative-change stop rule.  prime != 0 marks the prime step.
+ * Replaces RegManager.energy_and_grad (graspcg/regularization/manager.py:38-142)
+ * and the reductions of DirFR / DirPRPlus (graspcg/numerics/directions.py:75-162). */
+int mrh_cg_gradient(mrh_cg* eng, const void* x, void* g, const void* d, const void* g_prev,
+                    int prime, void* stream);
+
+/* d = beta d - g (prime: d = -g); PR+ also copies g into g_prev.  Frozen for
+ * problems whose status is not ACTIVE.  (graspcg/numerics/directions.py:158-161,
+ * graspcg/solvers/cg.py:183-184) */
+int mrh_cg_direction(mrh_cg* eng, void* d, const void* g, void* g_prev, int prime, void* stream);
+
+/* Copy status (int[P]) and iteration counts (int[P]) to host (synchronising `stream`). */
+int mrh_cg_read_status(mrh_cg* eng, int* status, int* iters, void* stream);
+/* Copy history (host mrh_cg_hist[n_iter * P], iteration-major) to host. */
+int mrh_cg_read_history(mrh_cg* eng, mrh_cg_hist* hist, int n_iter, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MRIRECON_HIP_H */

@@ -1,0 +1,708 @@
+// Objective, Huber-TV and device-side NLCG kernels for MI355X / gfx950.
+//
+// All reductions accumulate in float64 per thread, reduce per block in a fixed
+// order and are finished by a second kernel that sums the block partials in a
+// fixed order: results are bit-reproducible run to run.
+//
+// TV follows the reference stencils (graspcg/ops/ndops.py:28-149): forward
+// difference with the last slice zero, divergence = backward difference with
+// the first slice kept, isotropic energy sqrt(sum |w_a D_a x|^2 + eps^2) - eps,
+// flux w_a D_a x / max(den, 1e-12), g += -lambda div(flux)
+// (graspcg/regularization/tv_nd.py:70-88).  Temporal differences stay inside
+// each problem (slice) of F frames.
+#include "mrh_common.h"
+#include "mrirecon_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#define RT 256            // threads per reduction block
+#define RED_BLOCKS 1024   // blocks of the standalone reductions
+#define NT 21             // line-search candidates: t = 0, 1, beta, ..., beta^19
+#define PI_STRIDE 32      // doubles per block partial (image pass)
+#define PK_STRIDE 4       // doubles per block partial (k-space pass)
+
+// ---------------------------------------------------------------- block reduction
+template <int NV>
+__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out) {
+  __shared__ double sh[NV][RT / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double w = warp_sum_d(v[k]);
+    if (lane == 0) sh[k][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    double s = 0.0;
+#pragma unroll
+    for (int w = 0; w < RT / 64; ++w) s += sh[threadIdx.x][w];
+    out[threadIdx.x] = s;
+  }
+}
+
+// Sum n partials (stride `stride`, value offset `v`) in a fixed order with one block.
+__device__ double block_sum_partials(const double* part, int n, int stride, int v) {
+  __shared__ double sh[RT];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += RT) s += part[(int64_t)i * stride + v];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = RT / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(RT) void k_finish_sum(const double* part, int n, double* out) {
+  double s = block_sum_partials(part, n, 1, 0);
+  if (threadIdx.x == 0) out[0] = s;
+}
+
+extern "C" int64_t mrh_reduce_workspace_len(void) { return (int64_t)RED_BLOCKS * 64; }
+
+static int red_blocks(int64_t n) {
+  int64_t b = (n + RT * 4 - 1) / (RT * 4);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, b));
+}
+
+// ---------------------------------------------------------------- standalone ops
+__global__ __launch_bounds__(RT) void k_residual_energy(float2* r, const float2* Ax, const float2* Ad,
+                                                        const float2* y, float t, int64_t n, double* part) {
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    float2 a = Ad[i];
+    a.x *= t; a.y *= t;
+    const float2 ax = Ax[i], yy = y[i];
+    a.x += ax.x; a.y += ax.y;
+    a.x -= yy.x; a.y -= yy.y;
+    r[i] = a;
+    v[0] += (double)a.x * a.x + (double)a.y * a.y;
+  }
+  block_reduce_store<1>(v, part + blockIdx.x);
+}
+
+extern "C" int mrh_residual_energy(void* r, const void* Ax, const void* Ad, const void* y, double t,
+                                   int64_t n, double* work, double* out, void* stream) {
+  MRH_REQUIRE(r && Ax && Ad && y && work && out, MRH_EINVAL, "residual_energy: null argument");
+  MRH_REQUIRE(n >= 0, MRH_EINVAL, "residual_energy: n < 0");
+  hipStream_t st = (hipStream_t)stream;
+  int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_residual_energy, dim3(nb), dim3(RT), 0, st, (float2*)r, (const float2*)Ax,
+                     (const float2*)Ad, (const float2*)y, (float)t, n, work);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(RT), 0, st, (const double*)work, nb, out);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+__global__ __launch_bounds__(RT) void k_dot_real(const float2* a, const float2* b, int64_t n, double* part) {
+  double v[1] = {0.0};
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < n; i += (int64_t)gridDim.x * RT) {
+    const float2 p = a[i], q = b[i];
+    v[0] += (double)p.x * q.x + (double)p.y * q.y;
+  }
+  block_reduce_store<1>(v, part + blockIdx.x);
+}
+
+extern "C" int mrh_dot_real(const void* a, const void* b, int64_t n, double* work, double* out, void* stream) {
+  MRH_REQUIRE(a && b && work && out, MRH_EINVAL, "dot_real: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  int nb = red_blocks(n);
+  hipLaunchKernelGGL(k_dot_real, dim3(nb), dim3(RT), 0, st, (const float2*)a, (const float2*)b, n, work);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finish_sum, dim3(1), dim3(RT), 0, st, (const double*)work, nb, out);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+// ---------------------------------------------------------------- TV device helpers
+struct TVTermDev {
+  float lam, eps, w[3];
+  int axes, iso;
+};
+struct TVDev {
+  int n, F, S0, S1;
+  TVTermDev t[MRH_TV_MAX_TERMS];
+};
+
+static int make_tvdev(const mrh_tv_desc* d, TVDev* o) {
+  MRH_REQUIRE(d->P >= 1 && d->F >= 1 && d->S0 >= 1 && d->S1 >= 1, MRH_EINVAL, "tv: bad layout");
+  MRH_REQUIRE(d->n_terms >= 0 && d->n_terms <= MRH_TV_MAX_TERMS, MRH_EINVAL, "tv: n_terms outside [0,3]");
+  std::memset(o, 0, sizeof(*o));
+  o->n = d->n_terms; o->F = d->F; o->S0 = d->S0; o->S1 = d->S1;
+  for (int i = 0; i < d->n_terms; ++i) {
+    const mrh_tv_term& t = d->term[i];
+    MRH_REQUIRE((t.axes & ~7) == 0, MRH_EINVAL, "tv: bad axes mask %d", t.axes);
+    o->t[i].lam = t.weight; o->t[i].eps = t.eps; o->t[i].axes = t.axes; o->t[i].iso = t.isotropic;
+    for (int a = 0; a < 3; ++a) o->t[i].w[a] = t.axis_w[a];
+  }
+  return MRH_OK;
+}
+
+// Weighted forward difference along axis a (0: frame, 1: X, 2: Y) at (f, ix, iy), n = flat index.
+__device__ __forceinline__ float2 fdiff(const float2* __restrict__ v, int64_t n, int f, int ix, int iy,
+                                        int a, const TVDev& tv, float w) {
+  int64_t nn;
+  if (a == 0) { if (f + 1 >= tv.F) return make_float2(0.f, 0.f); nn = n + (int64_t)tv.S0 * tv.S1; }
+  else if (a == 1) { if (ix + 1 >= tv.S0) return make_float2(0.f, 0.f); nn = n + tv.S1; }
+  else { if (iy + 1 >= tv.S1) return make_float2(0.f, 0.f); nn = n + 1; }
+  const float2 p = v[nn], q = v[n];
+  return make_float2(w * (p.x - q.x), w * (p.y - q.y));
+}
+
+// Flux components p_a and energy density of one term at (f, ix, iy).
+__device__ __forceinline__ float term_flux(const float2* __restrict__ x, int64_t n, int f, int ix, int iy,
+                                           const TVTermDev& tm, const TVDev& tv, float2 (&p)[3]) {
+  float2 dv[3];
+  float ss = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    dv[a] = make_float2(0.f, 0.f);
+    if (tm.axes & (1 << a)) dv[a] = fdiff(x, n, f, ix, iy, a, tv, tm.w[a]);
+  }
+  const float e2 = tm.eps * tm.eps;
+  float e = 0.f;
+  if (tm.iso) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) ss += dv[a].x * dv[a].x + dv[a].y * dv[a].y;
+    const float den = sqrtf(ss + e2);
+    e = den - tm.eps;
+    const float inv = 1.f / fmaxf(den, 1e-12f);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) p[a] = cscale(dv[a], inv);
+  } else {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (tm.axes & (1 << a)) {
+        const float den = sqrtf(dv[a].x * dv[a].x + dv[a].y * dv[a].y + e2);
+        e += den - tm.eps;
+        p[a] = cscale(dv[a], 1.f / fmaxf(den, 1e-12f));
+      } else {
+        p[a] = make_float2(0.f, 0.f);
+      }
+    }
+  }
+  return e;
+}
+
+// TV gradient contribution (-lambda div flux) and lambda * energy density of all terms at pixel n.
+__device__ __forceinline__ float2 tv_grad_at(const float2* __restrict__ x, int64_t n, int f, int ix, int iy,
+                                             const TVDev& tv, float& energy) {
+  float2 gsum = make_float2(0.f, 0.f);
+  energy = 0.f;
+  for (int k = 0; k < tv.n; ++k) {
+    const TVTermDev& tm = tv.t[k];
+    if (tm.lam == 0.f || tm.axes == 0) continue;
+    float2 p[3];
+    const float e = term_flux(x, n, f, ix, iy, tm, tv, p);
+    float2 div = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      if (!(tm.axes & (1 << a))) continue;
+      div = cadd(div, p[a]);
+      int fb = f, xb = ix, yb = iy;
+      int64_t nb;
+      bool has;
+      if (a == 0) { has = f > 0; fb = f - 1; nb = n - (int64_t)tv.S0 * tv.S1; }
+      else if (a == 1) { has = ix > 0; xb = ix - 1; nb = n - tv.S1; }
+      else { has = iy > 0; yb = iy - 1; nb = n - 1; }
+      if (has) {
+        float2 pb[3];
+        (void)term_flux(x, nb, fb, xb, yb, tm, tv, pb);
+        div = csub(div, pb[a]);
+      }
+    }
+    gsum.x -= tm.lam * div.x;
+    gsum.y -= tm.lam * div.y;
+    energy += tm.lam * e;
+  }
+  return gsum;
+}
+
+// ---------------------------------------------------------------- standalone TV
+__global__ __launch_bounds__(RT) void k_tv_energy_grad(const float2* __restrict__ x, float2* __restrict__ g,
+                                                       TVDev tv, int64_t npix_p, double* part) {
+  const int p = blockIdx.y;
+  double v[1] = {0.0};
+  const int plane = tv.S0 * tv.S1;
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
+    const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
+    const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
+    const int64_t n = (int64_t)p * npix_p + i;
+    float e;
+    const float2 gt = tv_grad_at(x, n, f, ix, iy, tv, e);
+    if (g) { float2 gv = g[n]; g[n] = cadd(gv, gt); }
+    v[0] += (double)e;
+  }
+  block_reduce_store<1>(v, part + (int64_t)p * gridDim.x + blockIdx.x);
+}
+
+__global__ __launch_bounds__(RT) void k_finish_per_problem(const double* part, int nb, double* out) {
+  const int p = blockIdx.x;
+  double s = block_sum_partials(part + (int64_t)p * nb, nb, 1, 0);
+  if (threadIdx.x == 0) out[p] = s;
+}
+
+extern "C" int mrh_tv_energy_grad(const mrh_tv_desc* desc, const void* x, void* g, double* work,
+                                  double* energy, void* stream) {
+  MRH_REQUIRE(desc && x && work && energy, MRH_EINVAL, "tv_energy_grad: null argument");
+  TVDev tv;
+  int rc = make_tvdev(desc, &tv);
+  if (rc) return rc;
+  const int64_t npix_p = (int64_t)desc->F * desc->S0 * desc->S1;
+  int nb = red_blocks(npix_p);
+  nb = std::max(1, std::min<int>(nb, (int)(mrh_reduce_workspace_len() / desc->P)));
+  MRH_REQUIRE(desc->P <= 65535, MRH_EINVAL, "tv: too many problems");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_tv_energy_grad, dim3(nb, desc->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
+                     tv, npix_p, work);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finish_per_problem, dim3(desc->P), dim3(RT), 0, st, (const double*)work, nb, energy);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+// ---------------------------------------------------------------- NLCG engine
+struct ProbState {
+  double a, b, c, g0d, dd;
+  double tv[NT];
+  double f0, t, ft, rho, rho_prev, beta, gpg, freg, gdot_step, f_prev;
+  int status, iter, have_rho_prev, have_f_prev;
+};
+
+struct mrh_cg {
+  int P, F, S0, S1, direction, ls_max_iter, max_hist;
+  double c1, ls_beta, rel_tol;
+  int64_t npix_p, kmax_p;
+  int nb_i, nb_k;
+  TVDev tv;
+  int64_t* d_kprob;
+  double* d_part_i;   // [P][nb_i][PI_STRIDE]
+  double* d_part_k;   // [P][nb_k][PK_STRIDE]
+  ProbState* d_st;
+  mrh_cg_hist* d_hist;  // [max_hist][P]
+};
+
+__global__ void k_cg_reset(ProbState* st, int P) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  ProbState s;
+  double* z = (double*)&s;
+  for (size_t i = 0; i < sizeof(ProbState) / sizeof(double); ++i) z[i] = 0.0;
+  s.status = MRH_CG_ACTIVE; s.iter = 0; s.have_rho_prev = 0; s.have_f_prev = 0;
+  st[p] = s;
+}
+
+__global__ __launch_bounds__(RT) void k_cg_residual(const float2* __restrict__ Ax, const float2* __restrict__ y,
+                                                    float2* __restrict__ r, const int64_t* __restrict__ kprob) {
+  const int p = blockIdx.y;
+  const int64_t k0 = kprob[p], k1 = kprob[p + 1];
+  for (int64_t i = k0 + blockIdx.x * (int64_t)RT + threadIdx.x; i < k1; i += (int64_t)gridDim.x * RT) {
+    const float2 a = Ax[i], b = y[i];
+    r[i] = make_float2(a.x - b.x, a.y - b.y);
+  }
+}
+
+// Pass 1 of the line search: a = |Ax - y|^2, b = Re<Ax - y, Ad>, c = |Ad|^2 per problem.
+__global__ __launch_bounds__(RT) void k_ls_kspace(const float2* __restrict__ Ax, const float2* __restrict__ Ad,
+                                                  const float2* __restrict__ y, const int64_t* __restrict__ kprob,
+                                                  const ProbState* __restrict__ st, double* part) {
+  const int p = blockIdx.y;
+  double v[3] = {0.0, 0.0, 0.0};
+  if (st[p].status == MRH_CG_ACTIVE) {
+    const int64_t k0 = kprob[p], k1 = kprob[p + 1];
+    for (int64_t i = k0 + blockIdx.x * (int64_t)RT + threadIdx.x; i < k1; i += (int64_t)gridDim.x * RT) {
+      const float2 ax = Ax[i], yy = y[i], ad = Ad[i];
+      const float rx = ax.x - yy.x, ry = ax.y - yy.y;
+      v[0] += (double)rx * rx + (double)ry * ry;
+      v[1] += (double)rx * ad.x + (double)ry * ad.y;
+      v[2] += (double)ad.x * ad.x + (double)ad.y * ad.y;
+    }
+  }
+  block_reduce_store<3>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PK_STRIDE);
+}
+
+// Pass 2 of the line search: Re<g, d>, |d|^2 and TV(x + t_j d) for all candidates t_j.
+__global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, const float2* __restrict__ d,
+                                                 const float2* __restrict__ g, TVDev tv, int64_t npix_p,
+                                                 double ls_beta, const ProbState* __restrict__ st, double* part) {
+  const int p = blockIdx.y;
+  double v[2 + NT];
+#pragma unroll
+  for (int j = 0; j < 2 + NT; ++j) v[j] = 0.0;
+  float tj[NT];
+  tj[0] = 0.f;
+  {
+    double t = 1.0;
+#pragma unroll
+    for (int j = 1; j < NT; ++j) { tj[j] = (float)t; t *= ls_beta; }
+  }
+  if (st[p].status == MRH_CG_ACTIVE) {
+    const int plane = tv.S0 * tv.S1;
+    for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
+      const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
+      const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
+      const int64_t n = (int64_t)p * npix_p + i;
+      const float2 dn = d[n], gn = g[n];
+      v[0] += (double)gn.x * dn.x + (double)gn.y * dn.y;
+      v[1] += (double)dn.x * dn.x + (double)dn.y * dn.y;
+      for (int k = 0; k < tv.n; ++k) {
+        const TVTermDev& tm = tv.t[k];
+        if (tm.lam == 0.f || tm.axes == 0) continue;
+        float2 dx[3], dd[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          dx[a] = make_float2(0.f, 0.f); dd[a] = make_float2(0.f, 0.f);
+          if (tm.axes & (1 << a)) {
+            dx[a] = fdiff(x, n, f, ix, iy, a, tv, tm.w[a]);
+            dd[a] = fdiff(d, n, f, ix, iy, a, tv, tm.w[a]);
+          }
+        }
+        const float e2 = tm.eps * tm.eps;
+#pragma unroll
+        for (int j = 0; j < NT; ++j) {
+          const float t = tj[j];
+          float e = 0.f;
+          if (tm.iso) {
+            float ss = 0.f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const float zx = fmaf(t, dd[a].x, dx[a].x), zy = fmaf(t, dd[a].y, dx[a].y);
+              ss = fmaf(zx, zx, fmaf(zy, zy, ss));
+            }
+            e = sqrtf(ss + e2) - tm.eps;
+          } else {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              if (tm.axes & (1 << a)) {
+                const float zx = fmaf(t, dd[a].x, dx[a].x), zy = fmaf(t, dd[a].y, dx[a].y);
+                e += sqrtf(fmaf(zx, zx, zy * zy) + e2) - tm.eps;
+              }
+            }
+          }
+          v[2 + j] += (double)(tm.lam * e);
+        }
+      }
+    }
+  }
+  block_reduce_store<2 + NT>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
+}
+
+// Finish the line-search reductions and select the step (one block per problem).
+__global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* part_i, const double* part_k,
+                                                  int nb_i, int nb_k, double c1, double ls_beta, int ls_max,
+                                                  mrh_cg_hist* hist, int max_hist, int P) {
+  const int p = blockIdx.x;
+  __shared__ double vals[3 + 2 + NT];
+  for (int v = 0; v < 3; ++v) {
+    double s = block_sum_partials(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, v);
+    if (threadIdx.x == 0) vals[v] = s;
+  }
+  for (int v = 0; v < 2 + NT; ++v) {
+    double s = block_sum_partials(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, v);
+    if (threadIdx.x == 0) vals[3 + v] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  ProbState& s = st[p];
+  if (s.status != MRH_CG_ACTIVE) { s.t = 0.0; return; }
+  s.a = vals[0]; s.b = vals[1]; s.c = vals[2]; s.g0d = vals[3]; s.dd = vals[4];
+  for (int j = 0; j < NT; ++j) s.tv[j] = vals[5 + j];
+  const double f0 = 0.5 * s.a + s.tv[0];
+  s.f0 = f0;
+  s.t = 0.0;
+  s.ft = f0;
+  if (!(s.g0d < 0.0) || !isfinite(s.g0d)) {
+    s.status = MRH_CG_NOT_DESCENT;
+    return;
+  }
+  double t = 1.0;
+  bool ok = false;
+  for (int j = 1; j <= ls_max && j < NT; ++j) {
+    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j];
+    if (ft <= f0 + c1 * t * s.g0d) { s.t = t; s.ft = ft; ok = true; break; }
+    t *= ls_beta;
+  }
+  if (!ok) { s.status = MRH_CG_LS_FAILED; s.t = 0.0; return; }
+  if (s.iter < max_hist) {
+    mrh_cg_hist& h = hist[(int64_t)s.iter * P + p];
+    h.f0 = f0; h.g0d = s.g0d; h.t = s.t; h.ft = s.ft;
+  }
+}
+
+// x += t d (image) for every problem.
+__global__ __launch_bounds__(RT) void k_accept_img(float2* __restrict__ x, const float2* __restrict__ d,
+                                                   int64_t npix_p, const ProbState* __restrict__ st) {
+  const int p = blockIdx.y;
+  const float t = (float)st[p].t;
+  if (t == 0.f) return;
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
+    const int64_t n = (int64_t)p * npix_p + i;
+    const float2 dv = d[n];
+    float2 xv = x[n];
+    xv.x = fmaf(t, dv.x, xv.x);
+    xv.y = fmaf(t, dv.y, xv.y);
+    x[n] = xv;
+  }
+}
+
+// Ax += t Ad, r = Ax - y (k-space) for every problem.
+__global__ __launch_bounds__(RT) void k_accept_k(float2* __restrict__ Ax, const float2* __restrict__ Ad,
+                                                 const float2* __restrict__ y, float2* __restrict__ r,
+                                                 const int64_t* __restrict__ kprob, const ProbState* __restrict__ st) {
+  const int p = blockIdx.y;
+  const float t = (float)st[p].t;
+  const int64_t k0 = kprob[p], k1 = kprob[p + 1];
+  for (int64_t i = k0 + blockIdx.x * (int64_t)RT + threadIdx.x; i < k1; i += (int64_t)gridDim.x * RT) {
+    float2 a = Ax[i];
+    if (t != 0.f) {
+      const float2 ad = Ad[i];
+      a.x = fmaf(t, ad.x, a.x);
+      a.y = fmaf(t, ad.y, a.y);
+      Ax[i] = a;
+    }
+    const float2 yy = y[i];
+    r[i] = make_float2(a.x - yy.x, a.y - yy.y);
+  }
+}
+
+// g += TV gradient at x; partials: TV energy, |g|^2, Re<g_prev, g>, Re<g, d>.
+__global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, float2* __restrict__ g,
+                                                const float2* __restrict__ d, const float2* __restrict__ gp,
+                                                TVDev tv, int64_t npix_p, const ProbState* __restrict__ st,
+                                                int prime, double* part) {
+  const int p = blockIdx.y;
+  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (prime || st[p].status == MRH_CG_ACTIVE) {
+    const int plane = tv.S0 * tv.S1;
+    for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
+      const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
+      const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
+      const int64_t n = (int64_t)p * npix_p + i;
+      float e = 0.f;
+      float2 gv = g[n];
+      if (tv.n > 0) gv = cadd(gv, tv_grad_at(x, n, f, ix, iy, tv, e));
+      g[n] = gv;
+      const float2 xv = x[n];
+      v[4] += (double)xv.x * xv.x + (double)xv.y * xv.y;
+      v[0] += (double)e;
+      v[1] += (double)gv.x * gv.x + (double)gv.y * gv.y;
+      if (gp) { const float2 q = gp[n]; v[2] += (double)q.x * gv.x + (double)q.y * gv.y; }
+      if (d) { const float2 q = d[n]; v[3] += (double)q.x * gv.x + (double)q.y * gv.y; }
+    }
+  }
+  block_reduce_store<5>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
+}
+
+// Finish gradient reductions: beta, stop rule, history, iteration count.
+__global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* part_i, int nb_i, int prime,
+                                                int direction, double rel_tol, mrh_cg_hist* hist, int max_hist, int P) {
+  const int p = blockIdx.x;
+  __shared__ double vals[5];
+  for (int v = 0; v < 5; ++v) {
+    double s = block_sum_partials(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, v);
+    if (threadIdx.x == 0) vals[v] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  ProbState& s = st[p];
+  if (!prime && s.status != MRH_CG_ACTIVE) return;
+  s.freg = vals[0];
+  s.rho = vals[1];
+  s.gpg = vals[2];
+  s.gdot_step = vals[3];
+  if (prime) {
+    s.beta = 0.0;
+    if (direction == MRH_CG_DIR_PRPLUS) { s.rho_prev = s.rho; s.have_rho_prev = 1; }
+    return;
+  }
+  double beta;
+  if (direction == MRH_CG_DIR_PRPLUS) {
+    beta = (s.rho - s.gpg) / fmax(s.rho_prev, 1e-20);
+    beta = fmax(beta, 0.0);
+  } else {
+    if (!s.have_rho_prev) { s.rho_prev = s.rho; s.have_rho_prev = 1; }  // first beta = 1 (directions.py:145-149)
+    beta = s.rho / fmax(s.rho_prev, 1e-20);
+  }
+  s.rho_prev = s.rho;
+  s.beta = beta;
+  if (s.iter < max_hist) {
+    mrh_cg_hist& h = hist[(int64_t)s.iter * P + p];
+    h.rho = s.rho; h.beta = beta; h.freg = s.freg; h.gdot_step = s.gdot_step;
+    h.xnorm2 = vals[4]; h.dnorm2 = s.dd;
+  }
+  s.iter += 1;
+  if (rel_tol > 0.0 && s.have_f_prev && fabs(s.ft - s.f_prev) <= rel_tol * fabs(s.f_prev))
+    s.status = MRH_CG_REL_TOL;
+  s.f_prev = s.ft;
+  s.have_f_prev = 1;
+}
+
+// d = beta d - g (prime: d = -g); PR+ keeps g in g_prev.
+__global__ __launch_bounds__(RT) void k_cg_direction(float2* __restrict__ d, const float2* __restrict__ g,
+                                                     float2* __restrict__ gp, int64_t npix_p,
+                                                     const ProbState* __restrict__ st, int prime) {
+  const int p = blockIdx.y;
+  if (!prime && st[p].status != MRH_CG_ACTIVE) return;
+  const float beta = prime ? 0.f : (float)st[p].beta;
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
+    const int64_t n = (int64_t)p * npix_p + i;
+    const float2 gv = g[n];
+    float2 dv = prime ? make_float2(0.f, 0.f) : d[n];
+    dv.x = beta * dv.x - gv.x;
+    dv.y = beta * dv.y - gv.y;
+    d[n] = dv;
+    if (gp) gp[n] = gv;
+  }
+}
+
+// ---------------------------------------------------------------- engine C ABI
+static int blocks_for(int64_t n) {
+  int64_t b = (n + RT * 8 - 1) / (RT * 8);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(512, b));
+}
+
+extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
+  MRH_REQUIRE(d && out && d->k_prob, MRH_EINVAL, "cg_create: null argument");
+  *out = nullptr;
+  MRH_REQUIRE(d->ls_max_iter >= 1 && d->ls_max_iter <= NT - 1, MRH_EUNSUP,
+              "ls_max_iter must be in [1, %d]", NT - 1);
+  MRH_REQUIRE(d->direction == MRH_CG_DIR_FR || d->direction == MRH_CG_DIR_PRPLUS, MRH_EINVAL,
+              "direction must be FR or PR+");
+  MRH_REQUIRE(d->tv.P >= 1 && d->tv.P <= 65535, MRH_EINVAL, "P outside [1, 65535]");
+  mrh_cg* e = new mrh_cg();
+  std::memset(e, 0, sizeof(*e));
+  int rc = make_tvdev(&d->tv, &e->tv);
+  if (rc) { delete e; return rc; }
+  e->P = d->tv.P; e->F = d->tv.F; e->S0 = d->tv.S0; e->S1 = d->tv.S1;
+  e->direction = d->direction; e->ls_max_iter = d->ls_max_iter;
+  e->c1 = d->c1; e->ls_beta = d->ls_beta; e->rel_tol = d->rel_tol;
+  e->max_hist = std::max(1, d->max_hist);
+  e->npix_p = (int64_t)e->F * e->S0 * e->S1;
+  int64_t kmax = 0;
+  for (int p = 0; p < e->P; ++p) {
+    int64_t n = d->k_prob[p + 1] - d->k_prob[p];
+    if (n < 0) { delete e; mrh_set_error("k_prob must be non-decreasing"); return MRH_EINVAL; }
+    kmax = std::max(kmax, n);
+  }
+  e->kmax_p = kmax;
+  e->nb_i = blocks_for(e->npix_p);
+  e->nb_k = blocks_for(kmax);
+  hipError_t err;
+#define ALLOC(ptr, bytes)                                                         \
+  err = hipMalloc((void**)&(ptr), (bytes));                                      \
+  if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: hipMalloc failed: %s", hipGetErrorString(err)); return MRH_ENOMEM; }
+  ALLOC(e->d_kprob, sizeof(int64_t) * (e->P + 1));
+  ALLOC(e->d_part_i, sizeof(double) * (size_t)e->P * e->nb_i * PI_STRIDE);
+  ALLOC(e->d_part_k, sizeof(double) * (size_t)e->P * e->nb_k * PK_STRIDE);
+  ALLOC(e->d_st, sizeof(ProbState) * e->P);
+  ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
+#undef ALLOC
+  err = hipMemcpy(e->d_kprob, d->k_prob, sizeof(int64_t) * (e->P + 1), hipMemcpyHostToDevice);
+  if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: copy failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
+  hipLaunchKernelGGL(k_cg_reset, dim3((e->P + 63) / 64), dim3(64), 0, 0, e->d_st, e->P);
+  err = hipDeviceSynchronize();
+  if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: reset failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
+  *out = e;
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_destroy(mrh_cg* e) {
+  if (!e) return MRH_OK;
+  void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist};
+  for (void* b : bufs) if (b) (void)hipFree(b);
+  delete e;
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_reset(mrh_cg* e, void* stream) {
+  MRH_REQUIRE(e, MRH_EINVAL, "cg_reset: null engine");
+  hipLaunchKernelGGL(k_cg_reset, dim3((e->P + 63) / 64), dim3(64), 0, (hipStream_t)stream, e->d_st, e->P);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_residual(mrh_cg* e, const void* Ax, const void* y, void* r, void* stream) {
+  MRH_REQUIRE(e && Ax && y && r, MRH_EINVAL, "cg_residual: null argument");
+  hipLaunchKernelGGL(k_cg_residual, dim3(e->nb_k, e->P), dim3(RT), 0, (hipStream_t)stream, (const float2*)Ax,
+                     (const float2*)y, (float2*)r, e->d_kprob);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const void* g, const void* Ax,
+                                 const void* Ad, const void* y, void* stream) {
+  MRH_REQUIRE(e && x && d && g && Ax && Ad && y, MRH_EINVAL, "cg_linesearch: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_ls_kspace, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
+                     (const float2*)y, e->d_kprob, e->d_st, e->d_part_k);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ls_image, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (const float2*)d,
+                     (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
+                     e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_accept(mrh_cg* e, void* x, const void* d, void* Ax, const void* Ad, const void* y,
+                             void* r, void* stream) {
+  MRH_REQUIRE(e && x && d && Ax && Ad && y && r, MRH_EINVAL, "cg_accept: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_accept_img, dim3(e->nb_i, e->P), dim3(RT), 0, st, (float2*)x, (const float2*)d,
+                     e->npix_p, e->d_st);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_accept_k, dim3(e->nb_k, e->P), dim3(RT), 0, st, (float2*)Ax, (const float2*)Ad,
+                     (const float2*)y, (float2*)r, e->d_kprob, e->d_st);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d, const void* g_prev,
+                               int prime, void* stream) {
+  MRH_REQUIRE(e && x && g, MRH_EINVAL, "cg_gradient: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_cg_grad, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
+                     (const float2*)d, (const float2*)g_prev, e->tv, e->npix_p, e->d_st, prime, e->d_part_i);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
+                     e->direction, e->rel_tol, e->d_hist, e->max_hist, e->P);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_direction(mrh_cg* e, void* d, const void* g, void* g_prev, int prime, void* stream) {
+  MRH_REQUIRE(e && d && g, MRH_EINVAL, "cg_direction: null argument");
+  hipLaunchKernelGGL(k_cg_direction, dim3(e->nb_i, e->P), dim3(RT), 0, (hipStream_t)stream, (float2*)d,
+                     (const float2*)g, (float2*)g_prev, e->npix_p, e->d_st, prime);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_read_status(mrh_cg* e, int* status, int* iters, void* stream) {
+  MRH_REQUIRE(e, MRH_EINVAL, "cg_read_status: null engine");
+  MRH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  std::vector<ProbState> h(e->P);
+  MRH_HIP(hipMemcpy(h.data(), e->d_st, sizeof(ProbState) * e->P, hipMemcpyDeviceToHost));
+  for (int p = 0; p < e->P; ++p) {
+    if (status) status[p] = h[p].status;
+    if (iters) iters[p] = h[p].iter;
+  }
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_read_history(mrh_cg* e, mrh_cg_hist* hist, int n_iter, void* stream) {
+  MRH_REQUIRE(e && hist, MRH_EINVAL, "cg_read_history: null argument");
+  MRH_REQUIRE(n_iter >= 0 && n_iter <= e->max_hist, MRH_EINVAL, "n_iter outside [0, %d]", e->max_hist);
+  MRH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  if (n_iter == 0) return MRH_OK;
+  MRH_HIP(hipMemcpy(hist, e->d_hist, sizeof(mrh_cg_hist) * (size_t)n_iter * e->P, hipMemcpyDeviceToHost));
+  return MRH_OK;
+}

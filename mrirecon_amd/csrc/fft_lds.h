@@ -1,0 +1,163 @@
+// LDS-resident Stockham FFT for CDNA4, one transform per group of NTHR lanes.
+//
+// Formulation (Govindaraju et al., "High performance discrete Fourier
+// transforms on graphics processors", SC'08): pass p with radix R and
+// accumulated length Ns reads v[r] = buf[j + r N/R], multiplies by the twiddle
+// W_N^{(j mod Ns) r N/(Ns R)}, applies an R-point DFT and writes
+// buf[(j - j mod Ns) R + j mod Ns + r Ns].  The output is in natural order, so
+// no bit reversal is needed.  Every pass holds its butterflies in registers
+// across a barrier, so one LDS buffer per transform is enough.
+//
+// Sizes are compile-time (N = 2^a 3^b 5^c); radices are chosen 8 first, then
+// 4, 2, 3, 5.  LDS indices are padded by one element per 16 (PIDX) so that the
+// stride-R writes of the early passes spread over the banks.
+#pragma once
+#include "mrh_common.h"
+
+#define PIDX(i) ((i) + ((i) >> 4))
+
+__host__ __device__ constexpr int fft_padded(int n) { return n + (n >> 4); }
+
+template <bool INV>
+__device__ __forceinline__ float2 mul_mi(float2 z) {  // z * (-i) forward, z * (+i) inverse
+  return INV ? make_float2(-z.y, z.x) : make_float2(z.y, -z.x);
+}
+
+template <int R, bool INV> struct DFT;
+
+template <bool INV> struct DFT<2, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b); v[1] = csub(a, b);
+  }
+};
+
+template <bool INV> struct DFT<4, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    float2 t2 = cadd(v[1], v[3]), t3 = mul_mi<INV>(csub(v[1], v[3]));
+    v[0] = cadd(t0, t2); v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3); v[3] = csub(t1, t3);
+  }
+};
+
+template <bool INV> struct DFT<8, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    float2 e[4] = {v[0], v[2], v[4], v[6]};
+    float2 o[4] = {v[1], v[3], v[5], v[7]};
+    DFT<4, INV>::run(e);
+    DFT<4, INV>::run(o);
+    const float h = 0.70710678118654752440f;
+    // w^1 = (1 -/+ i)/sqrt2, w^2 = -/+ i, w^3 = (-1 -/+ i)/sqrt2
+    float2 o1 = INV ? make_float2(h * (o[1].x - o[1].y), h * (o[1].x + o[1].y))
+                    : make_float2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+    float2 o2 = mul_mi<INV>(o[2]);
+    float2 o3 = INV ? make_float2(-h * (o[3].x + o[3].y), h * (o[3].x - o[3].y))
+                    : make_float2(h * (o[3].y - o[3].x), -h * (o[3].x + o[3].y));
+    v[0] = cadd(e[0], o[0]); v[4] = csub(e[0], o[0]);
+    v[1] = cadd(e[1], o1);   v[5] = csub(e[1], o1);
+    v[2] = cadd(e[2], o2);   v[6] = csub(e[2], o2);
+    v[3] = cadd(e[3], o3);   v[7] = csub(e[3], o3);
+  }
+};
+
+template <bool INV> struct DFT<3, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    const float c = -0.5f, s = 0.86602540378443864676f;
+    float2 sm = cadd(v[1], v[2]), df = csub(v[1], v[2]);
+    float2 a = make_float2(fmaf(c, sm.x, v[0].x), fmaf(c, sm.y, v[0].y));
+    float2 b = mul_mi<INV>(cscale(df, s));
+    v[0] = cadd(v[0], sm);
+    v[1] = cadd(a, b);
+    v[2] = csub(a, b);
+  }
+};
+
+template <bool INV> struct DFT<5, INV> {
+  __device__ __forceinline__ static void run(float2* v) {
+    // y_q = sum_r v_r w^{rq}, w = exp(-/+ 2 pi i / 5), via the symmetric pairs
+    const float c1 = 0.30901699437494742410f, c2 = -0.80901699437494742410f;
+    const float s1 = 0.95105651629515357212f, s2 = 0.58778525229247312917f;
+    float2 a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
+    float2 a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
+    float2 r1 = make_float2(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    float2 r2 = make_float2(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    float2 i1 = mul_mi<INV>(make_float2(s1 * b1.x + s2 * b2.x, s1 * b1.y + s2 * b2.y));
+    float2 i2 = mul_mi<INV>(make_float2(s2 * b1.x - s1 * b2.x, s2 * b1.y - s1 * b2.y));
+    v[0] = make_float2(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
+    v[1] = cadd(r1, i1); v[4] = csub(r1, i1);
+    v[2] = cadd(r2, i2); v[3] = csub(r2, i2);
+  }
+};
+
+__host__ __device__ constexpr int fft_next_radix(int rem) {
+  return (rem % 8 == 0) ? 8 : (rem % 4 == 0) ? 4 : (rem % 2 == 0) ? 2
+       : (rem % 3 == 0) ? 3 : (rem % 5 == 0) ? 5 : 0;
+}
+
+__host__ __device__ constexpr bool fft_supported(int n) {
+  while (n > 1) {
+    int r = fft_next_radix(n);
+    if (r == 0) return false;
+    n /= r;
+  }
+  return true;
+}
+
+// One Stockham pass.  `tw` holds W_N^k = exp(-2 pi i k / N), k < N (LDS).
+template <int N, int R, int NS, bool INV, int NTHR>
+__device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int t) {
+  constexpr int NBF = N / R;
+  constexpr int NB = (NBF + NTHR - 1) / NTHR;
+  constexpr int TSTEP = N / (NS * R);
+  float2 v[NB][R];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int j = t + q * NTHR;
+    if (NBF % NTHR == 0 || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[q][r] = buf[PIDX(j + r * NBF)];
+      if (NS > 1) {
+        const int step = (j % NS) * TSTEP;
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          float2 w = tw[step * r];
+          if (INV) w.y = -w.y;
+          v[q][r] = cmul(v[q][r], w);
+        }
+      }
+      DFT<R, INV>::run(v[q]);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int j = t + q * NTHR;
+    if (NBF % NTHR == 0 || j < NBF) {
+      const int jm = j % NS;
+      const int base = (j - jm) * R + jm;
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[PIDX(base + r * NS)] = v[q][r];
+    }
+  }
+  __syncthreads();
+}
+
+template <int N, int NS, bool INV, int NTHR>
+struct FFTStages {
+  __device__ __forceinline__ static void run(float2* buf, const float2* tw, int t) {
+    if constexpr (NS < N) {
+      constexpr int R = fft_next_radix(N / NS);
+      static_assert(R != 0, "unsupported FFT size");
+      fft_pass<N, R, NS, INV, NTHR>(buf, tw, t);
+      FFTStages<N, NS * R, INV, NTHR>::run(buf, tw, t);
+    }
+  }
+};
+
+// In-place length-N DFT of buf (padded layout); every thread of the block must
+// call it (it contains barriers).  Forward: exp(-i), inverse: exp(+i), unscaled.
+template <int N, bool INV, int NTHR>
+__device__ __forceinline__ void fft_lds(float2* buf, const float2* tw, int t) {
+  FFTStages<N, 1, INV, NTHR>::run(buf, tw, t);
+}

@@ -1,0 +1,667 @@
+// Kaiser-Bessel gridding NUFFT (2-D SENSE operator) for MI355X / gfx950.
+//
+// Forward  A: x (nbatch, S0, S1) complex64 -> y (flat k-space)
+//   1. k_fwd_rows   : x * map_c * deapod, centred zero-pad to G1, length-G1 FFT
+//                     along the second image axis for every image row (LDS
+//                     Stockham), written to the strip-major intermediate T
+//                     (cf, strip, S0, Tc).  Also clears this coil-frame's y.
+//   2. k_fwd_strips : one workgroup per (coil-frame, strip of Tc grid
+//                     columns): loads the S0 x Tc block of T, zero-pads to G0,
+//                     FFTs the Tc columns in LDS, then interpolates every
+//                     sample whose KB footprint touches the strip straight out
+//                     of LDS.  A footprint spans at most two strips, so each
+//                     sample receives at most two partial sums (atomic adds of
+//                     two terms onto zero commute exactly).
+// Adjoint  A^H: the same two passes reversed: k_adj_strips spreads the strip's
+//   samples into an LDS-resident strip (LDS float atomics), inverse-FFTs the
+//   columns and writes only the S0 cropped rows; k_adj_rows inverse-FFTs each
+//   row of every coil, crops, deapodises and accumulates conj(map_c) over the
+//   coils in registers.
+// The G0 x G1 oversampled grid never exists in HBM.
+//
+// Operator (pinned by the FP64 oracle):
+//   y_k = wA_k sum_{i,j<W} phi(u_k - m0x - i) phi(v_k - m0y - j)
+//         * FFT2[ pad( x * S_c * dpx dpy ) ][(m0x+i) mod G0, (m0y+j) mod G1]
+//   u = w0 G0 / 2pi, m0x = floor(u - W/2) + 1, phi(d) = I0(beta sqrt(1-(2d/W)^2)) / I0(beta),
+//   image index i placed at grid index (i - S/2) mod G.
+#include "mrh_common.h"
+#include "fft_lds.h"
+#include "mrirecon_hip.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+// ---------------------------------------------------------------- KB window
+#define MRH_KB_MAXNP 32
+struct KBPoly {
+  float qscale;      // beta^2 / 4
+  float two_over_w;  // 2 / W
+  float a[MRH_KB_MAXNP];  // a_n = 1 / ((n!)^2 I0(beta)), zero-padded
+};
+
+// phi(d) = sum_n a_n q^n with q = (beta^2/4)(1 - (2d/W)^2): the power series of
+// I0(beta sqrt(1 - (2d/W)^2)) / I0(beta) in Horner form (all terms positive).
+template <int NP>
+__device__ __forceinline__ float kb_weight(const KBPoly& kb, float d) {
+  const float s = d * kb.two_over_w;
+  const float q = kb.qscale * fmaxf(1.f - s * s, 0.f);
+  float p = kb.a[NP - 1];
+#pragma unroll
+  for (int n = NP - 2; n >= 0; --n) p = fmaf(p, q, kb.a[n]);
+  return p;
+}
+
+#define MRH_WMAX 8
+
+// ---------------------------------------------------------------- plan
+struct mrh_nufft_plan {
+  int S0, S1, G0, G1, W, C, B, like_max, Tc, nstrips, np;
+  double beta;
+  int64_t Ktot, n_entries;
+  int max_strip_entries;
+  KBPoly kb;
+  int device;
+  int64_t* d_koff;
+  int2* d_em;        // (m0x | m0y << 16, k)
+  float4* d_ef;      // (u - m0x, v - m0y, wA, wAH)
+  int* d_soff;       // B * nstrips + 1
+  float* d_dx;
+  float* d_dy;
+  float2* d_tw0;
+  float2* d_tw1;
+  float2* d_maps;
+  float2* d_T;
+  size_t T_elems;
+};
+
+static double bessel_i0(double x) {
+  double q = 0.25 * x * x, term = 1.0, sum = 1.0;
+  for (int n = 1; n < 200; ++n) {
+    term *= q / ((double)n * n);
+    sum += term;
+    if (term < 1e-18 * sum) break;
+  }
+  return sum;
+}
+
+// Fourier transform of the normalised KB window (Jackson et al. 1991).
+static double kb_ft(double xi, int W, double beta) {
+  double z = beta * beta - (M_PI * W * xi) * (M_PI * W * xi);
+  double v;
+  if (z > 0) { double s = std::sqrt(z); v = std::sinh(s) / s; }
+  else if (z < 0) { double s = std::sqrt(-z); v = std::sin(s) / s; }
+  else v = 1.0;
+  return W * v / bessel_i0(beta);
+}
+
+static bool fft_size_ok(int n) {
+  switch (n) {
+    case 32: case 64: case 96: case 128: case 192: case 256: case 320: case 384:
+    case 512: case 640: case 768: case 1024: return true;
+    default: return false;
+  }
+}
+
+static int free_plan(mrh_nufft_plan* p) {
+  if (!p) return MRH_OK;
+  void* bufs[] = {p->d_koff, p->d_em, p->d_ef, p->d_soff, p->d_dx, p->d_dy,
+                  p->d_tw0, p->d_tw1, p->d_maps, p->d_T};
+  for (void* b : bufs) if (b) (void)hipFree(b);
+  delete p;
+  return MRH_OK;
+}
+
+template <typename T>
+static int upload(T** dst, const T* src, size_t n) {
+  *dst = nullptr;
+  if (n == 0) n = 1;
+  MRH_HIP(hipMalloc((void**)dst, n * sizeof(T)));
+  if (src) MRH_HIP(hipMemcpy(*dst, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return MRH_OK;
+}
+
+static void twiddles(std::vector<float2>& tw, int n) {
+  tw.resize(n);
+  for (int k = 0; k < n; ++k) {
+    double a = -2.0 * M_PI * (double)k / (double)n;
+    tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  }
+}
+
+extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** out) {
+  MRH_REQUIRE(d && out, MRH_EINVAL, "plan_create: null argument");
+  *out = nullptr;
+  MRH_REQUIRE(d->ndim == 2, MRH_EUNSUP, "only ndim=2 is implemented (got %d)", d->ndim);
+  MRH_REQUIRE(d->S0 >= 2 && d->S1 >= 2, MRH_EINVAL, "image size must be >= 2");
+  MRH_REQUIRE(d->G0 >= d->S0 && d->G1 >= d->S1, MRH_EINVAL, "grid must be at least the image size");
+  MRH_REQUIRE(fft_size_ok(d->G0) && fft_size_ok(d->G1), MRH_EUNSUP,
+              "grid sizes %d x %d not in the compiled FFT set {32..1024}", d->G0, d->G1);
+  MRH_REQUIRE(d->W >= 2 && d->W <= MRH_WMAX, MRH_EUNSUP, "KB width %d outside [2, %d]", d->W, MRH_WMAX);
+  MRH_REQUIRE(d->C >= 1 && d->B >= 1 && d->like >= 1, MRH_EINVAL, "C, B, like must be >= 1");
+  MRH_REQUIRE(d->k_offsets && d->traj, MRH_EINVAL, "k_offsets and traj are required (host pointers)");
+  MRH_REQUIRE(d->maps, MRH_EINVAL, "maps (device pointer) is required");
+  int Tc = d->strip_cols > 0 ? d->strip_cols : (d->G1 % 16 == 0 ? 16 : 8);
+  MRH_REQUIRE(Tc == 8 || Tc == 16, MRH_EUNSUP, "strip width must be 8 or 16 (got %d)", Tc);
+  MRH_REQUIRE(d->G1 % Tc == 0, MRH_EINVAL, "G1=%d not a multiple of the strip width %d", d->G1, Tc);
+  MRH_REQUIRE(Tc >= d->W, MRH_EINVAL, "strip width %d must be >= KB width %d", Tc, d->W);
+  MRH_REQUIRE(d->G0 < 65536 && d->G1 < 65536, MRH_EINVAL, "grid too large");
+
+  mrh_nufft_plan* p = new mrh_nufft_plan();
+  std::memset(p, 0, sizeof(*p));
+  p->S0 = d->S0; p->S1 = d->S1; p->G0 = d->G0; p->G1 = d->G1; p->W = d->W;
+  p->C = d->C; p->B = d->B; p->like_max = d->like; p->Tc = Tc; p->nstrips = d->G1 / Tc;
+  (void)hipGetDevice(&p->device);
+
+  // KB parameters
+  double osf = (double)d->G0 / (double)d->S0;
+  double beta = d->beta > 0 ? d->beta
+                            : M_PI * std::sqrt(std::pow((d->W / osf) * (osf - 0.5), 2) - 0.8);
+  p->beta = beta;
+  double i0b = bessel_i0(beta), q = beta * beta / 4.0;
+  double coef[64];
+  {
+    double fact = 1.0;
+    for (int n = 0; n < 64; ++n) {
+      if (n > 0) fact *= n;
+      coef[n] = 1.0 / (fact * fact * i0b);
+    }
+  }
+  int np = 16;
+  while (np < MRH_KB_MAXNP && coef[np] * std::pow(q, np) > 1e-10) np += 8;
+  if (np > MRH_KB_MAXNP) np = MRH_KB_MAXNP;
+  MRH_REQUIRE(coef[np] * std::pow(q, np) < 1e-6, MRH_EUNSUP, "KB beta %.3f too large for the series", beta);
+  p->np = np;
+  p->kb.qscale = (float)q;
+  p->kb.two_over_w = (float)(2.0 / d->W);
+  for (int n = 0; n < MRH_KB_MAXNP; ++n) p->kb.a[n] = n < np ? (float)coef[n] : 0.f;
+
+  // frame offsets
+  const int B = d->B;
+  std::vector<int64_t> koff(d->k_offsets, d->k_offsets + B + 1);
+  for (int b = 0; b < B; ++b)
+    if (koff[b + 1] < koff[b]) { free_plan(p); mrh_set_error("k_offsets must be non-decreasing"); return MRH_EINVAL; }
+  p->Ktot = koff[B];
+  if (p->Ktot >= (int64_t)1 << 31) { free_plan(p); mrh_set_error("too many samples for 32-bit entry indices"); return MRH_EUNSUP; }
+
+  // footprints and strip binning (stable counting sort by (frame, strip))
+  const int W = d->W, G0 = d->G0, G1 = d->G1, ns = p->nstrips;
+  const double sc0 = G0 / (2.0 * M_PI), sc1 = G1 / (2.0 * M_PI);
+  std::vector<int> cnt((size_t)B * ns + 1, 0);
+  struct Foot { int mx, my, s0, s1; float ux, uy; };
+  std::vector<Foot> foot((size_t)p->Ktot);
+  for (int b = 0; b < B; ++b) {
+    const int64_t Kb = koff[b + 1] - koff[b];
+    const float* t0 = d->traj + 2 * koff[b];
+    const float* t1 = t0 + Kb;
+    for (int64_t k = 0; k < Kb; ++k) {
+      double u = (double)t0[k] * sc0, v = (double)t1[k] * sc1;
+      double fu = std::floor(u - 0.5 * W) + 1.0, fv = std::floor(v - 0.5 * W) + 1.0;
+      int64_t mx = ((int64_t)fu % G0 + G0) % G0, my = ((int64_t)fv % G1 + G1) % G1;
+      Foot f;
+      f.mx = (int)mx; f.my = (int)my;
+      f.ux = (float)(u - fu); f.uy = (float)(v - fv);
+      f.s0 = (int)(my / Tc);
+      f.s1 = (int)(((my + W - 1) % G1) / Tc);
+      foot[koff[b] + k] = f;
+      cnt[(size_t)b * ns + f.s0]++;
+      if (f.s1 != f.s0) cnt[(size_t)b * ns + f.s1]++;
+    }
+  }
+  std::vector<int> soff((size_t)B * ns + 1, 0);
+  int64_t acc = 0;
+  int maxe = 0;
+  for (size_t i = 0; i < (size_t)B * ns; ++i) {
+    soff[i] = (int)acc;
+    acc += cnt[i];
+    maxe = std::max(maxe, cnt[i]);
+  }
+  soff[(size_t)B * ns] = (int)acc;
+  p->n_entries = acc;
+  p->max_strip_entries = maxe;
+  std::vector<int2> em((size_t)acc);
+  std::vector<float4> ef((size_t)acc);
+  std::vector<int> cur(soff.begin(), soff.end() - 1);
+  for (int b = 0; b < B; ++b) {
+    const int64_t Kb = koff[b + 1] - koff[b];
+    for (int64_t k = 0; k < Kb; ++k) {
+      const Foot& f = foot[koff[b] + k];
+      float dval = 1.f;
+      if (d->dcf) dval = d->dcf[koff[b] + k];
+      float wA = 1.f, wAH = 1.f;
+      if (d->dcf && d->dcf_mode == MRH_DCF_BALANCED) {
+        wA = wAH = (float)std::sqrt(std::max((double)dval, 0.0));
+      } else if (d->dcf && d->dcf_mode == MRH_DCF_STANDARD) {
+        wA = d->dcf_in_fwd ? dval : 1.f;
+        wAH = d->dcf_in_adj ? dval : 1.f;
+      }
+      int2 m = make_int2(f.mx | (f.my << 16), (int)k);
+      float4 fv = make_float4(f.ux, f.uy, wA, wAH);
+      int e = cur[(size_t)b * ns + f.s0]++;
+      em[e] = m; ef[e] = fv;
+      if (f.s1 != f.s0) {
+        e = cur[(size_t)b * ns + f.s1]++;
+        em[e] = m; ef[e] = fv;
+      }
+    }
+  }
+
+  // deapodisation 1 / Phi((i - S/2) / G)
+  std::vector<float> dx(d->S0), dy(d->S1);
+  for (int i = 0; i < d->S0; ++i) dx[i] = (float)(1.0 / kb_ft((double)(i - d->S0 / 2) / G0, W, beta));
+  for (int i = 0; i < d->S1; ++i) dy[i] = (float)(1.0 / kb_ft((double)(i - d->S1 / 2) / G1, W, beta));
+  std::vector<float2> tw0, tw1;
+  twiddles(tw0, G0);
+  twiddles(tw1, G1);
+
+  int rc;
+#define UP(dst, src, n) if ((rc = upload(&(dst), (src), (n))) != MRH_OK) { free_plan(p); return rc; }
+  UP(p->d_koff, koff.data(), koff.size());
+  UP(p->d_em, em.data(), em.size());
+  UP(p->d_ef, ef.data(), ef.size());
+  UP(p->d_soff, soff.data(), soff.size());
+  UP(p->d_dx, dx.data(), dx.size());
+  UP(p->d_dy, dy.data(), dy.size());
+  UP(p->d_tw0, tw0.data(), tw0.size());
+  UP(p->d_tw1, tw1.data(), tw1.size());
+  size_t nmaps = (size_t)d->C * d->S0 * d->S1;
+  UP(p->d_maps, (const float2*)nullptr, nmaps);
+#undef UP
+  hipError_t e = hipMemcpy(p->d_maps, d->maps, nmaps * sizeof(float2), hipMemcpyDeviceToDevice);
+  if (e != hipSuccess) { free_plan(p); mrh_set_error("maps copy failed: %s", hipGetErrorString(e)); return MRH_EHIP; }
+  p->T_elems = (size_t)B * d->like * d->C * d->S0 * G1;
+  e = hipMalloc((void**)&p->d_T, p->T_elems * sizeof(float2));
+  if (e != hipSuccess) {
+    free_plan(p);
+    mrh_set_error("workspace allocation of %zu bytes failed: %s", p->T_elems * sizeof(float2), hipGetErrorString(e));
+    return MRH_ENOMEM;
+  }
+  *out = p;
+  return MRH_OK;
+}
+
+extern "C" int mrh_nufft_plan_destroy(mrh_nufft_plan* p) { return free_plan(p); }
+
+extern "C" int mrh_nufft_plan_info(const mrh_nufft_plan* p, mrh_nufft_info* info) {
+  MRH_REQUIRE(p && info, MRH_EINVAL, "plan_info: null argument");
+  info->G0 = p->G0; info->G1 = p->G1; info->W = p->W; info->strip_cols = p->Tc;
+  info->n_strips = p->nstrips; info->kb_terms = p->np; info->beta = p->beta;
+  info->n_samples = p->Ktot; info->n_entries = p->n_entries;
+  info->max_strip_entries = p->max_strip_entries;
+  info->workspace_bytes = (int64_t)(p->T_elems * sizeof(float2));
+  return MRH_OK;
+}
+
+// ---------------------------------------------------------------- kernels
+#define ROWS_PER_BLOCK 4
+
+// Forward row pass.  Grid (ceil(S0/4), nbatch*C), 256 threads: one wavefront per image row.
+template <int G1>
+__global__ __launch_bounds__(256) void k_fwd_rows(
+    const float2* __restrict__ x, const float2* __restrict__ maps, const float* __restrict__ dx,
+    const float* __restrict__ dy, const float2* __restrict__ tw1, float2* __restrict__ T,
+    float2* __restrict__ y, const int64_t* __restrict__ koff, int S0, int S1, int C, int like,
+    int Tc, int x_has_coils) {
+  constexpr int LP = fft_padded(G1);
+  __shared__ float2 sbuf[ROWS_PER_BLOCK * LP];
+  __shared__ float2 stw[G1];
+  const int tid = threadIdx.x, lane = tid & 63, wrow = tid >> 6;
+  const int cf = blockIdx.y, bb = cf / C, c = cf - bb * C;
+  const int ix = blockIdx.x * ROWS_PER_BLOCK + wrow;
+  for (int i = tid; i < G1; i += blockDim.x) stw[i] = tw1[i];
+  {  // clear this block's share of the coil-frame's k-space (the strip pass accumulates into it)
+    const int b = bb / like, l = bb - b * like;
+    const int64_t Kb = koff[b + 1] - koff[b];
+    const int64_t base = (int64_t)C * (koff[b] * like + (int64_t)l * Kb) + (int64_t)c * Kb;
+    const int64_t lo = Kb * blockIdx.x / gridDim.x, hi = Kb * (blockIdx.x + 1) / gridDim.x;
+    for (int64_t k = lo + tid; k < hi; k += blockDim.x) y[base + k] = make_float2(0.f, 0.f);
+  }
+  float2* buf = sbuf + wrow * LP;
+  const int h1 = S1 / 2;
+  if (ix < S0) {
+    const float2* xr = x + ((int64_t)(x_has_coils ? cf : bb) * S0 + ix) * S1;
+    const float2* mr = maps + ((int64_t)c * S0 + ix) * S1;
+    const float dxi = dx[ix];
+    for (int pp = lane; pp < G1; pp += 64) {
+      float2 v = make_float2(0.f, 0.f);
+      const bool lo = pp < S1 - h1, hi = pp >= G1 - h1;
+      if (lo || hi) {
+        const int iy = lo ? pp + h1 : pp - G1 + h1;
+        float2 xv = xr[iy];
+        if (!x_has_coils) xv = cmul(xv, mr[iy]);
+        v = cscale(xv, dxi * dy[iy]);
+      }
+      buf[PIDX(pp)] = v;
+    }
+  } else {
+    for (int pp = lane; pp < G1; pp += 64) buf[PIDX(pp)] = make_float2(0.f, 0.f);
+  }
+  __syncthreads();
+  fft_lds<G1, false, 64>(buf, stw, lane);
+  if (ix < S0) {
+    const int nstr = G1 / Tc;
+    float2* Tcf = T + (int64_t)cf * nstr * S0 * Tc;
+    for (int pp = lane; pp < G1; pp += 64) {
+      const int s = pp / Tc, j = pp - s * Tc;
+      Tcf[((int64_t)s * S0 + ix) * Tc + j] = buf[PIDX(pp)];
+    }
+  }
+}
+
+// Forward strip pass.  Grid (nstrips, nbatch*C), 64*Tc threads (one wavefront per column).
+template <int G0, int NP>
+__global__ __launch_bounds__(1024) void k_fwd_strips(
+    const float2* __restrict__ T, float2* __restrict__ y, const int2* __restrict__ em,
+    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Tc, int W) {
+  constexpr int CS = fft_padded(G0) + 1;
+  extern __shared__ float2 smem[];
+  float2* stw = smem + Tc * CS;
+  const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
+  const int s = blockIdx.x, cf = blockIdx.y, nstr = gridDim.x;
+  const int bb = cf / C, c = cf - bb * C, b = bb / like, l = bb - b * like;
+  for (int i = tid; i < G0; i += blockDim.x) stw[i] = tw0[i];
+  const float2* Tb = T + ((int64_t)cf * nstr + s) * S0 * Tc;
+  const int h0 = S0 / 2;
+  for (int e = tid; e < G0 * Tc; e += blockDim.x) {
+    const int r = e / Tc, j = e - r * Tc;
+    float2 v = make_float2(0.f, 0.f);
+    if (r < S0 - h0) v = Tb[(r + h0) * Tc + j];
+    else if (r >= G0 - h0) v = Tb[(r - G0 + h0) * Tc + j];
+    smem[j * CS + PIDX(r)] = v;
+  }
+  __syncthreads();
+  fft_lds<G0, false, 64>(smem + col * CS, stw, lane);
+
+  const int e0 = soff[b * nstr + s], e1 = soff[b * nstr + s + 1];
+  const int64_t Kb = koff[b + 1] - koff[b];
+  float2* yb = y + (int64_t)C * (koff[b] * like + (int64_t)l * Kb) + (int64_t)c * Kb;
+  const int c0 = s * Tc;
+  for (int e = e0 + tid; e < e1; e += blockDim.x) {
+    const int2 m = em[e];
+    const float4 f = ef[e];
+    const int mx = m.x & 0xffff, my = (int)((unsigned)m.x >> 16);
+    float wx[MRH_WMAX];
+#pragma unroll
+    for (int i = 0; i < MRH_WMAX; ++i) wx[i] = (i < W) ? kb_weight<NP>(kb, f.x - (float)i) : 0.f;
+    float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int j = 0; j < MRH_WMAX; ++j) {
+      if (j < W) {
+        int cc = my + j;
+        if (cc >= G1) cc -= G1;
+        const int lc = cc - c0;
+        if ((unsigned)lc < (unsigned)Tc) {
+          const float wy = kb_weight<NP>(kb, f.y - (float)j);
+          const float2* colp = smem + lc * CS;
+          float2 rs = make_float2(0.f, 0.f);
+#pragma unroll
+          for (int i = 0; i < MRH_WMAX; ++i) {
+            if (i < W) {
+              int rr = mx + i;
+              if (rr >= G0) rr -= G0;
+              const float2 g = colp[PIDX(rr)];
+              rs.x = fmaf(g.x, wx[i], rs.x);
+              rs.y = fmaf(g.y, wx[i], rs.y);
+            }
+          }
+          acc.x = fmaf(rs.x, wy, acc.x);
+          acc.y = fmaf(rs.y, wy, acc.y);
+        }
+      }
+    }
+    unsafeAtomicAdd(&yb[m.y].x, acc.x * f.z);
+    unsafeAtomicAdd(&yb[m.y].y, acc.y * f.z);
+  }
+}
+
+// Adjoint strip pass: spread into the LDS strip, inverse column FFT, crop rows.
+template <int G0, int NP>
+__global__ __launch_bounds__(1024) void k_adj_strips(
+    const float2* __restrict__ y, float2* __restrict__ T, const int2* __restrict__ em,
+    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Tc, int W) {
+  constexpr int CS = fft_padded(G0) + 1;
+  extern __shared__ float2 smem[];
+  float2* stw = smem + Tc * CS;
+  const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
+  const int s = blockIdx.x, cf = blockIdx.y, nstr = gridDim.x;
+  const int bb = cf / C, c = cf - bb * C, b = bb / like, l = bb - b * like;
+  for (int i = tid; i < G0; i += blockDim.x) stw[i] = tw0[i];
+  for (int i = tid; i < Tc * CS; i += blockDim.x) smem[i] = make_float2(0.f, 0.f);
+  __syncthreads();
+
+  const int e0 = soff[b * nstr + s], e1 = soff[b * nstr + s + 1];
+  const int64_t Kb = koff[b + 1] - koff[b];
+  const float2* yb = y + (int64_t)C * (koff[b] * like + (int64_t)l * Kb) + (int64_t)c * Kb;
+  const int c0 = s * Tc;
+  for (int e = e0 + tid; e < e1; e += blockDim.x) {
+    const int2 m = em[e];
+    const float4 f = ef[e];
+    const int mx = m.x & 0xffff, my = (int)((unsigned)m.x >> 16);
+    const float2 yv = cscale(yb[m.y], f.w);
+    float wx[MRH_WMAX];
+#pragma unroll
+    for (int i = 0; i < MRH_WMAX; ++i) wx[i] = (i < W) ? kb_weight<NP>(kb, f.x - (float)i) : 0.f;
+#pragma unroll
+    for (int j = 0; j < MRH_WMAX; ++j) {
+      if (j < W) {
+        int cc = my + j;
+        if (cc >= G1) cc -= G1;
+        const int lc = cc - c0;
+        if ((unsigned)lc < (unsigned)Tc) {
+          const float wy = kb_weight<NP>(kb, f.y - (float)j);
+          const float2 v = cscale(yv, wy);
+          float2* colp = smem + lc * CS;
+#pragma unroll
+          for (int i = 0; i < MRH_WMAX; ++i) {
+            if (i < W) {
+              int rr = mx + i;
+              if (rr >= G0) rr -= G0;
+              float2* dst = colp + PIDX(rr);
+              atomicAdd(&dst->x, v.x * wx[i]);
+              atomicAdd(&dst->y, v.y * wx[i]);
+            }
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  fft_lds<G0, true, 64>(smem + col * CS, stw, lane);
+  float2* Tb = T + ((int64_t)cf * nstr + s) * S0 * Tc;
+  const int h0 = S0 / 2;
+  for (int e = tid; e < S0 * Tc; e += blockDim.x) {
+    const int ix = e / Tc, j = e - ix * Tc;
+    const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
+    Tb[e] = smem[j * CS + PIDX(r)];
+  }
+}
+
+// Adjoint row pass: per image row, inverse row FFT of every coil, crop,
+// conj(map) combine in registers, deapodise.  Grid (ceil(S0/4), nbatch).
+template <int G1>
+__global__ __launch_bounds__(256) void k_adj_rows(
+    const float2* __restrict__ T, const float2* __restrict__ maps, const float* __restrict__ dx,
+    const float* __restrict__ dy, const float2* __restrict__ tw1, float2* __restrict__ x,
+    int S0, int S1, int C, int Tc) {
+  constexpr int LP = fft_padded(G1);
+  constexpr int NA = (G1 + 63) / 64;
+  __shared__ float2 sbuf[ROWS_PER_BLOCK * LP];
+  __shared__ float2 stw[G1];
+  const int tid = threadIdx.x, lane = tid & 63, wrow = tid >> 6;
+  const int bb = blockIdx.y;
+  const int ix = blockIdx.x * ROWS_PER_BLOCK + wrow;
+  for (int i = tid; i < G1; i += blockDim.x) stw[i] = tw1[i];
+  float2* buf = sbuf + wrow * LP;
+  const int nstr = G1 / Tc, h1 = S1 / 2;
+  float2 acc[NA];
+#pragma unroll
+  for (int q = 0; q < NA; ++q) acc[q] = make_float2(0.f, 0.f);
+  for (int c = 0; c < C; ++c) {
+    const int cf = bb * C + c;
+    if (ix < S0) {
+      const float2* Tcf = T + (int64_t)cf * nstr * S0 * Tc;
+      for (int pp = lane; pp < G1; pp += 64) {
+        const int s = pp / Tc, j = pp - s * Tc;
+        buf[PIDX(pp)] = Tcf[((int64_t)s * S0 + ix) * Tc + j];
+      }
+    } else {
+      for (int pp = lane; pp < G1; pp += 64) buf[PIDX(pp)] = make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft_lds<G1, true, 64>(buf, stw, lane);
+    if (ix < S0) {
+      const float2* mr = maps + ((int64_t)c * S0 + ix) * S1;
+#pragma unroll
+      for (int q = 0; q < NA; ++q) {
+        const int iy = lane + 64 * q;
+        if (iy < S1) {
+          const int pp = iy >= h1 ? iy - h1 : iy - h1 + G1;
+          const float2 g = buf[PIDX(pp)];
+          const float2 mv = mr[iy];
+          acc[q] = cadd(acc[q], cmulc(g, mv));
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (ix < S0) {
+    const float dxi = dx[ix];
+    float2* xr = x + ((int64_t)bb * S0 + ix) * S1;
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+      const int iy = lane + 64 * q;
+      if (iy < S1) xr[iy] = cscale(acc[q], dxi * dy[iy]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- dispatch
+template <int G>
+static int launch_rows_fwd(const mrh_nufft_plan* p, const float2* x, float2* y, int nb, int like,
+                           int x_has_coils, hipStream_t st) {
+  dim3 grid((p->S0 + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK, nb * p->C);
+  hipLaunchKernelGGL(k_fwd_rows<G>, grid, dim3(256), 0, st, x, p->d_maps, p->d_dx, p->d_dy,
+                     p->d_tw1, p->d_T, y, p->d_koff, p->S0, p->S1, p->C, like, p->Tc, x_has_coils);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+template <int G>
+static int launch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStream_t st) {
+  dim3 grid((p->S0 + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK, nb);
+  hipLaunchKernelGGL(k_adj_rows<G>, grid, dim3(256), 0, st, p->d_T, p->d_maps, p->d_dx, p->d_dy,
+                     p->d_tw1, x, p->S0, p->S1, p->C, p->Tc);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+template <int G, int NP>
+static int launch_strips(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
+                         int nb, int like, hipStream_t st) {
+  size_t lds = ((size_t)p->Tc * (fft_padded(G) + 1) + G) * sizeof(float2);
+  dim3 grid(p->nstrips, nb * p->C), block(64 * p->Tc);
+  if (adjoint) {
+    static bool attr = false;
+    if (!attr) { (void)hipFuncSetAttribute((const void*)k_adj_strips<G, NP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); attr = true; }
+    hipLaunchKernelGGL((k_adj_strips<G, NP>), grid, block, lds, st, yin, p->d_T, p->d_em, p->d_ef,
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->Tc, p->W);
+  } else {
+    static bool attr = false;
+    if (!attr) { (void)hipFuncSetAttribute((const void*)k_fwd_strips<G, NP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); attr = true; }
+    hipLaunchKernelGGL((k_fwd_strips<G, NP>), grid, block, lds, st, p->d_T, yout, p->d_em, p->d_ef,
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->Tc, p->W);
+  }
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+#define MRH_FOR_EACH_G(X) X(32) X(64) X(96) X(128) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024)
+
+static int dispatch_rows_fwd(const mrh_nufft_plan* p, const float2* x, float2* y, int nb, int like,
+                             int xc, hipStream_t st) {
+  switch (p->G1) {
+#define X(G) case G: return launch_rows_fwd<G>(p, x, y, nb, like, xc, st);
+    MRH_FOR_EACH_G(X)
+#undef X
+  }
+  mrh_set_error("unsupported G1=%d", p->G1);
+  return MRH_EUNSUP;
+}
+
+static int dispatch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStream_t st) {
+  switch (p->G1) {
+#define X(G) case G: return launch_rows_adj<G>(p, x, nb, st);
+    MRH_FOR_EACH_G(X)
+#undef X
+  }
+  mrh_set_error("unsupported G1=%d", p->G1);
+  return MRH_EUNSUP;
+}
+
+template <int NP>
+static int dispatch_strips_np(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
+                              int nb, int like, hipStream_t st) {
+  switch (p->G0) {
+#define X(G) case G: return launch_strips<G, NP>(p, adj, yin, yout, nb, like, st);
+    MRH_FOR_EACH_G(X)
+#undef X
+  }
+  mrh_set_error("unsupported G0=%d", p->G0);
+  return MRH_EUNSUP;
+}
+
+static int dispatch_strips(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
+                           int nb, int like, hipStream_t st) {
+  switch (p->np) {
+    case 16: return dispatch_strips_np<16>(p, adj, yin, yout, nb, like, st);
+    case 24: return dispatch_strips_np<24>(p, adj, yin, yout, nb, like, st);
+    default: return dispatch_strips_np<32>(p, adj, yin, yout, nb, like, st);
+  }
+}
+
+extern "C" int mrh_nufft_forward(mrh_nufft_plan* p, const void* x, void* y, int like,
+                                 int x_has_coils, void* stream) {
+  MRH_REQUIRE(p && x && y, MRH_EINVAL, "nufft_forward: null argument");
+  MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = p->B * like;
+  int rc = dispatch_rows_fwd(p, (const float2*)x, (float2*)y, nb, like, x_has_coils ? 1 : 0, st);
+  if (rc) return rc;
+  return dispatch_strips(p, false, nullptr, (float2*)y, nb, like, st);
+}
+
+extern "C" int mrh_nufft_adjoint(mrh_nufft_plan* p, const void* y, void* x, int like, void* stream) {
+  MRH_REQUIRE(p && x && y, MRH_EINVAL, "nufft_adjoint: null argument");
+  MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = p->B * like;
+  int rc = dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, st);
+  if (rc) return rc;
+  return dispatch_rows_adj(p, (float2*)x, nb, st);
+}
+
+extern "C" int mrh_nufft_forward_stage(mrh_nufft_plan* p, int stage, const void* x, void* y, int like,
+                                       int x_has_coils, void* stream) {
+  MRH_REQUIRE(p && x && y, MRH_EINVAL, "nufft_forward_stage: null argument");
+  MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
+  MRH_REQUIRE(stage == 0 || stage == 1, MRH_EINVAL, "stage must be 0 (rows) or 1 (strips)");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = p->B * like;
+  if (stage == 0) return dispatch_rows_fwd(p, (const float2*)x, (float2*)y, nb, like, x_has_coils ? 1 : 0, st);
+  return dispatch_strips(p, false, nullptr, (float2*)y, nb, like, st);
+}
+
+extern "C" int mrh_nufft_adjoint_stage(mrh_nufft_plan* p, int stage, const void* y, void* x, int like,
+                                       void* stream) {
+  MRH_REQUIRE(p && x && y, MRH_EINVAL, "nufft_adjoint_stage: null argument");
+  MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
+  MRH_REQUIRE(stage == 0 || stage == 1, MRH_EINVAL, "stage must be 0 (strips) or 1 (rows)");
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = p->B * like;
+  if (stage == 0) return dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, st);
+  return dispatch_rows_adj(p, (float2*)x, nb, st);
+}

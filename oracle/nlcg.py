@@ -1,0 +1,256 @@
+"""ORACLE (test infrastructure only) -- CPU restatement of the reference's NLCG.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may use
+this module.  It follows the reference solver stack line by line, quirks
+included, on any operator exposing ``forward`` / ``adjoint`` over
+(B, S0, S1) images and flat k-space vectors:
+
+* Huber TV stencils: graspcg/ops/ndops.py:28-59 (forward difference with the
+  last slice zeroed, backward-difference divergence with the first slice kept),
+  isotropic / anisotropic energy and flux ndops.py:66-135, accumulation
+  ``g += -lambda * div`` and ``E = sum(lambda * e)`` at
+  graspcg/regularization/tv_nd.py:54-88;
+* objective: residual ``r = t*Ad + Ax - y`` in that order
+  (graspcg/ops/objective.py:31-45), f_data = 0.5 ||r||^2 (:229), grad = A^H r
+  (:225), regulariser evaluated at x + t d with x mutated and restored
+  (:256-266), slope Re<g, d> (:268-277);
+* Armijo backtracking from t = 1 with factor 0.5, c1 = 1e-4, at most 20 trials,
+  rejecting a non-descent / non-finite slope (graspcg/numerics/line_search.py:24-79);
+* CG loop: prime d = -g (graspcg/solvers/cg.py:178-190), accept x += t d
+  (:205-207), stats norms (:213-220), the reference convergence test
+  (graspcg/solvers/utils.py:40-43), direction update FR with the first beta = 1
+  quirk (graspcg/numerics/directions.py:141-162) or PR+ (:75-109), D = 1
+  (cg.py:95).
+
+Pinning (tests/test_oracle_nlcg.py): closed-form quadratic with a unitary
+operator (graspcg/tests/test_numerics_ns.py:110-194), rejection of an ascent
+direction (:229-248), the Armijo inequality and monotone history
+(graspcg/tests/test_cg_descend_compile_ns.py:75-123), the Richardson
+finite-difference slope (:180-229), TV of a constant, the fwd/bwd adjoint
+identity and the hand-computable step-edge energy.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Sequence
+
+import numpy as np
+
+
+# --------------------------------------------------------------- TV (ndops.py)
+
+def fwd_diff(x: np.ndarray, ax: int) -> np.ndarray:
+    out = np.roll(x, -1, axis=ax) - x
+    idx = [slice(None)] * x.ndim
+    idx[ax] = -1
+    out[tuple(idx)] = 0
+    return out
+
+
+def bwd_diff(p: np.ndarray, ax: int) -> np.ndarray:
+    out = p - np.roll(p, 1, axis=ax)
+    idx = [slice(None)] * p.ndim
+    idx[ax] = 0
+    out[tuple(idx)] = p[tuple(idx)]
+    return out
+
+
+@dataclass
+class TV:
+    """One Huber-TV term: weight lambda, knee eps, axes of the (B, S0, S1) image."""
+    weight: float
+    axes: Sequence[int]
+    eps: float = 1e-3
+    isotropic: bool = True
+    axis_weights: Optional[Sequence[float]] = None
+
+
+def _abs2(z):
+    return z.real * z.real + z.imag * z.imag
+
+
+def tv_energy_grad(x: np.ndarray, reg: TV, g: Optional[np.ndarray] = None):
+    """Energy of one TV term at x; when ``g`` is given, g += -lambda * div(flux).
+
+    Slices are independent problems only when the caller passes one slice;
+    this routine differences across the whole array like the reference."""
+    rdt = x.real.dtype
+    if reg.weight == 0.0 or len(reg.axes) == 0:
+        return rdt.type(0.0)
+    ws = reg.axis_weights or [1.0] * len(reg.axes)
+    grads = [rdt.type(w) * fwd_diff(x, ax) for w, ax in zip(ws, reg.axes)]
+    eps = rdt.type(reg.eps)
+    lam = rdt.type(reg.weight)
+    if reg.isotropic:
+        ss = sum(_abs2(gi) for gi in grads)
+        den = np.sqrt(ss + eps * eps)
+        e = den - eps
+        dcl = np.maximum(den, rdt.type(1e-12))
+        flux = [gi / dcl for gi in grads]
+    else:
+        e = 0
+        flux = []
+        for gi in grads:
+            den = np.sqrt(_abs2(gi) + eps * eps)
+            e = e + (den - eps)
+            flux.append(gi / np.maximum(den, rdt.type(1e-12)))
+    if g is not None:
+        div = sum(bwd_diff(pi, ax) for pi, ax in zip(flux, reg.axes))
+        g += (-div) * lam
+    return (e * lam).sum(dtype=rdt)
+
+
+# --------------------------------------------------------------- objective
+
+class Objective:
+    """phi(t) = 0.5 ||A(x + t d) - y||^2 + sum_r TV_r(x + t d), reference evaluation order."""
+
+    def __init__(self, A: Callable, AH: Callable, y: np.ndarray, regs: Sequence[TV],
+                 mutate_x: bool = True):
+        self.A, self.AH, self.y, self.regs = A, AH, y, list(regs)
+        self.mutate_x = mutate_x
+        self.Ax = self.Ad = None
+        self.n_A = self.n_AH = 0
+
+    def begin_linesearch(self, x, d):
+        self.Ax = self.A(x)
+        self.Ad = self.A(d)
+        self.n_A += 2
+
+    def f_g(self, x: np.ndarray, d: np.ndarray, t: float):
+        rdt = x.real.dtype
+        tt = rdt.type(t)
+        r = self.Ad * tt
+        r = r + self.Ax
+        r = r - self.y
+        f_data = rdt.type(0.5) * (_abs2(r).sum(dtype=rdt))
+        g = self.AH(r).astype(x.dtype)
+        self.n_AH += 1
+        if self.mutate_x:
+            x += d * tt
+            xe = x
+        else:
+            xe = x + d * tt
+        f_reg = rdt.type(0.0)
+        for reg in self.regs:
+            f_reg = f_reg + tv_energy_grad(xe, reg, g)
+        if self.mutate_x:
+            x += d * (-tt)
+        gdot = rdt.type(np.vdot(g.reshape(-1), d.reshape(-1)).real)
+        return rdt.type(f_data + f_reg), gdot, g
+
+
+# --------------------------------------------------------------- line search
+
+def armijo(obj: "Objective", x: np.ndarray, d: np.ndarray, c1: float = 1e-4, ls_beta: float = 0.5,
+           ls_max_iter: int = 20):
+    """Backtracking Armijo from t = 1 (graspcg/numerics/line_search.py:24-79).
+
+    Returns (ok, t, f0, g0d, f_t, gdot_t, g) with g the gradient at the last
+    evaluated point; a non-descent or non-finite slope returns ok=False, t=0."""
+    rdt = x.real.dtype
+    obj.begin_linesearch(x, d)
+    f0, g0d, g = obj.f_g(x, d, 0.0)
+    if not (g0d < 0) or not np.isfinite(g0d):
+        return False, rdt.type(0.0), f0, g0d, f0, g0d, g
+    t = rdt.type(1.0)
+    ft, gdt = f0, g0d
+    for _ in range(ls_max_iter):
+        ft, gdt, g = obj.f_g(x, d, t)
+        if ft <= f0 + rdt.type(c1) * t * g0d:
+            return True, t, f0, g0d, ft, gdt, g
+        t = rdt.type(t * ls_beta)
+    return False, t, f0, g0d, ft, gdt, g
+
+
+# --------------------------------------------------------------- solver
+
+@dataclass
+class CGResult:
+    x: np.ndarray
+    history: List[dict] = field(default_factory=list)
+    f_final: float = float("nan")
+    n_iter: int = 0
+    stop: str = ""
+    n_A: int = 0
+    n_AH: int = 0
+
+
+def _norm(v, rdt):
+    return rdt.type(math.sqrt(max(float(np.vdot(v.reshape(-1), v.reshape(-1)).real), 0.0)))
+
+
+def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequence[TV],
+         max_iter: int = 50, direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5,
+         ls_max_iter: int = 20, tol_g: float = 1e-4, tol_step: float = 1e-9,
+         rel_tol: Optional[float] = None, mutate_x: bool = True) -> CGResult:
+    """Reference-semantics NLCG (graspcg/solvers/cg.py:162-246) with Armijo search.
+
+    ``rel_tol`` adds the stop rule |f_k - f_{k-1}| / |f_{k-1}| < rel_tol on the
+    accepted objective values (not a reference rule; used for time-to-tolerance)."""
+    x = np.array(x0, copy=True)
+    rdt = x.real.dtype
+    obj = Objective(A, AH, y, regs, mutate_x=mutate_x)
+    d = np.zeros_like(x)
+    res = CGResult(x=x)
+    # prime (cg.py:178-190)
+    obj.begin_linesearch(x, d)
+    _, _, g = obj.f_g(x, d, 0.0)
+    d = -g
+    rho_prev = None
+    g_prev = g.copy()
+    f_prev = None
+    for k in range(max_iter):
+        ok, t, f0, g0d, ft, gdt, g_new = armijo(obj, x, d, c1=c1, ls_beta=ls_beta, ls_max_iter=ls_max_iter)
+        if g_new is not None:
+            g = g_new
+        if not ok:
+            res.stop = "not_descent" if t == 0 and not (g0d < 0 and np.isfinite(g0d)) else "ls_failed"
+            break
+        x += d * t
+        xn, sn = _norm(x, rdt), _norm(d, rdt)
+        gn = _norm(g, rdt)
+        res.history.append(dict(iter=k, f=float(f0), gdot=float(g0d), gdot_at_step=float(gdt),
+                                step=float(t), f_step=float(ft), xnorm=float(xn),
+                                gnorm=float(gn), stepnorm=float(sn)))
+        res.n_iter = k + 1
+        if gn <= (1 + xn) * tol_g and sn <= (1 + xn) * tol_step:
+            res.stop = "converged"
+            break
+        if rel_tol is not None and f_prev is not None and abs(ft - f_prev) <= rel_tol * abs(f_prev):
+            f_prev = ft
+            res.stop = "rel_tol"
+            break
+        f_prev = ft
+        # direction update (directions.py)
+        rho = rdt.type(np.vdot(g.reshape(-1), g.reshape(-1)).real)
+        if direction == "fr":
+            if rho_prev is None:
+                rho_prev = rho
+            beta = rho / max(rho_prev, rdt.type(1e-20))
+            rho_prev = rho
+        elif direction == "prplus":
+            num = rho - rdt.type(np.vdot(g_prev.reshape(-1), g.reshape(-1)).real)
+            den = rdt.type(np.vdot(g_prev.reshape(-1), g_prev.reshape(-1)).real)
+            beta = max(num / max(den, rdt.type(1e-20)), rdt.type(0.0))
+            g_prev = g.copy()
+        else:
+            raise ValueError(direction)
+        d = d * rdt.type(beta) - g
+    if not res.stop:
+        res.stop = "max_iter"
+    res.x = x
+    res.f_final = float(objective_value(A, y, x, regs))
+    res.n_A, res.n_AH = obj.n_A, obj.n_AH
+    return res
+
+
+def objective_value(A: Callable, y: np.ndarray, x: np.ndarray, regs: Sequence[TV]) -> float:
+    """f(x) evaluated in float64 from the given iterate."""
+    r = np.asarray(A(x), dtype=np.complex128) - np.asarray(y, dtype=np.complex128)
+    f = 0.5 * float(np.sum(_abs2(r)))
+    x64 = np.asarray(x, dtype=np.complex128)
+    for reg in regs:
+        f += float(tv_energy_grad(x64, reg))
+    return f

@@ -1,0 +1,86 @@
+"""The C-ABI library loads, exports every entry point include/mrirecon_hip.h declares,
+and its ctypes mirror has the same struct layouts as the C compiler (no GPU needed)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from mrirecon_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "mrirecon_hip.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(mrh_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    names = declared_functions()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert sorted(names) == _lib.exported_symbols()
+
+
+def test_abi_version_and_workspace():
+    lib = _lib.load()
+    assert lib.mrh_abi_version() == _lib.ABI_VERSION
+    assert lib.mrh_reduce_workspace_len() >= 1024
+
+
+def test_errors_are_reported_without_a_gpu():
+    lib = _lib.load()
+    rc = lib.mrh_nufft_plan_create(None, None)
+    assert rc == 1 and b"null" in lib.mrh_last_error()
+    with pytest.raises(ValueError):
+        _lib.check(rc)
+    d = _lib.NufftDesc()
+    d.ndim = 3
+    h = ctypes.c_void_p()
+    rc = lib.mrh_nufft_plan_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == 4
+    with pytest.raises(NotImplementedError):
+        _lib.check(rc)
+
+
+C_PROBE = r"""
+#include <stddef.h>
+#include <stdio.h>
+#include "mrirecon_hip.h"
+#define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+int main(void) {
+  printf("mrh_nufft_desc %zu\nmrh_nufft_info %zu\nmrh_tv_term %zu\nmrh_tv_desc %zu\nmrh_cg_desc %zu\nmrh_cg_hist %zu\n",
+         sizeof(mrh_nufft_desc), sizeof(mrh_nufft_info), sizeof(mrh_tv_term), sizeof(mrh_tv_desc),
+         sizeof(mrh_cg_desc), sizeof(mrh_cg_hist));
+  P(mrh_nufft_desc, beta) P(mrh_nufft_desc, k_offsets) P(mrh_nufft_desc, maps) P(mrh_nufft_desc, strip_cols)
+  P(mrh_nufft_info, beta) P(mrh_nufft_info, workspace_bytes)
+  P(mrh_tv_desc, term) P(mrh_cg_desc, k_prob) P(mrh_cg_desc, c1) P(mrh_cg_desc, max_hist)
+  return 0;
+}
+"""
+
+
+def test_struct_layouts_match_c(tmp_path):
+    src = tmp_path / "probe.c"
+    src.write_text(C_PROBE)
+    exe = tmp_path / "probe"
+    subprocess.run(["gcc", "-std=c11", f"-I{os.path.join(ROOT, 'include')}", str(src), "-o", str(exe)],
+                   check=True)
+    out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                             check=True).stdout.split("\n") if line)
+    sizes = {"mrh_nufft_desc": _lib.NufftDesc, "mrh_nufft_info": _lib.NufftInfo, "mrh_tv_term": _lib.TVTerm,
+             "mrh_tv_desc": _lib.TVDesc, "mrh_cg_desc": _lib.CGDesc, "mrh_cg_hist": _lib.CGHist}
+    for cname, cls in sizes.items():
+        assert int(out[cname]) == ctypes.sizeof(cls), cname
+    offs = {"mrh_nufft_desc": _lib.NufftDesc, "mrh_nufft_info": _lib.NufftInfo, "mrh_tv_desc": _lib.TVDesc,
+            "mrh_cg_desc": _lib.CGDesc}
+    for key, val in out.items():
+        if "." in key:
+            cname, field = key.split(".")
+            assert getattr(offs[cname], field).offset == int(val), key

@@ -1,0 +1,240 @@
+"""Parity of the HIP NUFFT (C ABI via the 'hip' backend) with the FP64 oracle.
+
+Bar: relative L2 <= 1e-5 against the FP64 KB-gridding oracle of the same
+operator (complex64 results); the error against the FP64 NUDFT (the
+reference's own ground truth, graspcg/tests/test_nufft_precision_gt.py) is
+printed and held to the reference's 2e-2.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from mrirecon_amd import NUFFT, AxisSpec, NUFFTConfig, synth
+from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+from oracle.kbnufft import KBNufft, nudft_adjoint, nudft_forward, rel_l2
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+TOL = 1e-5
+
+
+def c64(a):
+    return torch.from_numpy(np.ascontiguousarray(a).astype(np.complex64)).to(DEV)
+
+
+def crand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def adapter_for(wl: synth.Workload, strip_cols=0) -> HipNUFFTAdapter:
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", dcf_mode="balanced", kb_width=wl.W, osf=wl.osf,
+                         strip_cols=strip_cols)
+    be.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(wl.traj), torch.from_numpy(wl.dcf), c64(wl.maps),
+               device=DEV, k_offsets=wl.k_offsets)
+    return be
+
+
+def oracle_for(wl: synth.Workload, frames=None) -> KBNufft:
+    off = wl.k_offsets
+    if frames is None:
+        return KBNufft(wl.maps, wl.traj, off, wl.dcf, wl.W, wl.osf)
+    idx = np.concatenate([np.arange(off[b], off[b + 1]) for b in frames])
+    sub_off = np.concatenate([[0], np.cumsum([off[b + 1] - off[b] for b in frames])])
+    return KBNufft(wl.maps, wl.traj[:, idx], sub_off, wl.dcf[idx], wl.W, wl.osf)
+
+
+def test_reference_precision_problem():
+    """graspcg/tests/test_nufft_precision_gt.py:123-183 through the NUFFT front-end."""
+    rng = np.random.default_rng(0)
+    B, L, C, H, W, K = 2, 2, 3, 32, 32, 1024
+    maps = ((rng.standard_normal((C, H, W)) + 1j * rng.standard_normal((C, H, W))) / math.sqrt(C))
+    traj = rng.uniform(-math.pi, math.pi, (B, 2, K)).astype(np.float32)
+    dcf = (rng.uniform(0, 1, (B, K)) + 0.1).astype(np.float32)
+    x = crand((B, L, 1, H, W), 1)
+    axis = AxisSpec(image_labels=("B", "L", "coil", "X", "Y"), kspace_labels=("B", "L", "coil", "K"),
+                    image_fft_labels=("X", "Y"), coil_label="coil")
+    op = NUFFT(c64(maps), torch.from_numpy(traj).to(DEV), torch.from_numpy(dcf).to(DEV), axis=axis,
+               config=NUFFTConfig(ndim=2, traj_units="rad", dcf_mode="balanced"), device=DEV)
+    y = torch.empty((B, L, C, K), dtype=torch.complex64, device=DEV)
+    op.A(c64(x), out=y)
+    xh = torch.empty((B, L, 1, H, W), dtype=torch.complex64, device=DEV)
+    op.AH(y, out=xh)
+    torch.cuda.synchronize()
+    y_np, xh_np = y.cpu().numpy(), xh.cpu().numpy()
+    maps64 = maps.astype(np.complex64).astype(np.complex128)
+    ygt = np.stack([nudft_forward(maps64, x[b, :, 0], traj[b], dcf[b]) for b in range(B)])
+    xgt = np.stack([nudft_adjoint(maps64, y_np[b], traj[b], dcf[b]) for b in range(B)])
+    relA, relAH = rel_l2(y_np, ygt), rel_l2(xh_np[:, :, 0], xgt)
+    ora = KBNufft(maps64, traj.transpose(1, 0, 2).reshape(2, -1), np.arange(B + 1) * K, dcf.reshape(-1), 6,
+                  2.0, like=L)
+    relA_o = rel_l2(y_np, ora.forward(x.reshape(B * L, H, W)))
+    relAH_o = rel_l2(xh_np.reshape(B * L, H, W), ora.adjoint(y_np.reshape(-1)))
+    print(f"[hip] vs NUDFT relA={relA:.3e} relAH={relAH:.3e}; vs FP64 KB relA={relA_o:.3e} relAH={relAH_o:.3e}")
+    assert relA <= 2e-2 and relAH <= 2e-2
+    assert relA_o <= TOL and relAH_o <= TOL
+
+
+@pytest.mark.parametrize("strip_cols", [8, 16])
+def test_config1_forward_adjoint_parity(strip_cols):
+    wl = synth.config_c1()
+    be = adapter_for(wl, strip_cols)
+    ora = oracle_for(wl)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(wl.x_true), y)
+    yr = crand(be.plan.n_k, 3)
+    xa = torch.empty((1, wl.S, wl.S), dtype=torch.complex64, device=DEV)
+    be.plan.adjoint(c64(yr), xa)
+    torch.cuda.synchronize()
+    e_f = rel_l2(y.cpu().numpy(), ora.forward(wl.x_true))
+    ref_a = ora.adjoint(yr)
+    e_a = rel_l2(xa.cpu().numpy(), ref_a)
+    print(f"[c1 Tc={strip_cols}] forward rel={e_f:.3e} adjoint rel={e_a:.3e} "
+          f"maxabs={np.abs(xa.cpu().numpy() - ref_a).max():.3e}")
+    assert e_f <= TOL and e_a <= TOL
+
+
+def test_config3_precision_study():
+    """384^2, 16 coils, 402 spokes x 768: FP32 kernels vs the FP64 operator (rel L2, max-abs)."""
+    wl = synth.config_c3()
+    be = adapter_for(wl)
+    ora = oracle_for(wl)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(wl.x_true), y)
+    yr = crand(be.plan.n_k, 303)
+    xa = torch.empty((1, wl.S, wl.S), dtype=torch.complex64, device=DEV)
+    be.plan.adjoint(c64(yr), xa)
+    torch.cuda.synchronize()
+    yf = ora.forward(wl.x_true)
+    xr = ora.adjoint(yr)
+    e_f, e_a = rel_l2(y.cpu().numpy(), yf), rel_l2(xa.cpu().numpy(), xr)
+    m_f = np.abs(y.cpu().numpy() - yf).max() / np.abs(yf).max()
+    m_a = np.abs(xa.cpu().numpy() - xr).max() / np.abs(xr).max()
+    print(f"[c3] forward rel={e_f:.3e} max={m_f:.3e}; adjoint rel={e_a:.3e} max={m_a:.3e}")
+    assert e_f <= TOL and e_a <= TOL
+
+
+def test_config2_full_size_frames_and_adjointness():
+    """GRASP shape (34 frames x 12 coils, 256^2, W=6): first/last frame vs the oracle,
+    and <A x, y> = <x, A^H y> over the whole problem."""
+    wl = synth.config_c2()
+    be = adapter_for(wl)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    x = c64(wl.x_true)
+    be.plan.forward(x, y)
+    yr = c64(crand(be.plan.n_k, 7))
+    xa = torch.empty_like(x)
+    be.plan.adjoint(yr, xa)
+    torch.cuda.synchronize()
+    lhs = torch.vdot(y.reshape(-1).to(torch.complex128), yr.reshape(-1).to(torch.complex128)).item()
+    rhs = torch.vdot(x.reshape(-1).to(torch.complex128), xa.reshape(-1).to(torch.complex128)).item()
+    print(f"[c2] adjointness rel={abs(lhs - rhs) / abs(lhs):.3e}")
+    assert abs(lhs - rhs) <= 1e-5 * abs(lhs)
+    C, off = wl.n_coils, wl.k_offsets
+    for b in (0, wl.B - 1):
+        ora = oracle_for(wl, [b])
+        yb = y[C * off[b]:C * off[b + 1]].cpu().numpy()
+        e_f = rel_l2(yb, ora.forward(wl.x_true[b:b + 1]))
+        e_a = rel_l2(xa[b].cpu().numpy(), ora.adjoint(yr[C * off[b]:C * off[b + 1]].cpu().numpy())[0])
+        print(f"[c2 frame {b}] forward rel={e_f:.3e} adjoint rel={e_a:.3e}")
+        assert e_f <= TOL and e_a <= TOL
+
+
+def test_ragged_frames_parity():
+    """Uneven spokes per frame (the batched workload's layout) at a small size."""
+    S, C, R = 64, 3, 128
+    spokes = [13, 55, 21, 34, 17]
+    traj, dcf, off = synth.radial_frames(spokes, R)
+    maps = synth.gaussian_maps(C, S)
+    wl = synth.Workload("ragged", S, 1, len(spokes), C, R, 6, 2.0, spokes, maps, traj, dcf, off,
+                        crand((len(spokes), S, S), 9))
+    be = adapter_for(wl)
+    ora = oracle_for(wl)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(wl.x_true), y)
+    xa = torch.empty((len(spokes), S, S), dtype=torch.complex64, device=DEV)
+    yr = crand(be.plan.n_k, 10)
+    be.plan.adjoint(c64(yr), xa)
+    torch.cuda.synchronize()
+    assert rel_l2(y.cpu().numpy(), ora.forward(wl.x_true)) <= TOL
+    assert rel_l2(xa.cpu().numpy(), ora.adjoint(yr)) <= TOL
+
+
+def test_edge_of_kspace_and_empty_frame():
+    """Samples at w = -pi (footprint wrapping the grid edge) and a frame with no samples."""
+    S, C = 32, 2
+    rng = np.random.default_rng(11)
+    tr0 = np.stack([np.full(64, -math.pi), rng.uniform(-math.pi, math.pi, 64)]).astype(np.float32)
+    tr2 = rng.uniform(-math.pi, math.pi, (2, 100)).astype(np.float32)
+    traj = np.concatenate([tr0, tr2], axis=1)
+    off = np.array([0, 64, 64, 164])
+    dcf = rng.uniform(0.1, 1, 164).astype(np.float32)
+    maps = crand((C, S, S), 12)
+    wl = synth.Workload("edge", S, 1, 3, C, 0, 6, 2.0, [], maps, traj, dcf, off, crand((3, S, S), 13))
+    be = adapter_for(wl)
+    ora = oracle_for(wl)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(wl.x_true), y)
+    xa = torch.empty((3, S, S), dtype=torch.complex64, device=DEV)
+    yr = crand(be.plan.n_k, 14)
+    be.plan.adjoint(c64(yr), xa)
+    torch.cuda.synchronize()
+    assert rel_l2(y.cpu().numpy(), ora.forward(wl.x_true)) <= TOL
+    xa_np = xa.cpu().numpy()
+    assert rel_l2(xa_np, ora.adjoint(yr)) <= TOL
+    assert np.abs(xa_np[1]).max() == 0.0
+
+
+def test_coil_images_input_and_layouts():
+    """Cin == C (coil images, torchkb.py:216-228) equals maps applied inside; permuted
+    user k-space layouts write through non-contiguous out views."""
+    wl = synth.config_c1()
+    be = adapter_for(wl)
+    K = wl.uniform_K
+    x = c64(wl.x_true)[None]                                  # (1, 1, S, S)
+    y1 = be.A(x)
+    xc = (x * be._maps[None]).contiguous()                    # (1, C, S, S)
+    y2 = be.A(xc)
+    torch.cuda.synchronize()
+    assert rel_l2(y2.cpu().numpy(), y1.cpu().numpy()) <= 1e-6
+    axis = AxisSpec(image=("B", "coil", "X", "Y"), kspace=("B", "K", "coil"), image_fft=("X", "Y"))
+    op = NUFFT(c64(wl.maps), torch.from_numpy(wl.traj.reshape(2, 1, K).transpose(1, 0, 2).copy()).to(DEV),
+               torch.from_numpy(wl.dcf.reshape(1, K)).to(DEV), axis=axis,
+               config=NUFFTConfig(ndim=2, traj_units="rad", kb_width=wl.W), device=DEV)
+    yu = op.A(x)                                              # (1, K, C) user order
+    assert tuple(yu.shape) == (1, K, wl.n_coils)
+    assert torch.equal(yu.permute(0, 2, 1).contiguous(), y1)
+    xb = op.AH(yu)
+    xr = be.AH(y1)
+    torch.cuda.synchronize()
+    assert rel_l2(xb.cpu().numpy(), xr.cpu().numpy()) <= 1e-6
+
+
+def test_forward_is_deterministic_and_shapes_are_checked():
+    wl = synth.config_c1()
+    be = adapter_for(wl)
+    x = c64(wl.x_true)[None]
+    a, b = be.A(x), be.A(x)
+    torch.cuda.synchronize()
+    assert torch.equal(a, b)
+    with pytest.raises(ValueError):
+        be.A(torch.zeros((1, 1, wl.S, wl.S + 1), dtype=torch.complex64, device=DEV))
+    with pytest.raises(ValueError):
+        be.AH(torch.zeros((1, wl.n_coils + 1, wl.uniform_K), dtype=torch.complex64, device=DEV))
+    with pytest.raises(ValueError):
+        be.A(x, out=torch.empty((1, wl.n_coils, 7), dtype=torch.complex64, device=DEV))
+
+
+def test_diag_AHA_scalar_known_answer():
+    """alpha = A^H A(delta at the centre of coil image 0)(ctr) / SoS(ctr) (torchkb.py:342-364);
+    the NUDFT gives exactly Re(conj(S_0(ctr))) * sum_k dcf_k for balanced weighting."""
+    wl = synth.config_c1()
+    be = adapter_for(wl)
+    alpha = be.diag_AHA_scalar()
+    ctr = (wl.S // 2, wl.S // 2)
+    sos = float((np.abs(wl.maps[:, ctr[0], ctr[1]]) ** 2).sum())
+    expect = float(np.conj(wl.maps[0, ctr[0], ctr[1]]).real) * float(wl.dcf.astype(np.float64).sum()) / sos
+    print(f"[c1] diag_AHA_scalar = {alpha:.5f} (NUDFT {expect:.5f})")
+    assert abs(alpha - expect) <= 1e-2 * abs(expect)

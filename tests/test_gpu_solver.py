@@ -1,0 +1,216 @@
+"""Parity of the device objective / TV / NLCG kernels with the CPU oracle.
+
+NLCG bar: objective after a fixed iteration count within 1e-4 relative of the
+reference-semantics oracle run (oracle/nlcg.py, FP64) on the same data.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from mrirecon_amd import CGConfig, CGSolver, NLCGEngine, RegManager, TVND, TVParams, _lib, synth
+from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+from oracle.kbnufft import KBNufft, rel_l2
+from oracle.nlcg import TV, nlcg, objective_value, tv_energy_grad
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def c64(a):
+    return torch.from_numpy(np.ascontiguousarray(a).astype(np.complex64)).to(DEV)
+
+
+def crand(shape, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+
+
+def workspace():
+    return torch.empty(int(_lib.load().mrh_reduce_workspace_len()), dtype=torch.float64, device=DEV)
+
+
+def test_residual_energy_and_dot():
+    lib = _lib.load()
+    n = 300_001
+    Ax, Ad, y = (c64(crand(n, s)) for s in (1, 2, 3))
+    r = torch.empty_like(Ax)
+    out = torch.zeros(1, dtype=torch.float64, device=DEV)
+    w = workspace()
+    _lib.check(lib.mrh_residual_energy(r.data_ptr(), Ax.data_ptr(), Ad.data_ptr(), y.data_ptr(), 0.37, n,
+                                       w.data_ptr(), out.data_ptr(), _lib.stream_of(r)))
+    rr = (0.37 * Ad.cpu().numpy().astype(np.complex128) + Ax.cpu().numpy() - y.cpu().numpy())
+    assert abs(out.item() - np.sum(np.abs(rr) ** 2)) <= 1e-6 * np.sum(np.abs(rr) ** 2)
+    assert rel_l2(r.cpu().numpy(), rr) <= 1e-6
+    _lib.check(lib.mrh_dot_real(Ax.data_ptr(), Ad.data_ptr(), n, w.data_ptr(), out.data_ptr(), _lib.stream_of(r)))
+    ref = np.vdot(Ax.cpu().numpy().astype(np.complex128), Ad.cpu().numpy().astype(np.complex128)).real
+    assert abs(out.item() - ref) <= 1e-9 * np.sqrt(n) * 2
+
+
+@pytest.mark.parametrize("iso", [True, False])
+def test_tv_energy_grad_matches_oracle(iso):
+    P, F, S0, S1 = 2, 3, 16, 20
+    x = crand((P * F, S0, S1), 4)
+    regs = [TVND("t", TVParams(weight=0.02, axes="temporal")),
+            TVND("s", TVParams(weight=0.01, axes="spatial", isotropic=iso)),
+            TVND("w", TVParams(weight=0.005, axes=(0, 2, 3), isotropic=iso, axis_weights=[0.5, 1.0, 2.0]))]
+    regm = RegManager(regs)
+    xt = c64(x)
+    g = c64(np.zeros_like(x))
+    e = regm.energy_and_grad_tensors(xt, g, problems=P).cpu().numpy()
+    oregs = [TV(0.02, (0,)), TV(0.01, (1, 2), isotropic=iso),
+             TV(0.005, (0, 1, 2), isotropic=iso, axis_weights=[0.5, 1.0, 2.0])]
+    for p in range(P):
+        xp = x[p * F:(p + 1) * F].astype(np.complex128)
+        gp = np.zeros_like(xp)
+        ep = sum(tv_energy_grad(xp, r, gp) for r in oregs)
+        assert abs(e[p] - ep) <= 2e-6 * abs(ep), (p, e[p], ep)
+        assert rel_l2(g.cpu().numpy()[p * F:(p + 1) * F], gp) <= 2e-6
+
+
+def small_dynamic(seed=21, F=5, S=64, C=4, spokes=13, R=128):
+    traj, dcf, off = synth.radial_frames([spokes] * F, R)
+    x = synth.dynamic_phantom(S, F, seed=seed, n_ellipses=6)
+    return synth.Workload("dyn", S, 1, F, C, R, 6, 2.0, [spokes] * F, synth.gaussian_maps(C, S), traj, dcf,
+                          off, x, lam_t=0.02, lam_s=0.002, n_iter=15, noise_seed=seed + 1)
+
+
+def prepare(wl):
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=wl.W, osf=wl.osf)
+    be.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(wl.traj), torch.from_numpy(wl.dcf), c64(wl.maps),
+               device=DEV, k_offsets=wl.k_offsets)
+    ora = KBNufft(wl.maps, wl.traj, wl.k_offsets, wl.dcf, wl.W, wl.osf)
+    synth.measure(wl, ora.forward)
+    return be, ora
+
+
+def regs_for(wl):
+    g, o = [], []
+    if wl.lam_t:
+        g.append(TVND("tv_t", TVParams(weight=wl.lam_t, axes="temporal")))
+        o.append(TV(wl.lam_t, (0,)))
+    if wl.lam_s:
+        g.append(TVND("tv_s", TVParams(weight=wl.lam_s, axes="spatial")))
+        o.append(TV(wl.lam_s, (1, 2)))
+    return RegManager(g), o
+
+
+def run_pair(wl, n_iter, direction="fr", use_graph=True, rel_tol=None):
+    be, ora = prepare(wl)
+    regm, oregs = regs_for(wl)
+    x0 = ora.adjoint(wl.y)
+    eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, direction=direction,
+                     rel_tol=rel_tol or 0.0)
+    eng.prime(c64(x0))
+    eng.run(n_iter, use_graph=use_graph)
+    torch.cuda.synchronize()
+    hist = eng.history()[0]
+    x_gpu = eng.x.cpu().numpy()
+    ref = nlcg(ora.forward, ora.adjoint, wl.y.astype(np.complex128), x0.astype(np.complex128), oregs,
+               max_iter=n_iter, direction=direction, mutate_x=False, rel_tol=rel_tol)
+    f_gpu = objective_value(ora.forward, wl.y, x_gpu.astype(np.complex128), oregs)
+    return eng, hist, ref, f_gpu
+
+
+def test_nlcg_config1_matches_reference_semantics():
+    """c1: 20 FR iterations, spatial TV 0.01 -- objective within 1e-4 of the oracle run."""
+    wl = synth.config_c1()
+    eng, hist, ref, f_gpu = run_pair(wl, 20)
+    fh = np.array([h["f"] for h in hist])
+    fr = np.array([h["f"] for h in ref.history])
+    steps = [h["step"] for h in hist]
+    print(f"[c1] f_gpu={f_gpu:.8e} f_ref={ref.f_final:.8e} rel={abs(f_gpu - ref.f_final) / ref.f_final:.2e} "
+          f"steps={steps}")
+    assert len(hist) == 20 and ref.n_iter == 20
+    assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
+    assert abs(f_gpu - ref.f_final) <= 1e-4 * abs(ref.f_final)
+    assert [h["step"] for h in ref.history] == steps
+    for h in hist:       # Armijo inequality on the device-side selection
+        assert h["f_step"] <= h["f"] + 1e-4 * h["step"] * h["gdot"] + 1e-9 * abs(h["f"])
+
+
+@pytest.mark.parametrize("direction", ["fr", "prplus"])
+def test_nlcg_dynamic_temporal_spatial_tv(direction):
+    wl = small_dynamic()
+    eng, hist, ref, f_gpu = run_pair(wl, 15, direction=direction)
+    rel = abs(f_gpu - ref.f_final) / ref.f_final
+    print(f"[dyn {direction}] f_gpu={f_gpu:.8e} f_ref={ref.f_final:.8e} rel={rel:.2e}")
+    assert rel <= 1e-4
+    fh = np.array([h["f"] for h in hist])
+    fr = np.array([h["f"] for h in ref.history])
+    assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
+
+
+def test_graph_replay_equals_eager():
+    wl = small_dynamic(seed=31)
+    _, h_graph, _, f_graph = run_pair(wl, 8, use_graph=True)
+    _, h_eager, _, f_eager = run_pair(wl, 8, use_graph=False)
+    for a, b in zip(h_graph, h_eager):
+        assert a["step"] == b["step"]
+        assert abs(a["f"] - b["f"]) <= 1e-6 * abs(a["f"])
+    assert abs(f_graph - f_eager) <= 1e-6 * abs(f_eager)
+
+
+def test_rel_tol_stop_rule():
+    wl = small_dynamic(seed=41)
+    eng, hist, ref, _ = run_pair(wl, 60, rel_tol=1e-4)
+    print(f"[dyn rel_tol] gpu iters={len(hist)} stop={eng.stop_reasons()} ref iters={ref.n_iter} stop={ref.stop}")
+    assert eng.stop_reasons()[0] == "rel_tol" and ref.stop == "rel_tol"
+    assert abs(len(hist) - ref.n_iter) <= 1
+
+
+def test_batched_problems_equal_separate_runs():
+    """Two slices with uneven spokes in one engine == each slice alone."""
+    S, C, R, F = 48, 3, 96, 4
+    spokes = [[13, 21, 17, 30], [25, 14, 40, 19]]
+    ws = []
+    for i, sp in enumerate(spokes):
+        traj, dcf, off = synth.radial_frames(sp, R)
+        ws.append(synth.Workload(f"s{i}", S, 1, F, C, R, 6, 2.0, sp, synth.gaussian_maps(C, S), traj, dcf, off,
+                                 synth.dynamic_phantom(S, F, seed=50 + i, n_ellipses=5), lam_t=0.02,
+                                 noise_seed=60 + i))
+    solo = []
+    for wl in ws:
+        be, ora = prepare(wl)
+        regm, _ = regs_for(wl)
+        eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=F, regm=regm)
+        eng.prime(c64(ora.adjoint(wl.y)))
+        eng.run(6)
+        solo.append((eng.history()[0], eng.x.cpu().numpy()))
+    traj = np.concatenate([w.traj for w in ws], axis=1)
+    dcf = np.concatenate([w.dcf for w in ws])
+    off = np.concatenate([[0], np.cumsum([o for w in ws for o in np.diff(w.k_offsets)])])
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=6)
+    be.prepare((C, S, S), torch.from_numpy(traj), torch.from_numpy(dcf), c64(ws[0].maps), device=DEV,
+               k_offsets=off)
+    y = np.concatenate([w.y for w in ws])
+    x0 = np.concatenate([KBNufft(w.maps, w.traj, w.k_offsets, w.dcf, 6).adjoint(w.y) for w in ws])
+    eng = NLCGEngine(be.plan, c64(y), problems=2, frames=F, regm=regs_for(ws[0])[0])
+    eng.prime(c64(x0))
+    eng.run(6)
+    hb = eng.history()
+    xb = eng.x.cpu().numpy()
+    for p in range(2):
+        assert [h["step"] for h in hb[p]] == [h["step"] for h in solo[p][0]]
+        for a, b in zip(hb[p], solo[p][0]):
+            assert abs(a["f"] - b["f"]) <= 1e-6 * abs(b["f"])
+        assert rel_l2(xb[p * F:(p + 1) * F], solo[p][1]) <= 1e-5
+
+
+def test_cgsolver_reference_surface():
+    """CGSolver(y, nufft_op, regm, cfg).run().result()/history() with the reference's keys."""
+    wl = synth.config_c1()
+    be, ora = prepare(wl)
+    regm, _ = regs_for(wl)
+    y = c64(wl.y).reshape(1, wl.n_coils, -1)
+    solver = CGSolver(y, be, regm, CGConfig(max_iter=5, init_mode="backproj"))
+    solver.run()
+    h = solver.history()
+    assert len(h) == 5
+    for k in ("iter", "f", "gdot", "gdot_at_step", "step", "xnorm", "gnorm", "stepnorm"):
+        assert k in h[0]
+    assert tuple(solver.result().shape) == (1, 1, wl.S, wl.S)
+    assert solver.ws.get("x").data_ptr() == solver.ws.get("var").data_ptr()
+    with pytest.raises(NotImplementedError):
+        CGSolver(y, be, regm, CGConfig(ls_name="wolfe"))

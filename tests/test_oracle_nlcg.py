@@ -1,0 +1,168 @@
+"""Pins of the CPU oracle's NLCG / TV / Armijo restatement (oracle/nlcg.py).
+
+Mirrors the reference's own numerics tests: the unitary-operator quadratic of
+graspcg/tests/test_numerics_ns.py:110-248, the Armijo inequality / monotone
+history and Richardson slope of graspcg/tests/test_cg_descend_compile_ns.py:75-229,
+TV of a constant (graspcg/tests/test_regularization_ns.py:92-103), plus the
+summation-by-parts identity of the stencils (graspcg/ops/ndops.py:28-59) and a
+hand-computable step-edge energy.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle.nlcg import TV, Objective, armijo, bwd_diff, fwd_diff, nlcg, objective_value, tv_energy_grad
+
+
+def fft_op():
+    A = lambda x: np.fft.fft2(x, norm="ortho").reshape(-1)
+    AH = lambda y: np.fft.ifft2(y.reshape(-1, 16, 16), norm="ortho")
+    return A, AH
+
+
+def rand_c(shape, seed, dtype=np.complex128):
+    rng = np.random.default_rng(seed)
+    return (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(dtype)
+
+
+def test_unitary_quadratic_closed_form():
+    """A unitary, y = 0, d = -x: f(t) = 0.5 (1-t)^2 |x|^2, g.d = -(1-t) |x|^2 (test_numerics_ns.py:110-139)."""
+    A, AH = fft_op()
+    x = rand_c((2, 16, 16), 0)
+    obj = Objective(A, AH, np.zeros(2 * 256, complex), [], mutate_x=False)
+    d = -x
+    obj.begin_linesearch(x, d)
+    n2 = float(np.vdot(x, x).real)
+    for t in (0.0, 0.25, 0.5, 1.0, 1.5):
+        f, gd, _ = obj.f_g(x, d, t)
+        assert abs(f - 0.5 * (1 - t) ** 2 * n2) <= 1e-10 * n2
+        assert abs(gd + (1 - t) * n2) <= 1e-10 * n2
+
+
+def test_armijo_accepts_unit_step_on_quadratic():
+    A, AH = fft_op()
+    x = rand_c((2, 16, 16), 1)
+    obj = Objective(A, AH, np.zeros(2 * 256, complex), [], mutate_x=False)
+    ok, t, f0, g0d, ft, gdt, _ = armijo(obj, x, -x)
+    assert ok and t == 1.0 and abs(ft) < 1e-20 * f0
+
+
+def test_armijo_rejects_ascent_direction():
+    """d = +x is not a descent direction: rejected with t = 0 (test_numerics_ns.py:229-248)."""
+    A, AH = fft_op()
+    x = rand_c((2, 16, 16), 2)
+    obj = Objective(A, AH, np.zeros(2 * 256, complex), [], mutate_x=False)
+    ok, t, f0, g0d, *_ = armijo(obj, x, x.copy())
+    assert not ok and t == 0.0 and g0d >= 0
+
+
+def dense_problem(seed=3, n=6, m=40, F=3, weighted=True):
+    rng = np.random.default_rng(seed)
+    M = (rng.standard_normal((m, F * n * n)) + 1j * rng.standard_normal((m, F * n * n))) / math.sqrt(m)
+    A = lambda x: M @ x.reshape(-1)
+    AH = lambda y: (M.conj().T @ y).reshape(F, n, n)
+    xt = rand_c((F, n, n), seed + 1)
+    y = A(xt) + 0.01 * rand_c(m, seed + 2)
+    regs = [TV(0.05, (1, 2)), TV(0.1, (0,)),
+            TV(0.02, (1, 2), isotropic=False, axis_weights=[1.0, 0.5] if weighted else None)]
+    return A, AH, y, np.zeros((F, n, n), complex), regs
+
+
+def test_armijo_inequality_and_monotone_history():
+    A, AH, y, x0, regs = dense_problem()
+    res = nlcg(A, AH, y, x0, regs, max_iter=15)
+    assert res.n_iter == 15
+    for h in res.history:
+        assert h["f_step"] <= h["f"] + 1e-4 * h["step"] * h["gdot"] + 1e-12
+        assert h["gdot"] < 0
+    fs = [h["f"] for h in res.history] + [res.f_final]
+    assert all(b <= a + 1e-12 for a, b in zip(fs, fs[1:]))
+
+
+def test_richardson_fd_slope():
+    """|g_fd - g.d| <= max(1e-6, 1e-2 max(1, |g.d|)) (test_cg_descend_compile_ns.py:180-229)."""
+    A, AH, y, x0, regs = dense_problem(7, weighted=False)
+    x = rand_c(x0.shape, 8)
+    d = rand_c(x0.shape, 9)
+    obj = Objective(A, AH, y, regs, mutate_x=False)
+    obj.begin_linesearch(x, d)
+    f = lambda t: obj.f_g(x, d, t)[0]
+    for t in (0.0, 0.3):
+        h = 1e-4
+        d1 = (f(t + h) - f(t - h)) / (2 * h)
+        d2 = (f(t + h / 2) - f(t - h / 2)) / h
+        gfd = (4 * d2 - d1) / 3
+        gd = obj.f_g(x, d, t)[1]
+        assert abs(gfd - gd) <= max(1e-6, 1e-6 * max(1.0, abs(gd))), (t, gfd, gd)
+
+
+def test_tv_constant_is_zero():
+    x = np.full((3, 8, 8), 2.0 - 1.0j)
+    g = np.zeros_like(x)
+    for reg in (TV(0.3, (1, 2)), TV(0.3, (0,)), TV(0.3, (0, 1, 2), isotropic=False)):
+        assert tv_energy_grad(x, reg, g) == 0.0
+    assert np.abs(g).max() == 0.0
+
+
+def test_fwd_bwd_summation_by_parts():
+    """<D_f x, p> = -<x, bwd_diff p> whenever p's last slice along the axis is zero."""
+    x = rand_c((4, 5, 6), 10)
+    for ax in range(3):
+        p = fwd_diff(rand_c((4, 5, 6), 11 + ax), ax)     # last slice zero
+        lhs = np.vdot(p, fwd_diff(x, ax))
+        rhs = -np.vdot(bwd_diff(p, ax), x)
+        assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_step_edge_energy():
+    """One vertical edge of height h across S rows: E = lambda S (sqrt(h^2 + eps^2) - eps)."""
+    S, h, lam, eps = 16, 0.7, 0.05, 1e-3
+    x = np.zeros((1, S, S), complex)
+    x[:, :, S // 2:] = h
+    e = tv_energy_grad(x, TV(lam, (1, 2), eps=eps))
+    assert abs(e - lam * S * (math.sqrt(h * h + eps * eps) - eps)) < 1e-12
+
+
+def test_tv_gradient_matches_finite_differences():
+    x = rand_c((3, 6, 7), 12)
+    v = rand_c((3, 6, 7), 13)
+    for reg in (TV(0.2, (1, 2)), TV(0.1, (0,)), TV(0.1, (0, 1, 2), isotropic=False, axis_weights=[2, 1, 0.5])):
+        g = np.zeros_like(x)
+        tv_energy_grad(x, reg, g)
+        h = 1e-6
+        fd = (tv_energy_grad(x + h * v, reg) - tv_energy_grad(x - h * v, reg)) / (2 * h)
+        an = np.vdot(g, v).real
+        if reg.axis_weights is None:
+            assert abs(fd - an) <= 1e-6 * max(1.0, abs(fd))
+        else:
+            # reference quirk (graspcg/regularization/tv_nd.py:70-86): the weights scale the
+            # differences but not the divergence, so the gradient is not the energy's
+            assert abs(fd - an) > 1e-3 * abs(fd)
+
+
+def test_fr_first_beta_is_one():
+    """FR keeps no rho before the first update, so the first beta is exactly 1
+    (graspcg/numerics/directions.py:145-155)."""
+    A, AH, y, x0, regs = dense_problem(14)
+    r1 = nlcg(A, AH, y, x0, regs, max_iter=2)
+    # replay by hand: after iteration 0, d1 = 1 * d0 - g1
+    obj = Objective(A, AH, y, regs)
+    x = x0.copy()
+    obj.begin_linesearch(x, np.zeros_like(x))
+    _, _, g0 = obj.f_g(x, np.zeros_like(x), 0.0)
+    d0 = -g0
+    ok, t, f0, g0d, ft, gdt, g1 = armijo(obj, x, d0)
+    assert ok and abs(t - r1.history[0]["step"]) == 0
+    x1 = x + t * d0
+    d1 = d0 - g1
+    ok2, t2, f1, g1d, *_ = armijo(Objective(A, AH, y, regs), x1, d1)
+    assert abs(f1 - r1.history[1]["f"]) <= 1e-12 * abs(f1)
+
+
+def test_objective_value_matches_f_at_zero():
+    A, AH, y, x0, regs = dense_problem(15)
+    x = rand_c(x0.shape, 16)
+    obj = Objective(A, AH, y, regs, mutate_x=False)
+    obj.begin_linesearch(x, np.zeros_like(x))
+    assert abs(obj.f_g(x, np.zeros_like(x), 0.0)[0] - objective_value(A, y, x, regs)) < 1e-10
