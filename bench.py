@@ -186,7 +186,9 @@ def main():
         ymeas = y + torch.from_numpy(synth.complex_noise(ynp.shape, sigma, wl.noise_seed)).to(dev)
         regm = RegManager([TVND("tv_t", TVParams(weight=wl.lam_t, axes="temporal")),
                            TVND("tv_s", TVParams(weight=wl.lam_s, axes="spatial"))])
-        eng = NLCGEngine(plan, ymeas, problems=1, frames=wl.B, regm=regm, rel_tol=1e-4,
+        # PR+ (Polak-Ribiere): the reference default FR stalls on this workload after one step
+        # (its first beta = 1 yields an ascent direction; the oracle reproduces the stop)
+        eng = NLCGEngine(plan, ymeas, problems=1, frames=wl.B, regm=regm, rel_tol=1e-4, direction="prplus",
                          max_hist=args.tol_iters + 1)
         x0 = torch.empty_like(x)
         # warm the graph (capture) outside the timed region
@@ -201,7 +203,8 @@ def main():
         torch.cuda.synchronize()
         t_plain = time.perf_counter() - t0
         h = eng.history()[0]
-        ttt = {"plain_seconds": round(t_plain, 4), "plain_iterations": len(h), "stop": eng.stop_reasons()[0],
+        ttt = {"direction": "prplus", "rel_tol": 1e-4, "plain_seconds": round(t_plain, 4),
+               "plain_iterations": len(h), "stop": eng.stop_reasons()[0],
                "ms_per_iteration": round(t_plain / max(1, n) * 1e3, 3),
                "f_first": h[0]["f"] if h else None, "f_last": h[-1]["f_step"] if h else None,
                "warm_start": "not built in this tree (no learned warm start yet)"}

@@ -13,7 +13,8 @@ from pathlib import Path
 
 import torch  # noqa: F401  (loads the HIP runtime first)
 
-LIB_PATH = Path(__file__).resolve().parent / "libmrirecon_hip.so"
+# MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
+LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
 ABI_VERSION = 1
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2

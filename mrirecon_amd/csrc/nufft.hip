@@ -55,6 +55,9 @@ __device__ __forceinline__ float kb_weight(const KBPoly& kb, float d) {
 }
 
 #define MRH_WMAX 8
+#define STRIP_TC 8                     // grid columns per strip workgroup
+#define STRIP_THREADS (64 * STRIP_TC)  // one wavefront per column
+#define STRIP_EPT 1                    // sample footprints cached in registers per thread
 
 // ---------------------------------------------------------------- plan
 struct mrh_nufft_plan {
@@ -99,8 +102,8 @@ static double kb_ft(double xi, int W, double beta) {
 
 static bool fft_size_ok(int n) {
   switch (n) {
-    case 32: case 64: case 96: case 128: case 192: case 256: case 320: case 384:
-    case 512: case 640: case 768: case 1024: return true;
+    case 32: case 64: case 96: case 128: case 192: case 256: case 384:
+    case 512: case 768: case 1024: return true;
     default: return false;
   }
 }
@@ -143,8 +146,8 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   MRH_REQUIRE(d->C >= 1 && d->B >= 1 && d->like >= 1, MRH_EINVAL, "C, B, like must be >= 1");
   MRH_REQUIRE(d->k_offsets && d->traj, MRH_EINVAL, "k_offsets and traj are required (host pointers)");
   MRH_REQUIRE(d->maps, MRH_EINVAL, "maps (device pointer) is required");
-  int Tc = d->strip_cols > 0 ? d->strip_cols : (d->G1 % 16 == 0 ? 16 : 8);
-  MRH_REQUIRE(Tc == 8 || Tc == 16, MRH_EUNSUP, "strip width must be 8 or 16 (got %d)", Tc);
+  int Tc = d->strip_cols > 0 ? d->strip_cols : STRIP_TC;
+  MRH_REQUIRE(Tc == STRIP_TC, MRH_EUNSUP, "strip width must be %d (got %d)", STRIP_TC, Tc);
   MRH_REQUIRE(d->G1 % Tc == 0, MRH_EINVAL, "G1=%d not a multiple of the strip width %d", d->G1, Tc);
   MRH_REQUIRE(Tc >= d->W, MRH_EINVAL, "strip width %d must be >= KB width %d", Tc, d->W);
   MRH_REQUIRE(d->G0 < 65536 && d->G1 < 65536, MRH_EINVAL, "grid too large");
@@ -169,10 +172,9 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
       coef[n] = 1.0 / (fact * fact * i0b);
     }
   }
-  int np = 16;
-  while (np < MRH_KB_MAXNP && coef[np] * std::pow(q, np) > 1e-10) np += 8;
-  if (np > MRH_KB_MAXNP) np = MRH_KB_MAXNP;
-  MRH_REQUIRE(coef[np] * std::pow(q, np) < 1e-6, MRH_EUNSUP, "KB beta %.3f too large for the series", beta);
+  int np = 1;
+  while (np < MRH_KB_MAXNP && coef[np] * std::pow(q, np) > 1e-10) ++np;
+  MRH_REQUIRE(coef[np] * std::pow(q, np) < 1e-9, MRH_EUNSUP, "KB beta %.3f too large for the series", beta);
   p->np = np;
   p->kb.qscale = (float)q;
   p->kb.two_over_w = (float)(2.0 / d->W);
@@ -350,133 +352,263 @@ __global__ __launch_bounds__(256) void k_fwd_rows(
   }
 }
 
-// Forward strip pass.  Grid (nstrips, nbatch*C), 64*Tc threads (one wavefront per column).
-template <int G0, int NP>
-__global__ __launch_bounds__(1024) void k_fwd_strips(
-    const float2* __restrict__ T, float2* __restrict__ y, const int2* __restrict__ em,
-    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Tc, int W) {
-  constexpr int CS = fft_padded(G0) + 1;
-  extern __shared__ float2 smem[];
-  float2* stw = smem + Tc * CS;
-  const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
-  const int s = blockIdx.x, cf = blockIdx.y, nstr = gridDim.x;
-  const int bb = cf / C, c = cf - bb * C, b = bb / like, l = bb - b * like;
-  for (int i = tid; i < G0; i += blockDim.x) stw[i] = tw0[i];
-  const float2* Tb = T + ((int64_t)cf * nstr + s) * S0 * Tc;
-  const int h0 = S0 / 2;
-  for (int e = tid; e < G0 * Tc; e += blockDim.x) {
-    const int r = e / Tc, j = e - r * Tc;
-    float2 v = make_float2(0.f, 0.f);
-    if (r < S0 - h0) v = Tb[(r + h0) * Tc + j];
-    else if (r >= G0 - h0) v = Tb[(r - G0 + h0) * Tc + j];
-    smem[j * CS + PIDX(r)] = v;
-  }
-  __syncthreads();
-  fft_lds<G0, false, 64>(smem + col * CS, stw, lane);
+// ---------------------------------------------------------------- strip kernels
+// One workgroup per (batch frame, strip of STRIP_TC grid columns); it loops
+// over the coils of the frame.  Each thread evaluates the KB footprint of up to
+// STRIP_EPT of the strip's samples once and keeps it in registers, so the
+// weights are shared by all coils; samples beyond that spill to an
+// evaluate-on-the-fly path.
+template <int WM>
+struct Foot {
+  float wx[WM], wy[WM];  // wy[j] = 0 when column (my + j) mod G1 lies outside the strip
+  int r0, my, k;         // first footprint row / column (mod G), sample index in the frame (-1: empty)
+  float wA, wAH;
+};
 
-  const int e0 = soff[b * nstr + s], e1 = soff[b * nstr + s + 1];
-  const int64_t Kb = koff[b + 1] - koff[b];
-  float2* yb = y + (int64_t)C * (koff[b] * like + (int64_t)l * Kb) + (int64_t)c * Kb;
-  const int c0 = s * Tc;
-  for (int e = e0 + tid; e < e1; e += blockDim.x) {
-    const int2 m = em[e];
-    const float4 f = ef[e];
-    const int mx = m.x & 0xffff, my = (int)((unsigned)m.x >> 16);
-    float wx[MRH_WMAX];
+template <int WM>
+__device__ __forceinline__ void load_foot(Foot<WM>& F, int2 m, float4 f, int W, int c0, int G1,
+                                          const KBPoly& kb) {
+  F.r0 = m.x & 0xffff;
+  F.my = (int)((unsigned)m.x >> 16);
+  F.k = m.y;
+  F.wA = f.z;
+  F.wAH = f.w;
 #pragma unroll
-    for (int i = 0; i < MRH_WMAX; ++i) wx[i] = (i < W) ? kb_weight<NP>(kb, f.x - (float)i) : 0.f;
-    float2 acc = make_float2(0.f, 0.f);
+  for (int i = 0; i < WM; ++i) F.wx[i] = (i < W) ? kb_weight<MRH_KB_MAXNP>(kb, f.x - (float)i) : 0.f;
 #pragma unroll
-    for (int j = 0; j < MRH_WMAX; ++j) {
-      if (j < W) {
-        int cc = my + j;
-        if (cc >= G1) cc -= G1;
-        const int lc = cc - c0;
-        if ((unsigned)lc < (unsigned)Tc) {
-          const float wy = kb_weight<NP>(kb, f.y - (float)j);
-          const float2* colp = smem + lc * CS;
-          float2 rs = make_float2(0.f, 0.f);
-#pragma unroll
-          for (int i = 0; i < MRH_WMAX; ++i) {
-            if (i < W) {
-              int rr = mx + i;
-              if (rr >= G0) rr -= G0;
-              const float2 g = colp[PIDX(rr)];
-              rs.x = fmaf(g.x, wx[i], rs.x);
-              rs.y = fmaf(g.y, wx[i], rs.y);
-            }
-          }
-          acc.x = fmaf(rs.x, wy, acc.x);
-          acc.y = fmaf(rs.y, wy, acc.y);
-        }
-      }
-    }
-    unsafeAtomicAdd(&yb[m.y].x, acc.x * f.z);
-    unsafeAtomicAdd(&yb[m.y].y, acc.y * f.z);
+  for (int j = 0; j < WM; ++j) {
+    int cc = F.my + j;
+    if (cc >= G1) cc -= G1;
+    F.wy[j] = (j < W && (unsigned)(cc - c0) < (unsigned)STRIP_TC) ? kb_weight<MRH_KB_MAXNP>(kb, f.y - (float)j) : 0.f;
   }
 }
 
-// Adjoint strip pass: spread into the LDS strip, inverse column FFT, crop rows.
-template <int G0, int NP>
-__global__ __launch_bounds__(1024) void k_adj_strips(
-    const float2* __restrict__ y, float2* __restrict__ T, const int2* __restrict__ em,
-    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Tc, int W) {
-  constexpr int CS = fft_padded(G0) + 1;
-  extern __shared__ float2 smem[];
-  float2* stw = smem + Tc * CS;
-  const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
-  const int s = blockIdx.x, cf = blockIdx.y, nstr = gridDim.x;
-  const int bb = cf / C, c = cf - bb * C, b = bb / like, l = bb - b * like;
-  for (int i = tid; i < G0; i += blockDim.x) stw[i] = tw0[i];
-  for (int i = tid; i < Tc * CS; i += blockDim.x) smem[i] = make_float2(0.f, 0.f);
-  __syncthreads();
+// Forward: sum of the footprint over the LDS strip (column-major, padded rows).
+template <int WM, int CS>
+__device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* smem, int W, int c0, int G0,
+                                             int G1) {
+  float2 acc = make_float2(0.f, 0.f);
+#pragma unroll
+  for (int j = 0; j < WM; ++j) {
+    if (F.wy[j] != 0.f) {
+      int cc = F.my + j;
+      if (cc >= G1) cc -= G1;
+      const float2* colp = smem + (cc - c0) * CS;
+      float2 rs = make_float2(0.f, 0.f);
+#pragma unroll
+      for (int i = 0; i < WM; ++i) {
+        if (i < W) {
+          int rr = F.r0 + i;
+          if (rr >= G0) rr -= G0;
+          const float2 g = colp[PIDX(rr)];
+          rs.x = fmaf(g.x, F.wx[i], rs.x);
+          rs.y = fmaf(g.y, F.wx[i], rs.y);
+        }
+      }
+      acc.x = fmaf(rs.x, F.wy[j], acc.x);
+      acc.y = fmaf(rs.y, F.wy[j], acc.y);
+    }
+  }
+  return acc;
+}
 
+// Forward strip pass.  Grid (nstrips, nbatch), STRIP_THREADS threads (one wavefront per column).
+template <int G0, int WT>
+__global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
+    const float2* __restrict__ T, float2* __restrict__ y, const int2* __restrict__ em,
+    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt) {
+  constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
+  constexpr int WM = WT ? WT : MRH_WMAX;
+  const int W = WT ? WT : Wrt;
+  __shared__ float2 smem[TC * CS];
+  __shared__ float2 stw[G0];
+  const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
+  const int s = blockIdx.x, bb = blockIdx.y, nstr = gridDim.x;
+  const int b = bb / like, l = bb - b * like;
+  for (int i = tid; i < G0; i += NTH) stw[i] = tw0[i];
   const int e0 = soff[b * nstr + s], e1 = soff[b * nstr + s + 1];
   const int64_t Kb = koff[b + 1] - koff[b];
-  const float2* yb = y + (int64_t)C * (koff[b] * like + (int64_t)l * Kb) + (int64_t)c * Kb;
-  const int c0 = s * Tc;
-  for (int e = e0 + tid; e < e1; e += blockDim.x) {
-    const int2 m = em[e];
-    const float4 f = ef[e];
-    const int mx = m.x & 0xffff, my = (int)((unsigned)m.x >> 16);
-    const float2 yv = cscale(yb[m.y], f.w);
-    float wx[MRH_WMAX];
+  float2* ybase = y + (int64_t)C * (koff[b] * like + (int64_t)l * Kb);
+  const int c0 = s * TC;
+  Foot<WM> F[STRIP_EPT];
 #pragma unroll
-    for (int i = 0; i < MRH_WMAX; ++i) wx[i] = (i < W) ? kb_weight<NP>(kb, f.x - (float)i) : 0.f;
+  for (int q = 0; q < STRIP_EPT; ++q) {
+    const int e = e0 + tid + q * NTH;
+    if (e < e1) load_foot<WM>(F[q], em[e], ef[e], W, c0, G1, kb);
+    else F[q].k = -1;
+  }
+  const int h0 = S0 / 2;
+  const int64_t cstride = (int64_t)nstr * S0 * TC;
+  const float2* Tbb = T + ((int64_t)bb * C * nstr + s) * S0 * TC;
+  for (int c = 0; c < C; ++c) {
+    const float2* Tb = Tbb + c * cstride;
+    for (int e = tid; e < G0 * TC; e += NTH) {
+      const int r = e / TC, j = e - r * TC;
+      float2 v = make_float2(0.f, 0.f);
+      if (r < S0 - h0) v = Tb[(r + h0) * TC + j];
+      else if (r >= G0 - h0) v = Tb[(r - G0 + h0) * TC + j];
+      smem[j * CS + PIDX(r)] = v;
+    }
+    __syncthreads();
+    fft_lds<G0, false, 64>(smem + col * CS, stw, lane);
+    float2* yc = ybase + (int64_t)c * Kb;
 #pragma unroll
-    for (int j = 0; j < MRH_WMAX; ++j) {
-      if (j < W) {
-        int cc = my + j;
-        if (cc >= G1) cc -= G1;
-        const int lc = cc - c0;
-        if ((unsigned)lc < (unsigned)Tc) {
-          const float wy = kb_weight<NP>(kb, f.y - (float)j);
-          const float2 v = cscale(yv, wy);
-          float2* colp = smem + lc * CS;
+    for (int q = 0; q < STRIP_EPT; ++q) {
+      if (F[q].k >= 0) {
+        const float2 a = interp_one<WM, CS>(F[q], smem, W, c0, G0, G1);
+        unsafeAtomicAdd(&yc[F[q].k].x, a.x * F[q].wA);
+        unsafeAtomicAdd(&yc[F[q].k].y, a.y * F[q].wA);
+      }
+    }
+    for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
+      Foot<WM> Fo;
+      load_foot<WM>(Fo, em[e], ef[e], W, c0, G1, kb);
+      const float2 a = interp_one<WM, CS>(Fo, smem, W, c0, G0, G1);
+      unsafeAtomicAdd(&yc[Fo.k].x, a.x * Fo.wA);
+      unsafeAtomicAdd(&yc[Fo.k].y, a.y * Fo.wA);
+    }
+    __syncthreads();
+  }
+}
+
+// Exact fixed-point accumulation for the adjoint: contributions are scaled by
+// 2^sc (sc chosen per strip and coil so that the strip's largest possible sum
+// stays below 2^61) and added as two's-complement int64 with ds_add_u64, which
+// runs at full LDS rate on gfx950 (float LDS atomics do not).  The sum is exact
+// and independent of the order of the adds; converting back costs one rounding.
+__device__ __forceinline__ unsigned long long to_fixed(float v, int sc) {
+  const float a = ldexpf(fabsf(v), sc);                      // < 2^61, exact scaling
+  const float hf = floorf(a * 2.3283064365386963e-10f);     // a / 2^32 (exact), integer part
+  const float lf = fmaf(-hf, 4294967296.0f, a);             // a mod 2^32, exact, in [0, 2^32)
+  unsigned long long u = ((unsigned long long)(unsigned)hf << 32) | (unsigned)lf;
+  return v < 0.f ? (~u + 1ull) : u;
+}
+
+__device__ __forceinline__ float from_fixed(unsigned long long u, int sc) {
+  const long long q = (long long)u;
+  return (float)ldexp((double)q, -sc);
+}
+
+template <int WM, int CS>
+__device__ __forceinline__ void spread_one(const Foot<WM>& F, float2 v, unsigned long long* acc, int W, int c0,
+                                           int G0, int G1, int sc) {
 #pragma unroll
-          for (int i = 0; i < MRH_WMAX; ++i) {
-            if (i < W) {
-              int rr = mx + i;
-              if (rr >= G0) rr -= G0;
-              float2* dst = colp + PIDX(rr);
-              atomicAdd(&dst->x, v.x * wx[i]);
-              atomicAdd(&dst->y, v.y * wx[i]);
-            }
-          }
+  for (int j = 0; j < WM; ++j) {
+    if (F.wy[j] != 0.f) {
+      int cc = F.my + j;
+      if (cc >= G1) cc -= G1;
+      unsigned long long* colp = acc + 2 * (cc - c0) * CS;
+      const float2 vj = cscale(v, F.wy[j]);
+#pragma unroll
+      for (int i = 0; i < WM; ++i) {
+        if (i < W) {
+          int rr = F.r0 + i;
+          if (rr >= G0) rr -= G0;
+          unsigned long long* dst = colp + 2 * PIDX(rr);
+          atomicAdd(dst, to_fixed(vj.x * F.wx[i], sc));
+          atomicAdd(dst + 1, to_fixed(vj.y * F.wx[i], sc));
         }
       }
     }
   }
-  __syncthreads();
-  fft_lds<G0, true, 64>(smem + col * CS, stw, lane);
-  float2* Tb = T + ((int64_t)cf * nstr + s) * S0 * Tc;
+}
+
+// Adjoint strip pass: fixed-point spreading into the LDS strip, inverse column
+// FFT, crop to the S0 image rows.  Grid (nstrips, nbatch), STRIP_THREADS threads.
+template <int G0, int WT>
+__global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
+    const float2* __restrict__ y, float2* __restrict__ T, const int2* __restrict__ em,
+    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt) {
+  constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
+  constexpr int WM = WT ? WT : MRH_WMAX;
+  constexpr int NPT = (TC * CS + NTH - 1) / NTH;
+  const int W = WT ? WT : Wrt;
+  __shared__ unsigned long long acc[2 * TC * CS];
+  __shared__ float2 stw[G0];
+  __shared__ float wmax[NTH / 64];
+  float2* fbuf = reinterpret_cast<float2*>(acc);
+  const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
+  const int s = blockIdx.x, bb = blockIdx.y, nstr = gridDim.x;
+  const int b = bb / like, l = bb - b * like;
+  for (int i = tid; i < G0; i += NTH) stw[i] = tw0[i];
+  const int e0 = soff[b * nstr + s], e1 = soff[b * nstr + s + 1];
+  const int64_t Kb = koff[b + 1] - koff[b];
+  const float2* ybase = y + (int64_t)C * (koff[b] * like + (int64_t)l * Kb);
+  const int c0 = s * TC;
+  Foot<WM> F[STRIP_EPT];
+#pragma unroll
+  for (int q = 0; q < STRIP_EPT; ++q) {
+    const int e = e0 + tid + q * NTH;
+    if (e < e1) load_foot<WM>(F[q], em[e], ef[e], W, c0, G1, kb);
+    else F[q].k = -1;
+  }
   const int h0 = S0 / 2;
-  for (int e = tid; e < S0 * Tc; e += blockDim.x) {
-    const int ix = e / Tc, j = e - ix * Tc;
-    const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
-    Tb[e] = smem[j * CS + PIDX(r)];
+  const int64_t cstride = (int64_t)nstr * S0 * TC;
+  float2* Tbb = T + ((int64_t)bb * C * nstr + s) * S0 * TC;
+  for (int c = 0; c < C; ++c) {
+    const float2* yc = ybase + (int64_t)c * Kb;
+    for (int i = tid; i < 2 * TC * CS; i += NTH) acc[i] = 0ull;
+    // per (strip, coil) scale from the largest weighted sample component
+    float2 v[STRIP_EPT];
+    float m = 0.f;
+#pragma unroll
+    for (int q = 0; q < STRIP_EPT; ++q) {
+      v[q] = make_float2(0.f, 0.f);
+      if (F[q].k >= 0) {
+        v[q] = cscale(yc[F[q].k], F[q].wAH);
+        m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
+      }
+    }
+    for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
+      const float2 vo = cscale(yc[em[e].y], ef[e].w);
+      m = fmaxf(m, fmaxf(fabsf(vo.x), fabsf(vo.y)));
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    if (lane == 0) wmax[col] = m;
+    __syncthreads();
+    float vmax = 0.f;
+#pragma unroll
+    for (int w = 0; w < NTH / 64; ++w) vmax = fmaxf(vmax, wmax[w]);
+    int ex;
+    frexpf(vmax * (float)(e1 - e0), &ex);   // bound < 2^ex
+    const int sc = 61 - ex;
+    if (vmax > 0.f) {
+#pragma unroll
+      for (int q = 0; q < STRIP_EPT; ++q)
+        if (F[q].k >= 0) spread_one<WM, CS>(F[q], v[q], acc, W, c0, G0, G1, sc);
+      for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
+        Foot<WM> Fo;
+        load_foot<WM>(Fo, em[e], ef[e], W, c0, G1, kb);
+        spread_one<WM, CS>(Fo, cscale(yc[Fo.k], Fo.wAH), acc, W, c0, G0, G1, sc);
+      }
+    }
+    __syncthreads();
+    // fixed point -> float, in place (the float view overlaps points already read)
+    float2 cv[NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; ++t) {
+      const int p = tid + t * NTH;
+      cv[t] = make_float2(0.f, 0.f);
+      if (p < TC * CS) cv[t] = make_float2(from_fixed(acc[2 * p], sc), from_fixed(acc[2 * p + 1], sc));
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < NPT; ++t) {
+      const int p = tid + t * NTH;
+      if (p < TC * CS) fbuf[p] = cv[t];
+    }
+    __syncthreads();
+    fft_lds<G0, true, 64>(fbuf + col * CS, stw, lane);
+    float2* Tb = Tbb + c * cstride;
+    for (int e = tid; e < S0 * TC; e += NTH) {
+      const int ix = e / TC, j = e - ix * TC;
+      const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
+      Tb[e] = fbuf[j * CS + PIDX(r)];
+    }
+    __syncthreads();
   }
 }
 
@@ -559,27 +691,21 @@ static int launch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStream
   return MRH_OK;
 }
 
-template <int G, int NP>
+template <int G, int WT>
 static int launch_strips(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
                          int nb, int like, hipStream_t st) {
-  size_t lds = ((size_t)p->Tc * (fft_padded(G) + 1) + G) * sizeof(float2);
-  dim3 grid(p->nstrips, nb * p->C), block(64 * p->Tc);
-  if (adjoint) {
-    static bool attr = false;
-    if (!attr) { (void)hipFuncSetAttribute((const void*)k_adj_strips<G, NP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); attr = true; }
-    hipLaunchKernelGGL((k_adj_strips<G, NP>), grid, block, lds, st, yin, p->d_T, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->Tc, p->W);
-  } else {
-    static bool attr = false;
-    if (!attr) { (void)hipFuncSetAttribute((const void*)k_fwd_strips<G, NP>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); attr = true; }
-    hipLaunchKernelGGL((k_fwd_strips<G, NP>), grid, block, lds, st, p->d_T, yout, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->Tc, p->W);
-  }
+  dim3 grid(p->nstrips, nb), block(STRIP_THREADS);
+  if (adjoint)
+    hipLaunchKernelGGL((k_adj_strips<G, WT>), grid, block, 0, st, yin, p->d_T, p->d_em, p->d_ef,
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W);
+  else
+    hipLaunchKernelGGL((k_fwd_strips<G, WT>), grid, block, 0, st, p->d_T, yout, p->d_em, p->d_ef,
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
 
-#define MRH_FOR_EACH_G(X) X(32) X(64) X(96) X(128) X(192) X(256) X(320) X(384) X(512) X(640) X(768) X(1024)
+#define MRH_FOR_EACH_G(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
 static int dispatch_rows_fwd(const mrh_nufft_plan* p, const float2* x, float2* y, int nb, int like,
                              int xc, hipStream_t st) {
@@ -602,11 +728,11 @@ static int dispatch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStre
   return MRH_EUNSUP;
 }
 
-template <int NP>
-static int dispatch_strips_np(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
-                              int nb, int like, hipStream_t st) {
+template <int WT>
+static int dispatch_strips_w(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
+                             int nb, int like, hipStream_t st) {
   switch (p->G0) {
-#define X(G) case G: return launch_strips<G, NP>(p, adj, yin, yout, nb, like, st);
+#define X(G) case G: return launch_strips<G, WT>(p, adj, yin, yout, nb, like, st);
     MRH_FOR_EACH_G(X)
 #undef X
   }
@@ -616,10 +742,11 @@ static int dispatch_strips_np(const mrh_nufft_plan* p, bool adj, const float2* y
 
 static int dispatch_strips(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
                            int nb, int like, hipStream_t st) {
-  switch (p->np) {
-    case 16: return dispatch_strips_np<16>(p, adj, yin, yout, nb, like, st);
-    case 24: return dispatch_strips_np<24>(p, adj, yin, yout, nb, like, st);
-    default: return dispatch_strips_np<32>(p, adj, yin, yout, nb, like, st);
+  switch (p->W) {
+    case 4: return dispatch_strips_w<4>(p, adj, yin, yout, nb, like, st);
+    case 6: return dispatch_strips_w<6>(p, adj, yin, yout, nb, like, st);
+    case 8: return dispatch_strips_w<8>(p, adj, yin, yout, nb, like, st);
+    default: return dispatch_strips_w<0>(p, adj, yin, yout, nb, like, st);
   }
 }
 

@@ -127,8 +127,8 @@ class KBNufft:
             my, wy = footprint(v, self.W, self.beta)
             ix = (mx[:, None] + np.arange(self.W)[None, :]) % self.G0
             iy = (my[:, None] + np.arange(self.W)[None, :]) % self.G1
-            flat = (ix[:, :, None] * self.G1 + iy[:, None, :]).reshape(len(u), -1)
-            wgt = (wx[:, :, None] * wy[:, None, :]).reshape(len(u), -1).astype(self.rdt)
+            flat = (ix[:, :, None] * self.G1 + iy[:, None, :]).reshape(len(u), self.W * self.W)
+            wgt = (wx[:, :, None] * wy[:, None, :]).reshape(len(u), self.W * self.W).astype(self.rdt)
             self._fp.append((flat, wgt))
         r0 = (np.arange(self.S0) - self.S0 // 2) % self.G0
         r1 = (np.arange(self.S1) - self.S1 // 2) % self.G1
