@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance NLCG run")
     ap.add_argument("--tol-iters", type=int, default=150)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (reference for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of "
@@ -116,6 +118,15 @@ def main():
     from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
 
     dev = torch.device("cuda:0")
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_each_way": n * 4}), flush=True)
+        return
     print("bench: generating the GRASP workload", file=sys.stderr, flush=True)
     wl = synth.config_c2()
     be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=wl.W, osf=wl.osf, strip_cols=args.strip_cols)

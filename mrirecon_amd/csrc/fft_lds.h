@@ -104,8 +104,23 @@ __host__ __device__ constexpr bool fft_supported(int n) {
   return true;
 }
 
+// Synchronisation between the read and write halves of a pass: the whole
+// block (the transform's lanes span several wavefronts) or one wavefront (the
+// transform is owned by a single wavefront, whose LDS operations complete in
+// order; the fences order them for the compiler and the memory model).
+template <bool WAVE>
+__device__ __forceinline__ void fft_sync() {
+  if (WAVE) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  } else {
+    __syncthreads();
+  }
+}
+
 // One Stockham pass.  `tw` holds W_N^k = exp(-2 pi i k / N), k < N (LDS).
-template <int N, int R, int NS, bool INV, int NTHR>
+template <int N, int R, int NS, bool INV, int NTHR, bool WAVE = false>
 __device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int t) {
   constexpr int NBF = N / R;
   constexpr int NB = (NBF + NTHR - 1) / NTHR;
@@ -129,7 +144,7 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int t) {
       DFT<R, INV>::run(v[q]);
     }
   }
-  __syncthreads();
+  fft_sync<WAVE>();
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
     const int j = t + q * NTHR;
@@ -140,17 +155,17 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int t) {
       for (int r = 0; r < R; ++r) buf[PIDX(base + r * NS)] = v[q][r];
     }
   }
-  __syncthreads();
+  fft_sync<WAVE>();
 }
 
-template <int N, int NS, bool INV, int NTHR>
+template <int N, int NS, bool INV, int NTHR, bool WAVE>
 struct FFTStages {
   __device__ __forceinline__ static void run(float2* buf, const float2* tw, int t) {
     if constexpr (NS < N) {
       constexpr int R = fft_next_radix(N / NS);
       static_assert(R != 0, "unsupported FFT size");
-      fft_pass<N, R, NS, INV, NTHR>(buf, tw, t);
-      FFTStages<N, NS * R, INV, NTHR>::run(buf, tw, t);
+      fft_pass<N, R, NS, INV, NTHR, WAVE>(buf, tw, t);
+      FFTStages<N, NS * R, INV, NTHR, WAVE>::run(buf, tw, t);
     }
   }
 };
@@ -159,5 +174,15 @@ struct FFTStages {
 // call it (it contains barriers).  Forward: exp(-i), inverse: exp(+i), unscaled.
 template <int N, bool INV, int NTHR>
 __device__ __forceinline__ void fft_lds(float2* buf, const float2* tw, int t) {
-  FFTStages<N, 1, INV, NTHR>::run(buf, tw, t);
+  FFTStages<N, 1, INV, NTHR, false>::run(buf, tw, t);
+}
+
+// Same transform owned by one wavefront (lane = t in [0, 64)): no block
+// barriers, so the other wavefronts of the block are never held up.  The
+// caller must make the input visible to the wavefront and must not touch
+// `buf` from other wavefronts during the call.
+template <int N, bool INV>
+__device__ __forceinline__ void fft_lds_wave(float2* buf, const float2* tw, int lane) {
+  fft_sync<true>();
+  FFTStages<N, 1, INV, 64, true>::run(buf, tw, lane);
 }

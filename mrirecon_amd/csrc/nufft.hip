@@ -299,7 +299,9 @@ extern "C" int mrh_nufft_plan_info(const mrh_nufft_plan* p, mrh_nufft_info* info
 // ---------------------------------------------------------------- kernels
 #define ROWS_PER_BLOCK 4
 
-// Forward row pass.  Grid (ceil(S0/4), nbatch*C), 256 threads: one wavefront per image row.
+// Forward row pass.  Grid (ceil(S0/4), nbatch), 256 threads: one wavefront per
+// image row, looping over the coils (the deapodised image row is loaded once;
+// the next coil's map row is fetched while the current row is transformed).
 template <int G1>
 __global__ __launch_bounds__(256) void k_fwd_rows(
     const float2* __restrict__ x, const float2* __restrict__ maps, const float* __restrict__ dx,
@@ -307,48 +309,68 @@ __global__ __launch_bounds__(256) void k_fwd_rows(
     float2* __restrict__ y, const int64_t* __restrict__ koff, int S0, int S1, int C, int like,
     int Tc, int x_has_coils) {
   constexpr int LP = fft_padded(G1);
+  constexpr int NQ = (G1 + 63) / 64;
   __shared__ float2 sbuf[ROWS_PER_BLOCK * LP];
   __shared__ float2 stw[G1];
   const int tid = threadIdx.x, lane = tid & 63, wrow = tid >> 6;
-  const int cf = blockIdx.y, bb = cf / C, c = cf - bb * C;
+  const int bb = blockIdx.y;
   const int ix = blockIdx.x * ROWS_PER_BLOCK + wrow;
   for (int i = tid; i < G1; i += blockDim.x) stw[i] = tw1[i];
-  {  // clear this block's share of the coil-frame's k-space (the strip pass accumulates into it)
+  {  // clear this block's share of the frame's k-space (the strip pass accumulates into it)
     const int b = bb / like, l = bb - b * like;
     const int64_t Kb = koff[b + 1] - koff[b];
-    const int64_t base = (int64_t)C * (koff[b] * like + (int64_t)l * Kb) + (int64_t)c * Kb;
-    const int64_t lo = Kb * blockIdx.x / gridDim.x, hi = Kb * (blockIdx.x + 1) / gridDim.x;
+    const int64_t base = (int64_t)C * (koff[b] * like + (int64_t)l * Kb);
+    const int64_t n = (int64_t)C * Kb;
+    const int64_t lo = n * blockIdx.x / gridDim.x, hi = n * (blockIdx.x + 1) / gridDim.x;
     for (int64_t k = lo + tid; k < hi; k += blockDim.x) y[base + k] = make_float2(0.f, 0.f);
   }
   float2* buf = sbuf + wrow * LP;
   const int h1 = S1 / 2;
-  if (ix < S0) {
-    const float2* xr = x + ((int64_t)(x_has_coils ? cf : bb) * S0 + ix) * S1;
-    const float2* mr = maps + ((int64_t)c * S0 + ix) * S1;
-    const float dxi = dx[ix];
-    for (int pp = lane; pp < G1; pp += 64) {
-      float2 v = make_float2(0.f, 0.f);
-      const bool lo = pp < S1 - h1, hi = pp >= G1 - h1;
-      if (lo || hi) {
-        const int iy = lo ? pp + h1 : pp - G1 + h1;
-        float2 xv = xr[iy];
-        if (!x_has_coils) xv = cmul(xv, mr[iy]);
-        v = cscale(xv, dxi * dy[iy]);
-      }
-      buf[PIDX(pp)] = v;
+  const bool row_ok = ix < S0;
+  // grid column pp holds image column iy with (iy - S1/2) mod G1 == pp
+  float2 basev[NQ], fac[NQ];
+  int iyq[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) {
+    const int pp = lane + 64 * q;
+    const bool lo = pp < S1 - h1, hi = pp >= G1 - h1 && pp < G1;
+    iyq[q] = (row_ok && (lo || hi)) ? (lo ? pp + h1 : pp - G1 + h1) : -1;
+    basev[q] = make_float2(0.f, 0.f);
+    if (iyq[q] >= 0) {
+      const float dd = dx[ix] * dy[iyq[q]];
+      basev[q] = x_has_coils ? make_float2(dd, 0.f)
+                             : cscale(x[((int64_t)bb * S0 + ix) * S1 + iyq[q]], dd);
     }
-  } else {
-    for (int pp = lane; pp < G1; pp += 64) buf[PIDX(pp)] = make_float2(0.f, 0.f);
   }
+  auto load_fac = [&](int c) {
+    const float2* src = x_has_coils ? x + ((int64_t)(bb * C + c) * S0 + ix) * S1
+                                    : maps + ((int64_t)c * S0 + ix) * S1;
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) fac[q] = iyq[q] >= 0 ? src[iyq[q]] : make_float2(0.f, 0.f);
+  };
+  load_fac(0);
   __syncthreads();
-  fft_lds<G1, false, 64>(buf, stw, lane);
-  if (ix < S0) {
-    const int nstr = G1 / Tc;
-    float2* Tcf = T + (int64_t)cf * nstr * S0 * Tc;
-    for (int pp = lane; pp < G1; pp += 64) {
-      const int s = pp / Tc, j = pp - s * Tc;
-      Tcf[((int64_t)s * S0 + ix) * Tc + j] = buf[PIDX(pp)];
+  const int nstr = G1 / Tc;
+  for (int c = 0; c < C; ++c) {
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) {
+      const int pp = lane + 64 * q;
+      if (pp < G1) buf[PIDX(pp)] = cmul(basev[q], fac[q]);
     }
+    if (c + 1 < C) load_fac(c + 1);
+    fft_lds_wave<G1, false>(buf, stw, lane);
+    if (row_ok) {
+      float2* Tcf = T + (int64_t)(bb * C + c) * nstr * S0 * Tc;
+#pragma unroll
+      for (int q = 0; q < NQ; ++q) {
+        const int pp = lane + 64 * q;
+        if (pp < G1) {
+          const int st = pp / Tc, j = pp - st * Tc;
+          Tcf[((int64_t)st * S0 + ix) * Tc + j] = buf[PIDX(pp)];
+        }
+      }
+    }
+    fft_sync<true>();
   }
 }
 
@@ -421,7 +443,7 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
   constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
-  __shared__ float2 smem[TC * CS];
+  __shared__ float2 smem2[2 * TC * CS];   // double buffer: coil c fills one half while c-1 is read
   __shared__ float2 stw[G0];
   const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
   const int s = blockIdx.x, bb = blockIdx.y, nstr = gridDim.x;
@@ -441,17 +463,43 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
   const int h0 = S0 / 2;
   const int64_t cstride = (int64_t)nstr * S0 * TC;
   const float2* Tbb = T + ((int64_t)bb * C * nstr + s) * S0 * TC;
-  for (int c = 0; c < C; ++c) {
+  // the S0 x TC data block of T for coil c is fetched into registers one coil ahead
+  constexpr int DPT = ((G0 / 2) * TC + NTH - 1) / NTH;
+  const int ndata = S0 * TC;
+  float2 pre[DPT];
+  auto fetch = [&](int c) {
     const float2* Tb = Tbb + c * cstride;
-    for (int e = tid; e < G0 * TC; e += NTH) {
-      const int r = e / TC, j = e - r * TC;
-      float2 v = make_float2(0.f, 0.f);
-      if (r < S0 - h0) v = Tb[(r + h0) * TC + j];
-      else if (r >= G0 - h0) v = Tb[(r - G0 + h0) * TC + j];
-      smem[j * CS + PIDX(r)] = v;
+#pragma unroll
+    for (int t = 0; t < DPT; ++t) {
+      const int e = tid + t * NTH;
+      pre[t] = e < ndata ? Tb[e] : make_float2(0.f, 0.f);
     }
-    __syncthreads();
-    fft_lds<G0, false, 64>(smem + col * CS, stw, lane);
+  };
+  fetch(0);
+  for (int c = 0; c < C; ++c) {
+    float2* smem = smem2 + (c & 1) * TC * CS;
+#pragma unroll
+    for (int t = 0; t < DPT; ++t) {
+      const int e = tid + t * NTH;
+      if (e < ndata) {
+        const int ix = e / TC, j = e - ix * TC;
+        const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
+        smem[j * CS + PIDX(r)] = pre[t];
+      }
+    }
+    for (int e = DPT * NTH + tid; e < ndata; e += NTH) {   // images wider than G0/2 (osf < 2)
+      const int ix = e / TC, j = e - ix * TC;
+      const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
+      smem[j * CS + PIDX(r)] = Tbb[c * cstride + e];
+    }
+    for (int e = tid; e < (G0 - S0) * TC; e += NTH) {      // zero-padded rows
+      const int r = S0 - h0 + e / TC, j = e - (e / TC) * TC;
+      smem[j * CS + PIDX(r)] = make_float2(0.f, 0.f);
+    }
+    if (c + 1 < C) fetch(c + 1);
+    __syncthreads();                       // strip filled (and every thread is past coil c-1)
+    fft_lds_wave<G0, false>(smem + col * CS, stw, lane);
+    __syncthreads();                       // all columns transformed
     float2* yc = ybase + (int64_t)c * Kb;
 #pragma unroll
     for (int q = 0; q < STRIP_EPT; ++q) {
@@ -468,21 +516,19 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
       unsafeAtomicAdd(&yc[Fo.k].x, a.x * Fo.wA);
       unsafeAtomicAdd(&yc[Fo.k].y, a.y * Fo.wA);
     }
-    __syncthreads();
   }
 }
 
-// Exact fixed-point accumulation for the adjoint: contributions are scaled by
-// 2^sc (sc chosen per strip and coil so that the strip's largest possible sum
-// stays below 2^61) and added as two's-complement int64 with ds_add_u64, which
-// runs at full LDS rate on gfx950 (float LDS atomics do not).  The sum is exact
-// and independent of the order of the adds; converting back costs one rounding.
-__device__ __forceinline__ unsigned long long to_fixed(float v, int sc) {
-  const float a = ldexpf(fabsf(v), sc);                      // < 2^61, exact scaling
-  const float hf = floorf(a * 2.3283064365386963e-10f);     // a / 2^32 (exact), integer part
-  const float lf = fmaf(-hf, 4294967296.0f, a);             // a mod 2^32, exact, in [0, 2^32)
-  unsigned long long u = ((unsigned long long)(unsigned)hf << 32) | (unsigned)lf;
-  return v < 0.f ? (~u + 1ull) : u;
+// Deterministic fixed-point accumulation for the adjoint.  Per strip and coil
+// the weighted samples are scaled by 2^sc so that the largest component is
+// below 2^30; every contribution is rounded once to an integer (a quantum of
+// 2^-30 of that largest component, far below FP32 rounding of the large
+// terms) and summed exactly as int64 with ds_add_u64, which runs at full LDS
+// rate on gfx950 where float LDS atomics do not.  The sum is independent of
+// the order of the adds; converting back costs one rounding.
+__device__ __forceinline__ unsigned long long fixed_of(float scaled) {
+  const int q = (int)rintf(scaled);
+  return (unsigned long long)(long long)q;
 }
 
 __device__ __forceinline__ float from_fixed(unsigned long long u, int sc) {
@@ -490,24 +536,25 @@ __device__ __forceinline__ float from_fixed(unsigned long long u, int sc) {
   return (float)ldexp((double)q, -sc);
 }
 
+// vs: the weighted sample already scaled by 2^sc.
 template <int WM, int CS>
-__device__ __forceinline__ void spread_one(const Foot<WM>& F, float2 v, unsigned long long* acc, int W, int c0,
-                                           int G0, int G1, int sc) {
+__device__ __forceinline__ void spread_one(const Foot<WM>& F, float2 vs, unsigned long long* acc, int W, int c0,
+                                           int G0, int G1) {
 #pragma unroll
   for (int j = 0; j < WM; ++j) {
     if (F.wy[j] != 0.f) {
       int cc = F.my + j;
       if (cc >= G1) cc -= G1;
       unsigned long long* colp = acc + 2 * (cc - c0) * CS;
-      const float2 vj = cscale(v, F.wy[j]);
+      const float2 vj = cscale(vs, F.wy[j]);
 #pragma unroll
       for (int i = 0; i < WM; ++i) {
         if (i < W) {
           int rr = F.r0 + i;
           if (rr >= G0) rr -= G0;
           unsigned long long* dst = colp + 2 * PIDX(rr);
-          atomicAdd(dst, to_fixed(vj.x * F.wx[i], sc));
-          atomicAdd(dst + 1, to_fixed(vj.y * F.wx[i], sc));
+          atomicAdd(dst, fixed_of(vj.x * F.wx[i]));
+          atomicAdd(dst + 1, fixed_of(vj.y * F.wx[i]));
         }
       }
     }
@@ -523,7 +570,6 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
     const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt) {
   constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
-  constexpr int NPT = (TC * CS + NTH - 1) / NTH;
   const int W = WT ? WT : Wrt;
   __shared__ unsigned long long acc[2 * TC * CS];
   __shared__ float2 stw[G0];
@@ -547,20 +593,26 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
   const int h0 = S0 / 2;
   const int64_t cstride = (int64_t)nstr * S0 * TC;
   float2* Tbb = T + ((int64_t)bb * C * nstr + s) * S0 * TC;
+  // weighted samples of the cached footprints, fetched one coil ahead
+  float2 vn[STRIP_EPT];
+  auto fetch = [&](int c) {
+    const float2* yc = ybase + (int64_t)c * Kb;
+#pragma unroll
+    for (int q = 0; q < STRIP_EPT; ++q)
+      vn[q] = F[q].k >= 0 ? cscale(yc[F[q].k], F[q].wAH) : make_float2(0.f, 0.f);
+  };
+  fetch(0);
+  for (int i = tid; i < 2 * TC * CS; i += NTH) acc[i] = 0ull;
   for (int c = 0; c < C; ++c) {
     const float2* yc = ybase + (int64_t)c * Kb;
-    for (int i = tid; i < 2 * TC * CS; i += NTH) acc[i] = 0ull;
-    // per (strip, coil) scale from the largest weighted sample component
     float2 v[STRIP_EPT];
     float m = 0.f;
 #pragma unroll
     for (int q = 0; q < STRIP_EPT; ++q) {
-      v[q] = make_float2(0.f, 0.f);
-      if (F[q].k >= 0) {
-        v[q] = cscale(yc[F[q].k], F[q].wAH);
-        m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
-      }
+      v[q] = vn[q];
+      m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
     }
+    if (c + 1 < C) fetch(c + 1);
     for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
       const float2 vo = cscale(yc[em[e].y], ef[e].w);
       m = fmaxf(m, fmaxf(fabsf(vo.x), fabsf(vo.y)));
@@ -573,47 +625,57 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
 #pragma unroll
     for (int w = 0; w < NTH / 64; ++w) vmax = fmaxf(vmax, wmax[w]);
     int ex;
-    frexpf(vmax * (float)(e1 - e0), &ex);   // bound < 2^ex
-    const int sc = 61 - ex;
+    frexpf(vmax, &ex);   // vmax < 2^ex
+    const int sc = 30 - ex;
     if (vmax > 0.f) {
 #pragma unroll
       for (int q = 0; q < STRIP_EPT; ++q)
-        if (F[q].k >= 0) spread_one<WM, CS>(F[q], v[q], acc, W, c0, G0, G1, sc);
+        if (F[q].k >= 0)
+          spread_one<WM, CS>(F[q], make_float2(ldexpf(v[q].x, sc), ldexpf(v[q].y, sc)), acc, W, c0, G0, G1);
       for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
         Foot<WM> Fo;
         load_foot<WM>(Fo, em[e], ef[e], W, c0, G1, kb);
-        spread_one<WM, CS>(Fo, cscale(yc[Fo.k], Fo.wAH), acc, W, c0, G0, G1, sc);
+        const float2 vo = cscale(yc[Fo.k], Fo.wAH);
+        spread_one<WM, CS>(Fo, make_float2(ldexpf(vo.x, sc), ldexpf(vo.y, sc)), acc, W, c0, G0, G1);
       }
     }
-    __syncthreads();
-    // fixed point -> float, in place (the float view overlaps points already read)
-    float2 cv[NPT];
+    __syncthreads();                       // every contribution is in
+    // fixed point -> float by the wavefront that owns the column: column col's
+    // float view lies inside its own int64 region, so the conversion is local
+    {
+      unsigned long long* ac = acc + 2 * col * CS;
+      float2* fc = fbuf + 2 * col * CS;
+      constexpr int NPC = (CS + 63) / 64;
+      float2 cv[NPC];
 #pragma unroll
-    for (int t = 0; t < NPT; ++t) {
-      const int p = tid + t * NTH;
-      cv[t] = make_float2(0.f, 0.f);
-      if (p < TC * CS) cv[t] = make_float2(from_fixed(acc[2 * p], sc), from_fixed(acc[2 * p + 1], sc));
-    }
-    __syncthreads();
+      for (int t = 0; t < NPC; ++t) {
+        const int p = lane + 64 * t;
+        cv[t] = p < CS ? make_float2(from_fixed(ac[2 * p], sc), from_fixed(ac[2 * p + 1], sc))
+                       : make_float2(0.f, 0.f);
+      }
+      fft_sync<true>();
 #pragma unroll
-    for (int t = 0; t < NPT; ++t) {
-      const int p = tid + t * NTH;
-      if (p < TC * CS) fbuf[p] = cv[t];
+      for (int t = 0; t < NPC; ++t) {
+        const int p = lane + 64 * t;
+        if (p < CS) fc[p] = cv[t];
+      }
+      fft_lds_wave<G0, true>(fc, stw, lane);
     }
-    __syncthreads();
-    fft_lds<G0, true, 64>(fbuf + col * CS, stw, lane);
+    __syncthreads();                       // all columns transformed
     float2* Tb = Tbb + c * cstride;
     for (int e = tid; e < S0 * TC; e += NTH) {
       const int ix = e / TC, j = e - ix * TC;
       const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
-      Tb[e] = fbuf[j * CS + PIDX(r)];
+      Tb[e] = fbuf[2 * j * CS + PIDX(r)];
     }
-    __syncthreads();
+    __syncthreads();                       // strip read out before the next coil clears it
+    for (int i = tid; i < 2 * TC * CS; i += NTH) acc[i] = 0ull;
   }
 }
 
 // Adjoint row pass: per image row, inverse row FFT of every coil, crop,
-// conj(map) combine in registers, deapodise.  Grid (ceil(S0/4), nbatch).
+// conj(map) combine in registers, deapodise.  The next coil's row of T and of
+// the map are fetched while the current row is transformed.  Grid (ceil(S0/4), nbatch).
 template <int G1>
 __global__ __launch_bounds__(256) void k_adj_rows(
     const float2* __restrict__ T, const float2* __restrict__ maps, const float* __restrict__ dx,
@@ -626,41 +688,51 @@ __global__ __launch_bounds__(256) void k_adj_rows(
   const int tid = threadIdx.x, lane = tid & 63, wrow = tid >> 6;
   const int bb = blockIdx.y;
   const int ix = blockIdx.x * ROWS_PER_BLOCK + wrow;
+  const bool row_ok = ix < S0;
   for (int i = tid; i < G1; i += blockDim.x) stw[i] = tw1[i];
   float2* buf = sbuf + wrow * LP;
   const int nstr = G1 / Tc, h1 = S1 / 2;
-  float2 acc[NA];
+  float2 acc[NA], tp[NA], mp[NA];
 #pragma unroll
   for (int q = 0; q < NA; ++q) acc[q] = make_float2(0.f, 0.f);
-  for (int c = 0; c < C; ++c) {
-    const int cf = bb * C + c;
-    if (ix < S0) {
-      const float2* Tcf = T + (int64_t)cf * nstr * S0 * Tc;
-      for (int pp = lane; pp < G1; pp += 64) {
-        const int s = pp / Tc, j = pp - s * Tc;
-        buf[PIDX(pp)] = Tcf[((int64_t)s * S0 + ix) * Tc + j];
-      }
-    } else {
-      for (int pp = lane; pp < G1; pp += 64) buf[PIDX(pp)] = make_float2(0.f, 0.f);
-    }
-    __syncthreads();
-    fft_lds<G1, true, 64>(buf, stw, lane);
-    if (ix < S0) {
-      const float2* mr = maps + ((int64_t)c * S0 + ix) * S1;
+  auto load = [&](int c) {
+    const float2* Tcf = T + (int64_t)(bb * C + c) * nstr * S0 * Tc;
+    const float2* mr = maps + ((int64_t)c * S0 + ix) * S1;
 #pragma unroll
-      for (int q = 0; q < NA; ++q) {
-        const int iy = lane + 64 * q;
-        if (iy < S1) {
-          const int pp = iy >= h1 ? iy - h1 : iy - h1 + G1;
-          const float2 g = buf[PIDX(pp)];
-          const float2 mv = mr[iy];
-          acc[q] = cadd(acc[q], cmulc(g, mv));
-        }
+    for (int q = 0; q < NA; ++q) {
+      const int pp = lane + 64 * q;
+      tp[q] = make_float2(0.f, 0.f);
+      mp[q] = make_float2(0.f, 0.f);
+      if (row_ok && pp < G1) {
+        const int st = pp / Tc, j = pp - st * Tc;
+        tp[q] = Tcf[((int64_t)st * S0 + ix) * Tc + j];
+      }
+      if (row_ok && pp < S1) mp[q] = mr[pp];
+    }
+  };
+  load(0);
+  __syncthreads();
+  for (int c = 0; c < C; ++c) {
+    float2 mc[NA];
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+      const int pp = lane + 64 * q;
+      if (pp < G1) buf[PIDX(pp)] = tp[q];
+      mc[q] = mp[q];
+    }
+    if (c + 1 < C) load(c + 1);
+    fft_lds_wave<G1, true>(buf, stw, lane);
+#pragma unroll
+    for (int q = 0; q < NA; ++q) {
+      const int iy = lane + 64 * q;
+      if (row_ok && iy < S1) {
+        const int pp = iy >= h1 ? iy - h1 : iy - h1 + G1;
+        acc[q] = cadd(acc[q], cmulc(buf[PIDX(pp)], mc[q]));
       }
     }
-    __syncthreads();
+    fft_sync<true>();
   }
-  if (ix < S0) {
+  if (row_ok) {
     const float dxi = dx[ix];
     float2* xr = x + ((int64_t)bb * S0 + ix) * S1;
 #pragma unroll
@@ -675,7 +747,7 @@ __global__ __launch_bounds__(256) void k_adj_rows(
 template <int G>
 static int launch_rows_fwd(const mrh_nufft_plan* p, const float2* x, float2* y, int nb, int like,
                            int x_has_coils, hipStream_t st) {
-  dim3 grid((p->S0 + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK, nb * p->C);
+  dim3 grid((p->S0 + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK, nb);
   hipLaunchKernelGGL(k_fwd_rows<G>, grid, dim3(256), 0, st, x, p->d_maps, p->d_dx, p->d_dy,
                      p->d_tw1, p->d_T, y, p->d_koff, p->S0, p->S1, p->C, like, p->Tc, x_has_coils);
   MRH_CHECK_LAUNCH();
