@@ -164,6 +164,11 @@ typedef struct {
   int ls_max_iter;          /* trials (20)                                      */
   double rel_tol;           /* stop when |f_k - f_{k-1}| <= rel_tol |f_{k-1}|; 0 = never */
   int max_hist;             /* iterations of history kept on the device          */
+  double restart_c;         /* < 0: reference behaviour (a non-descent direction
+                               stops the problem).  >= 0: restart with beta = 0
+                               whenever the new direction -g + beta d would have
+                               Re<g, d_new> > -restart_c |g|^2 (not a reference
+                               rule; the descent safeguard of PR+ with restarts) */
 } mrh_cg_desc;
 
 /* Per-problem, per-iteration record copied out by mrh_cg_read_history. */
@@ -222,6 +227,62 @@ int mrh_cg_direction(mrh_cg* eng, void* d, const void* g, void* g_prev, int prim
 int mrh_cg_read_status(mrh_cg* eng, int* status, int* iters, void* stream);
 /* Copy history (host mrh_cg_hist[n_iter * P], iteration-major) to host. */
 int mrh_cg_read_history(mrh_cg* eng, mrh_cg_hist* hist, int n_iter, void* stream);
+
+/* ------------------------------------------------------------------ U-Net warm start
+ *
+ * No reference code: the reference's only warm-start hook is init_mode="keep",
+ * which keeps a caller-filled x as the initial point (graspcg/solvers/cg.py:48,142).
+ * These kernels train and run the 2D+t U-Net that fills it.
+ *
+ * Activations are BF16 (16-bit storage, pointers typed void*) channels-last
+ * (nvol * F, H, W, C).  Forward weights are BF16 [ntap][Cout_pad][Cin]; the
+ * backward-data copy is BF16 [ntap][Cin][Cout] with taps in reverse order.
+ * Weight and bias gradients, master weights and Adam moments are FP32.
+ * Temporal taps never cross a volume's first or last frame; every tap set
+ * zero-pads at the borders. */
+#define MRH_UNET_TAP_1X1 0   /* one tap */
+#define MRH_UNET_TAP_3X3 1   /* 9 spatial taps, t = 3 (dy + 1) + (dx + 1) */
+#define MRH_UNET_TAP_T3 2    /* 3 temporal taps, t = df + 1 */
+
+/* out[p][co] = act(bias[co] + sum_(t, ci) w[t][co][ci] x[p + off(t)][ci]),
+ * act = ReLU if relu != 0.  Cin % 8 == 0, Cout <= Cout_pad, Cout_pad % 32 == 0,
+ * bias (FP32, may be NULL) holds Cout values.  With w = the backward-data copy,
+ * x = dL/dout (Cin := Cout), Cout := Cin, bias NULL and relu 0 it computes dL/dx. */
+int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias, void* out, int kind, int nvol,
+                      int F, int H, int W, int Cin, int Cout, int Cout_pad, int relu, void* stream);
+/* dw[t][co][ci] += sum_p gout[p][co] x[p + off(t)][ci]; db[co] += sum_p gout[p][co]
+ * (db may be NULL).  FP32 atomics: the result is reproducible to rounding only. */
+int mrh_unet_conv_bwd_weight(const void* gout, const void* x, float* dw, float* db, int kind, int nvol,
+                             int F, int H, int W, int Cin, int Cout, int Cout_pad, void* stream);
+/* g[i] = 0 where y[i] <= 0 (y = the ReLU output), n elements. */
+int mrh_unet_relu_bwd(void* g, const void* y, int64_t n, void* stream);
+/* 2 x 2 average pooling (nf, H, W, C) -> (nf, H/2, W/2, C) and its adjoint
+ * gx = pool^T gy (+ add, which may be NULL). */
+int mrh_unet_pool2(const void* x, void* y, int nf, int H, int W, int C, void* stream);
+int mrh_unet_pool2_bwd(const void* gy, const void* add, void* gx, int nf, int H, int W, int C, void* stream);
+/* dst (nf, H, W, C1 + C2) = concat(nearest 2x upsampling of src (nf, H/2, W/2, C1),
+ * skip (nf, H, W, C2)), and its adjoint (gsrc = up^T gdst[..., :C1], gskip = gdst[..., C1:]). */
+int mrh_unet_up2_concat(const void* src, const void* skip, void* dst, int nf, int H, int W, int C1,
+                        int C2, void* stream);
+int mrh_unet_up2_concat_bwd(const void* gdst, void* gsrc, void* gskip, int nf, int H, int W, int C1,
+                            int C2, void* stream);
+/* complex64 image (npix) * scale -> BF16 (npix, C) channels (re, im, 0, ...). */
+int mrh_unet_pack_input(const void* img, void* out, int64_t npix, int C, float scale, void* stream);
+/* xout = xin + (net[:, 0] + i net[:, 1]) * inv_scale (complex64; net is BF16 (npix, Cn)). */
+int mrh_unet_unpack_residual(const void* xin, const void* net, void* xout, int64_t npix, int Cn,
+                             float inv_scale, void* stream);
+/* loss += weight * sum |(xout - xt) scale|^2 (FP64, device scalar) and
+ * gnet[:, 0:2] = 2 weight (xout - xt) scale, gnet[:, 2:] = 0 (BF16 (npix, Cn)). */
+int mrh_unet_mse_grad(const void* xout, const void* xt, void* gnet, double* loss, int64_t npix,
+                      int Cn, float scale, float weight, void* stream);
+/* p[0:n] = value (FP32); used to clear gradient accumulators. */
+int mrh_unet_fill(float* p, int64_t n, float value, void* stream);
+/* Adam step (bias-corrected, step >= 1) on the FP32 master weights wm and bias,
+ * then refresh the BF16 forward copy wf and, if wbd != NULL, the backward-data
+ * copy.  lr == 0 only refreshes the copies (gw, m, v may then be NULL). */
+int mrh_unet_adam(float* wm, const float* gw, float* m, float* v, void* wf, void* wbd, float* bias,
+                  const float* gb, float* mb, float* vb, int ntap, int Cout, int Cout_pad, int Cin,
+                  float lr, float beta1, float beta2, float eps, int step, void* stream);
 
 #ifdef __cplusplus
 }

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -53,7 +53,8 @@ class TVDesc(Structure):
 
 class CGDesc(Structure):
     _fields_ = [("tv", TVDesc), ("k_prob", POINTER(c_int64)), ("direction", c_int), ("c1", c_double),
-                ("ls_beta", c_double), ("ls_max_iter", c_int), ("rel_tol", c_double), ("max_hist", c_int)]
+                ("ls_beta", c_double), ("ls_max_iter", c_int), ("rel_tol", c_double), ("max_hist", c_int),
+                ("restart_c", c_double)]
 
 
 class CGHist(Structure):
@@ -86,6 +87,19 @@ _SIGS = {
     "mrh_cg_direction": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "mrh_cg_read_status": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), c_void_p]),
     "mrh_cg_read_history": (c_int, [c_void_p, POINTER(CGHist), c_int, c_void_p]),
+    "mrh_unet_conv_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 9 + [c_void_p]),
+    "mrh_unet_conv_bwd_weight": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 8 + [c_void_p]),
+    "mrh_unet_relu_bwd": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),
+    "mrh_unet_pool2": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
+    "mrh_unet_pool2_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
+    "mrh_unet_up2_concat": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 5 + [c_void_p]),
+    "mrh_unet_up2_concat_bwd": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 5 + [c_void_p]),
+    "mrh_unet_pack_input": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),
+    "mrh_unet_unpack_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),
+    "mrh_unet_mse_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_float,
+                                  c_void_p]),
+    "mrh_unet_fill": (c_int, [c_void_p, c_int64, c_float, c_void_p]),
+    "mrh_unet_adam": (c_int, [c_void_p] * 10 + [c_int] * 4 + [c_float] * 4 + [c_int, c_void_p]),
 }
 
 _ERR = {1: ValueError, 2: RuntimeError, 3: MemoryError, 4: NotImplementedError}

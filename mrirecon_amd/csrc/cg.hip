@@ -278,7 +278,7 @@ struct ProbState {
 
 struct mrh_cg {
   int P, F, S0, S1, direction, ls_max_iter, max_hist;
-  double c1, ls_beta, rel_tol;
+  double c1, ls_beta, rel_tol, restart_c;
   int64_t npix_p, kmax_p;
   int nb_i, nb_k;
   TVDev tv;
@@ -502,7 +502,8 @@ __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, fl
 
 // Finish gradient reductions: beta, stop rule, history, iteration count.
 __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* part_i, int nb_i, int prime,
-                                                int direction, double rel_tol, mrh_cg_hist* hist, int max_hist, int P) {
+                                                int direction, double rel_tol, double restart_c,
+                                                mrh_cg_hist* hist, int max_hist, int P) {
   const int p = blockIdx.x;
   __shared__ double vals[5];
   for (int v = 0; v < 5; ++v) {
@@ -530,6 +531,8 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
     if (!s.have_rho_prev) { s.rho_prev = s.rho; s.have_rho_prev = 1; }  // first beta = 1 (directions.py:145-149)
     beta = s.rho / fmax(s.rho_prev, 1e-20);
   }
+  // descent safeguard: Re<g, -g + beta d> = -rho + beta Re<g, d>
+  if (restart_c >= 0.0 && -s.rho + beta * s.gdot_step > -restart_c * s.rho) beta = 0.0;
   s.rho_prev = s.rho;
   s.beta = beta;
   if (s.iter < max_hist) {
@@ -582,7 +585,7 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   if (rc) { delete e; return rc; }
   e->P = d->tv.P; e->F = d->tv.F; e->S0 = d->tv.S0; e->S1 = d->tv.S1;
   e->direction = d->direction; e->ls_max_iter = d->ls_max_iter;
-  e->c1 = d->c1; e->ls_beta = d->ls_beta; e->rel_tol = d->rel_tol;
+  e->c1 = d->c1; e->ls_beta = d->ls_beta; e->rel_tol = d->rel_tol; e->restart_c = d->restart_c;
   e->max_hist = std::max(1, d->max_hist);
   e->npix_p = (int64_t)e->F * e->S0 * e->S1;
   int64_t kmax = 0;
@@ -673,7 +676,7 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
                      (const float2*)d, (const float2*)g_prev, e->tv, e->npix_p, e->d_st, prime, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
-                     e->direction, e->rel_tol, e->d_hist, e->max_hist, e->P);
+                     e->direction, e->rel_tol, e->restart_c, e->d_hist, e->max_hist, e->P);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

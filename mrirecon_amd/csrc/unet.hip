@@ -1,0 +1,562 @@
+// BF16 MFMA kernels of the 2D+t U-Net warm start (gfx950).
+//
+// Activations: BF16, channels-last (volumes * frames, H, W, C), C a multiple of
+// 8.  Forward weights: BF16 [tap][Cout_pad][Cin] (Cin contiguous, Cout_pad a
+// multiple of 32, padded rows zero).  Tap sets: 3x3 spatial (t = 3 (dy+1) +
+// (dx+1)), 3-tap temporal (t = df+1, never crossing a volume's first or last
+// frame) and 1x1.  Out-of-range neighbours read as zero (zero padding).
+//
+// Convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16 (D = A B, A
+// 16x32, B 32x16, FP32 accumulate).  Lane l holds A row (l % 16) and B
+// column (l % 16) with the eight K indices 8 (l/16) .. +7, and D rows
+// 4 (l/16) .. +3 of column (l % 16).
+//   forward          D[cout][pixel] = sum_(tap, cin) W[tap][cout][cin] x[pixel + off(tap)][cin]
+//                    epilogue: + bias, optional ReLU, round to BF16.
+//   backward-data    the same kernel applied to the output gradient with the
+//                    weights transposed (cin <-> cout) and tap-flipped
+//                    (off(T-1-t) = -off(t) for every tap set used here).
+//   backward-weight  dW[tap][cout][cin] = sum_pixel g[pixel][cout] x[pixel + off(tap)][cin]:
+//                    pixel chunks (split-K) staged transposed in LDS, FP32
+//                    atomics per 32 x 32 block.
+#include <algorithm>
+#include "mrh_common.h"
+#include "mrirecon_hip.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16;
+
+__device__ __forceinline__ float bf2f(u16 b) { return __uint_as_float((unsigned)b << 16); }
+__device__ __forceinline__ u16 f2bf(float f) {   // round to nearest even (finite inputs)
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (u16)(u >> 16);
+}
+
+struct TapGeom {
+  int kind;      // MRH_UNET_TAP_1X1, _3X3 or _T3
+  int ntap;      // 1, 9 or 3
+  int F, H, W;   // frames per volume, image size
+};
+
+__device__ __forceinline__ void tap_offset(const TapGeom& g, int t, int& df, int& dy, int& dx) {
+  df = dy = dx = 0;
+  if (g.kind == MRH_UNET_TAP_3X3) { dy = t / 3 - 1; dx = t - 3 * (t / 3) - 1; }
+  else if (g.kind == MRH_UNET_TAP_T3) { df = t - 1; }
+}
+
+__device__ __forceinline__ bf16x8 load8(const u16* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p));
+}
+__device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
+
+// ------------------------------------------------------------------ forward / backward-data
+// Grid (ceil(npix / 128), Cout_pad / 32), 256 threads; wave w computes couts
+// [32 by, +32) x pixels [128 bx + 32 w, +32) as 2 x 2 MFMA tiles.
+__global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, const u16* __restrict__ w,
+                                                  const float* __restrict__ bias, u16* __restrict__ out,
+                                                  TapGeom g, int npix, int Cin, int Cout, int Cout_pad,
+                                                  int relu) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int co0 = blockIdx.y * 32;
+  const int px0 = blockIdx.x * 128 + wave * 32;
+  const int r = lane & 15, kg = lane >> 4;
+  const int HW = g.H * g.W;
+  int pfr[2], py[2], pxx[2];
+  bool pv[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = px0 + 16 * h + r;
+    pv[h] = p < npix;
+    const int pp = pv[h] ? p : 0;
+    pxx[h] = pp % g.W;
+    const int t1 = pp / g.W;
+    py[h] = t1 % g.H;
+    pfr[h] = (t1 / g.H) % g.F;   // frame within its volume
+  }
+  f32x4 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) acc[a][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  const int cgroups = Cin >> 3;            // K = ntap * cgroups groups of 8 channels
+  const int ngroups = g.ntap * cgroups;
+  int t = kg / cgroups, cg = kg - t * cgroups;   // this lane's group (g0 + kg) as (tap, channel group)
+  const u16* wrow0 = w + (size_t)(co0 + r) * Cin;
+  const u16* wrow1 = w + (size_t)(co0 + 16 + r) * Cin;
+  const size_t wtap = (size_t)Cout_pad * Cin;
+  for (int g0 = 0; g0 < ngroups; g0 += 4) {
+    bf16x8 av[2], bv[2];
+    if (t < g.ntap) {
+      const int ci = cg * 8;
+      av[0] = load8(wrow0 + t * wtap + ci);
+      av[1] = load8(wrow1 + t * wtap + ci);
+      int df, dy, dx;
+      tap_offset(g, t, df, dy, dx);
+      const int off = df * HW + dy * g.W + dx;
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const bool ok = pv[h] && (unsigned)(pfr[h] + df) < (unsigned)g.F &&
+                        (unsigned)(py[h] + dy) < (unsigned)g.H && (unsigned)(pxx[h] + dx) < (unsigned)g.W;
+        bv[h] = ok ? load8(x + (size_t)(px0 + 16 * h + r + off) * Cin + ci) : zero8();
+      }
+    } else {
+      av[0] = av[1] = bv[0] = bv[1] = zero8();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        acc[a][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[h], acc[a][h], 0, 0, 0);
+    cg += 4;
+    while (cg >= cgroups) { cg -= cgroups; ++t; }
+  }
+  // lane holds couts co0 + 16a + 4kg + j of pixel px0 + 16h + r
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int p = px0 + 16 * h + r;
+    if (p >= npix) continue;
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int cb = co0 + 16 * a + 4 * kg;
+      if (cb >= Cout) continue;
+      u16 o[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float v = acc[a][h][j] + ((bias && cb + j < Cout) ? bias[cb + j] : 0.f);
+        if (relu) v = fmaxf(v, 0.f);
+        o[j] = f2bf(v);
+      }
+      if (cb + 3 < Cout) {
+        *reinterpret_cast<uint2*>(out + (size_t)p * Cout + cb) =
+            make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16), (unsigned)o[2] | ((unsigned)o[3] << 16));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (cb + j < Cout) out[(size_t)p * Cout + cb + j] = o[j];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ backward-weight
+// Grid (nchunks, Cout_pad / 32, ntap * ceil(Cin / 32)), 256 threads.  Block
+// computes dW[tap][32 couts][32 cins] over pixels [chunk * bx, +chunk) in
+// steps of 128 pixels; waves form a 2 x 2 grid of 16 x 16 MFMA tiles.
+#define BW_STEP 128
+__global__ __launch_bounds__(256) void k_conv_bw(const u16* __restrict__ gout, const u16* __restrict__ x,
+                                                 float* __restrict__ dw, TapGeom g, int npix, int chunk,
+                                                 int Cin, int Cout, int Cout_pad) {
+  __shared__ __align__(16) u16 sA[32][BW_STEP + 8];   // [cout][pixel]
+  __shared__ __align__(16) u16 sB[32][BW_STEP + 8];   // [cin][pixel]
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int co0 = blockIdx.y * 32;
+  const int cblocks = (Cin + 31) / 32;
+  const int t = blockIdx.z / cblocks, ci0 = (blockIdx.z - t * cblocks) * 32;
+  const int p_begin = blockIdx.x * chunk;
+  const int p_end = min(npix, p_begin + chunk);
+  const int HW = g.H * g.W;
+  const int r = lane & 15, kg = lane >> 4;
+  const int wa = wave >> 1, wb = wave & 1;
+  int df, dy, dx;
+  tap_offset(g, t, df, dy, dx);
+  const int off = df * HW + dy * g.W + dx;
+  const int sc = (tid & 7) * 4;               // staged channel quad
+  const bool a_full = co0 + sc + 3 < Cout, b_ok = ci0 + sc < Cin;   // Cin % 8 == 0: quads never straddle
+  f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int p0 = p_begin; p0 < p_end; p0 += BW_STEP) {
+    uint2 va[4], vb[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int sp = (tid >> 3) + 32 * s;
+      const int p = p0 + sp;
+      va[s] = vb[s] = make_uint2(0, 0);
+      if (p < p_end) {
+        if (a_full) {
+          va[s] = *reinterpret_cast<const uint2*>(gout + (size_t)p * Cout + co0 + sc);
+        } else {
+          unsigned q[4] = {0, 0, 0, 0};
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (co0 + sc + j < Cout) q[j] = gout[(size_t)p * Cout + co0 + sc + j];
+          va[s] = make_uint2(q[0] | (q[1] << 16), q[2] | (q[3] << 16));
+        }
+        if (b_ok) {
+          const int xx = p % g.W, t1 = p / g.W, yy = t1 % g.H, f = (t1 / g.H) % g.F;
+          if ((unsigned)(f + df) < (unsigned)g.F && (unsigned)(yy + dy) < (unsigned)g.H &&
+              (unsigned)(xx + dx) < (unsigned)g.W)
+            vb[s] = *reinterpret_cast<const uint2*>(x + (size_t)(p + off) * Cin + ci0 + sc);
+        }
+      }
+    }
+    __syncthreads();   // every wave has read the previous step's fragments
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int sp = (tid >> 3) + 32 * s;
+      sA[sc + 0][sp] = (u16)(va[s].x & 0xffffu); sA[sc + 1][sp] = (u16)(va[s].x >> 16);
+      sA[sc + 2][sp] = (u16)(va[s].y & 0xffffu); sA[sc + 3][sp] = (u16)(va[s].y >> 16);
+      sB[sc + 0][sp] = (u16)(vb[s].x & 0xffffu); sB[sc + 1][sp] = (u16)(vb[s].x >> 16);
+      sB[sc + 2][sp] = (u16)(vb[s].y & 0xffffu); sB[sc + 3][sp] = (u16)(vb[s].y >> 16);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BW_STEP / 32; ++kk) {
+      const bf16x8 a = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sA[16 * wa + r][32 * kk + 8 * kg]));
+      const bf16x8 b = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(&sB[16 * wb + r][32 * kk + 8 * kg]));
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+    }
+  }
+  const int ci = ci0 + 16 * wb + r;
+  if (ci < Cin) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int co = co0 + 16 * wa + 4 * kg + j;
+      if (co < Cout) unsafeAtomicAdd(&dw[((size_t)t * Cout_pad + co) * Cin + ci], acc[j]);
+    }
+  }
+}
+
+// db[c] += sum over a 4096-pixel chunk of g[p][c]; 256 threads = (256 / C) pixel lanes x C channels
+__global__ __launch_bounds__(256) void k_bias_grad(const u16* __restrict__ gout, float* __restrict__ db, int npix,
+                                                   int C) {
+  __shared__ float part[256];
+  const int lanes = 256 / C;
+  const int c = threadIdx.x % C, pl = threadIdx.x / C;
+  float s = 0.f;
+  if (pl < lanes) {
+    const int p1 = min(npix, (int)(blockIdx.x + 1) * 4096);
+    for (int p = blockIdx.x * 4096 + pl; p < p1; p += lanes) s += bf2f(gout[(size_t)p * C + c]);
+  }
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (pl == 0) {
+    for (int q = 1; q < lanes; ++q) s += part[q * C + c];
+    unsafeAtomicAdd(&db[c], s);
+  }
+}
+
+// ------------------------------------------------------------------ elementwise
+#define GRID_STRIDE(i, n) \
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
+
+__global__ void k_relu_bwd(u16* __restrict__ g, const u16* __restrict__ y, int64_t n) {
+  GRID_STRIDE(i, n) if (!(bf2f(y[i]) > 0.f)) g[i] = 0;
+}
+
+__global__ void k_pool_fwd(const u16* __restrict__ x, u16* __restrict__ y, int nf, int H, int W, int C) {
+  const int Ho = H / 2, Wo = W / 2;
+  GRID_STRIDE(i, (int64_t)nf * Ho * Wo * C) {
+    const int c = i % C;
+    int64_t q = i / C;
+    const int xo = q % Wo; q /= Wo;
+    const int yo = q % Ho;
+    const int64_t f = q / Ho;
+    const size_t b = (((size_t)f * H + 2 * yo) * W + 2 * xo) * C + c;
+    const size_t row = (size_t)W * C;
+    y[i] = f2bf(0.25f * (bf2f(x[b]) + bf2f(x[b + C]) + bf2f(x[b + row]) + bf2f(x[b + row + C])));
+  }
+}
+
+// gx = pool^T gy (+ add)
+__global__ void k_pool_bwd(const u16* __restrict__ gy, const u16* __restrict__ add, u16* __restrict__ gx, int nf,
+                           int H, int W, int C) {
+  GRID_STRIDE(i, (int64_t)nf * H * W * C) {
+    const int c = i % C;
+    int64_t q = i / C;
+    const int xx = q % W; q /= W;
+    const int yy = q % H;
+    const int64_t f = q / H;
+    float v = 0.25f * bf2f(gy[(((size_t)f * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]);
+    if (add) v += bf2f(add[i]);
+    gx[i] = f2bf(v);
+  }
+}
+
+// dst = concat(nearest_up2(src) [C1 channels], skip [C2 channels]) at H x W
+__global__ void k_up_concat(const u16* __restrict__ src, const u16* __restrict__ skip, u16* __restrict__ dst,
+                            int nf, int H, int W, int C1, int C2) {
+  const int C = C1 + C2;
+  GRID_STRIDE(i, (int64_t)nf * H * W * C) {
+    const int c = i % C;
+    const int64_t pix = i / C;
+    if (c < C1) {
+      const int xx = pix % W;
+      const int64_t q = pix / W;
+      const int yy = q % H;
+      const int64_t f = q / H;
+      dst[i] = src[(((size_t)f * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C1 + c];
+    } else {
+      dst[i] = skip[(size_t)pix * C2 + (c - C1)];
+    }
+  }
+}
+
+// gsrc = up2^T gdst[:, :C1] (H/2 x W/2), gskip = gdst[:, C1:]
+__global__ void k_up_concat_bwd(const u16* __restrict__ gdst, u16* __restrict__ gsrc, u16* __restrict__ gskip,
+                                int nf, int H, int W, int C1, int C2) {
+  const int C = C1 + C2, Ho = H / 2, Wo = W / 2;
+  const int64_t nsrc = (int64_t)nf * Ho * Wo * C1, nsk = (int64_t)nf * H * W * C2;
+  GRID_STRIDE(i, nsrc + nsk) {
+    if (i < nsrc) {
+      const int c = i % C1;
+      int64_t q = i / C1;
+      const int xo = q % Wo; q /= Wo;
+      const int yo = q % Ho;
+      const int64_t f = q / Ho;
+      const size_t b = (((size_t)f * H + 2 * yo) * W + 2 * xo) * C + c;
+      const size_t row = (size_t)W * C;
+      gsrc[i] = f2bf(bf2f(gdst[b]) + bf2f(gdst[b + C]) + bf2f(gdst[b + row]) + bf2f(gdst[b + row + C]));
+    } else {
+      const int64_t j = i - nsrc;
+      gskip[j] = gdst[(j / C2) * C + C1 + j % C2];
+    }
+  }
+}
+
+__global__ void k_pack_input(const float2* __restrict__ img, u16* __restrict__ out, int64_t npix, int C,
+                             float scale) {
+  GRID_STRIDE(i, npix) {
+    const float2 v = img[i];
+    u16* o = out + i * C;
+    o[0] = f2bf(v.x * scale);
+    o[1] = f2bf(v.y * scale);
+    for (int c = 2; c < C; ++c) o[c] = 0;
+  }
+}
+
+__global__ void k_unpack_residual(const float2* __restrict__ xin, const u16* __restrict__ net,
+                                  float2* __restrict__ xout, int64_t npix, int Cn, float inv_scale) {
+  GRID_STRIDE(i, npix) {
+    const float2 a = xin[i];
+    xout[i] = make_float2(a.x + bf2f(net[i * Cn]) * inv_scale, a.y + bf2f(net[i * Cn + 1]) * inv_scale);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mse_grad(const float2* __restrict__ xout, const float2* __restrict__ xt,
+                                                  u16* __restrict__ gnet, int64_t npix, int Cn, float scale,
+                                                  float w, double* __restrict__ loss) {
+  double s = 0.0;
+  GRID_STRIDE(i, npix) {
+    const float2 a = xout[i], b = xt[i];
+    const float ex = (a.x - b.x) * scale, ey = (a.y - b.y) * scale;
+    s += (double)(ex * ex + ey * ey) * w;
+    u16* o = gnet + i * Cn;
+    o[0] = f2bf(2.f * w * ex);
+    o[1] = f2bf(2.f * w * ey);
+    for (int c = 2; c < Cn; ++c) o[c] = 0;
+  }
+  s = warp_sum_d(s);
+  if ((threadIdx.x & 63) == 0) unsafeAtomicAdd(loss, s);
+}
+
+__global__ void k_fill_f32(float* __restrict__ p, int64_t n, float v) { GRID_STRIDE(i, n) p[i] = v; }
+
+// Adam on the FP32 master weights [tap][Cout_pad][Cin]; refreshes the BF16
+// forward copy (same layout, padded rows 0) and the backward-data copy
+// [T-1-tap][Cin][Cout].  lr == 0 only refreshes the copies.
+__global__ void k_adam(float* __restrict__ wm, const float* __restrict__ gw, float* __restrict__ m,
+                       float* __restrict__ v, u16* __restrict__ wf, u16* __restrict__ wbd, int ntap, int Cout,
+                       int Cout_pad, int Cin, float lr, float b1, float b2, float eps, float bc1, float bc2) {
+  GRID_STRIDE(i, (int64_t)ntap * Cout_pad * Cin) {
+    const int ci = i % Cin;
+    const int64_t q = i / Cin;
+    const int co = q % Cout_pad;
+    const int t = q / Cout_pad;
+    float wv = wm[i];
+    if (co < Cout && lr > 0.f) {
+      const float gg = gw[i];
+      const float mm = b1 * m[i] + (1.f - b1) * gg;
+      const float vv = b2 * v[i] + (1.f - b2) * gg * gg;
+      m[i] = mm;
+      v[i] = vv;
+      wv -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+      wm[i] = wv;
+    }
+    const u16 wb = co < Cout ? f2bf(wv) : (u16)0;
+    wf[i] = wb;
+    if (wbd && co < Cout) wbd[((size_t)(ntap - 1 - t) * Cin + ci) * Cout + co] = wb;
+  }
+}
+
+__global__ void k_adam_bias(float* __restrict__ b, const float* __restrict__ g, float* __restrict__ m,
+                            float* __restrict__ v, int n, float lr, float b1, float b2, float eps, float bc1,
+                            float bc2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float mm = b1 * m[i] + (1.f - b1) * g[i];
+  const float vv = b2 * v[i] + (1.f - b2) * g[i] * g[i];
+  m[i] = mm;
+  v[i] = vv;
+  b[i] -= lr * (mm / bc1) / (sqrtf(vv / bc2) + eps);
+}
+
+// ------------------------------------------------------------------ C ABI
+static unsigned grid_for(int64_t n) {
+  return (unsigned)std::min<int64_t>(262144, std::max<int64_t>(1, (n + 255) / 256));
+}
+
+static int make_geom(int kind, int F, int H, int W, TapGeom* g) {
+  MRH_REQUIRE(kind == MRH_UNET_TAP_1X1 || kind == MRH_UNET_TAP_3X3 || kind == MRH_UNET_TAP_T3, MRH_EINVAL,
+              "unet conv: tap kind %d is not 1x1 (0), 3x3 (1) or temporal-3 (2)", kind);
+  MRH_REQUIRE(F >= 1 && H >= 1 && W >= 1, MRH_EINVAL, "unet conv: bad geometry F=%d H=%d W=%d", F, H, W);
+  g->kind = kind;
+  g->ntap = kind == MRH_UNET_TAP_1X1 ? 1 : (kind == MRH_UNET_TAP_3X3 ? 9 : 3);
+  g->F = F; g->H = H; g->W = W;
+  return MRH_OK;
+}
+
+#define MRH_TRY(call) do { int _rc = (call); if (_rc) return _rc; } while (0)
+
+extern "C" int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias, void* out, int kind, int nvol,
+                                 int F, int H, int W, int Cin, int Cout, int Cout_pad, int relu, void* stream) {
+  MRH_REQUIRE(x && w && out, MRH_EINVAL, "unet_conv_fwd: null buffer");
+  TapGeom g;
+  MRH_TRY(make_geom(kind, F, H, W, &g));
+  MRH_REQUIRE(Cin >= 8 && Cin % 8 == 0, MRH_EINVAL, "unet_conv_fwd: Cin=%d must be a positive multiple of 8", Cin);
+  MRH_REQUIRE(Cout >= 1 && Cout <= Cout_pad && Cout_pad % 32 == 0, MRH_EINVAL,
+              "unet_conv_fwd: need 1 <= Cout=%d <= Cout_pad=%d, Cout_pad %% 32 == 0", Cout, Cout_pad);
+  const int64_t npix = (int64_t)nvol * F * H * W;
+  MRH_REQUIRE(nvol >= 1 && npix < (1ll << 31) - 256, MRH_EINVAL, "unet_conv_fwd: %lld pixels out of range",
+              (long long)npix);
+  dim3 grid((unsigned)((npix + 127) / 128), Cout_pad / 32);
+  hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, (hipStream_t)stream, (const u16*)x, (const u16*)w, bias,
+                     (u16*)out, g, (int)npix, Cin, Cout, Cout_pad, relu);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_conv_bwd_weight(const void* gout, const void* x, float* dw, float* db, int kind, int nvol,
+                                        int F, int H, int W, int Cin, int Cout, int Cout_pad, void* stream) {
+  MRH_REQUIRE(gout && x && dw, MRH_EINVAL, "unet_conv_bwd_weight: null buffer");
+  TapGeom g;
+  MRH_TRY(make_geom(kind, F, H, W, &g));
+  MRH_REQUIRE(Cin >= 8 && Cin % 8 == 0, MRH_EINVAL, "unet_conv_bwd_weight: Cin=%d must be a multiple of 8", Cin);
+  MRH_REQUIRE(Cout >= 1 && Cout <= Cout_pad && Cout_pad % 32 == 0 && Cout <= 256, MRH_EINVAL,
+              "unet_conv_bwd_weight: need 1 <= Cout=%d <= min(Cout_pad=%d, 256), Cout_pad %% 32 == 0", Cout,
+              Cout_pad);
+  const int64_t npix = (int64_t)nvol * F * H * W;
+  MRH_REQUIRE(nvol >= 1 && npix < (1ll << 31) - 8192, MRH_EINVAL, "unet_conv_bwd_weight: %lld pixels out of range",
+              (long long)npix);
+  hipStream_t st = (hipStream_t)stream;
+  // split-K over pixels: about 2048 blocks in total, at least one 128-pixel step each
+  const int64_t tiles = (int64_t)(Cout_pad / 32) * g.ntap * ((Cin + 31) / 32);
+  int64_t nchunks = std::max<int64_t>(1, (2048 + tiles - 1) / tiles);
+  int64_t chunk = (npix + nchunks - 1) / nchunks;
+  chunk = std::max<int64_t>(BW_STEP, (chunk + BW_STEP - 1) / BW_STEP * BW_STEP);
+  nchunks = (npix + chunk - 1) / chunk;
+  dim3 grid((unsigned)nchunks, Cout_pad / 32, g.ntap * ((Cin + 31) / 32));
+  hipLaunchKernelGGL(k_conv_bw, grid, dim3(256), 0, st, (const u16*)gout, (const u16*)x, dw, g, (int)npix,
+                     (int)chunk, Cin, Cout, Cout_pad);
+  MRH_CHECK_LAUNCH();
+  if (db) {
+    hipLaunchKernelGGL(k_bias_grad, dim3((unsigned)((npix + 4095) / 4096)), dim3(256), 0, st, (const u16*)gout, db,
+                       (int)npix, Cout);
+    MRH_CHECK_LAUNCH();
+  }
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_relu_bwd(void* g, const void* y, int64_t n, void* stream) {
+  MRH_REQUIRE(g && y && n >= 0, MRH_EINVAL, "unet_relu_bwd: bad arguments");
+  if (n == 0) return MRH_OK;
+  hipLaunchKernelGGL(k_relu_bwd, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (u16*)g, (const u16*)y, n);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_pool2(const void* x, void* y, int nf, int H, int W, int C, void* stream) {
+  MRH_REQUIRE(x && y && nf >= 1 && C >= 1 && H >= 2 && W >= 2 && H % 2 == 0 && W % 2 == 0, MRH_EINVAL,
+              "unet_pool2: need even H=%d, W=%d", H, W);
+  hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for((int64_t)nf * H * W * C / 4)), dim3(256), 0, (hipStream_t)stream,
+                     (const u16*)x, (u16*)y, nf, H, W, C);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_pool2_bwd(const void* gy, const void* add, void* gx, int nf, int H, int W, int C,
+                                  void* stream) {
+  MRH_REQUIRE(gy && gx && nf >= 1 && C >= 1 && H % 2 == 0 && W % 2 == 0 && H >= 2 && W >= 2, MRH_EINVAL,
+              "unet_pool2_bwd: need even H=%d, W=%d", H, W);
+  hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for((int64_t)nf * H * W * C)), dim3(256), 0, (hipStream_t)stream,
+                     (const u16*)gy, (const u16*)add, (u16*)gx, nf, H, W, C);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_up2_concat(const void* src, const void* skip, void* dst, int nf, int H, int W, int C1,
+                                   int C2, void* stream) {
+  MRH_REQUIRE(src && skip && dst && nf >= 1 && C1 >= 1 && C2 >= 1 && H % 2 == 0 && W % 2 == 0 && H >= 2 && W >= 2,
+              MRH_EINVAL, "unet_up2_concat: bad arguments (H=%d W=%d C1=%d C2=%d)", H, W, C1, C2);
+  hipLaunchKernelGGL(k_up_concat, dim3(grid_for((int64_t)nf * H * W * (C1 + C2))), dim3(256), 0,
+                     (hipStream_t)stream, (const u16*)src, (const u16*)skip, (u16*)dst, nf, H, W, C1, C2);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_up2_concat_bwd(const void* gdst, void* gsrc, void* gskip, int nf, int H, int W, int C1,
+                                       int C2, void* stream) {
+  MRH_REQUIRE(gdst && gsrc && gskip && nf >= 1 && C1 >= 1 && C2 >= 1 && H % 2 == 0 && W % 2 == 0 && H >= 2 &&
+                  W >= 2,
+              MRH_EINVAL, "unet_up2_concat_bwd: bad arguments (H=%d W=%d C1=%d C2=%d)", H, W, C1, C2);
+  const int64_t n = (int64_t)nf * (H / 2) * (W / 2) * C1 + (int64_t)nf * H * W * C2;
+  hipLaunchKernelGGL(k_up_concat_bwd, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (const u16*)gdst,
+                     (u16*)gsrc, (u16*)gskip, nf, H, W, C1, C2);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_pack_input(const void* img, void* out, int64_t npix, int C, float scale, void* stream) {
+  MRH_REQUIRE(img && out && npix >= 1 && C >= 2, MRH_EINVAL, "unet_pack_input: bad arguments");
+  hipLaunchKernelGGL(k_pack_input, dim3(grid_for(npix)), dim3(256), 0, (hipStream_t)stream, (const float2*)img,
+                     (u16*)out, npix, C, scale);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_unpack_residual(const void* xin, const void* net, void* xout, int64_t npix, int Cn,
+                                        float inv_scale, void* stream) {
+  MRH_REQUIRE(xin && net && xout && npix >= 1 && Cn >= 2, MRH_EINVAL, "unet_unpack_residual: bad arguments");
+  hipLaunchKernelGGL(k_unpack_residual, dim3(grid_for(npix)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)xin, (const u16*)net, (float2*)xout, npix, Cn, inv_scale);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_mse_grad(const void* xout, const void* xt, void* gnet, double* loss, int64_t npix, int Cn,
+                                 float scale, float weight, void* stream) {
+  MRH_REQUIRE(xout && xt && gnet && loss && npix >= 1 && Cn >= 2, MRH_EINVAL, "unet_mse_grad: bad arguments");
+  hipLaunchKernelGGL(k_mse_grad, dim3(std::min<unsigned>(grid_for(npix), 2048)), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)xout, (const float2*)xt, (u16*)gnet, npix, Cn, scale, weight, loss);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_fill(float* p, int64_t n, float value, void* stream) {
+  MRH_REQUIRE(p && n >= 0, MRH_EINVAL, "unet_fill: bad arguments");
+  if (n == 0) return MRH_OK;
+  hipLaunchKernelGGL(k_fill_f32, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, p, n, value);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_adam(float* wm, const float* gw, float* m, float* v, void* wf, void* wbd, float* bias,
+                             const float* gb, float* mb, float* vb, int ntap, int Cout, int Cout_pad, int Cin,
+                             float lr, float beta1, float beta2, float eps, int step, void* stream) {
+  MRH_REQUIRE(wm && wf && ntap >= 1 && Cin >= 1 && Cout >= 1 && Cout <= Cout_pad, MRH_EINVAL,
+              "unet_adam: bad arguments");
+  MRH_REQUIRE(lr == 0.f || (gw && m && v && step >= 1), MRH_EINVAL,
+              "unet_adam: an update (lr > 0) needs gradients, moments and step >= 1");
+  hipStream_t st = (hipStream_t)stream;
+  const float bc1 = step >= 1 ? 1.f - powf(beta1, (float)step) : 1.f;
+  const float bc2 = step >= 1 ? 1.f - powf(beta2, (float)step) : 1.f;
+  hipLaunchKernelGGL(k_adam, dim3(grid_for((int64_t)ntap * Cout_pad * Cin)), dim3(256), 0, st, wm, gw, m, v,
+                     (u16*)wf, (u16*)wbd, ntap, Cout, Cout_pad, Cin, lr, beta1, beta2, eps, bc1, bc2);
+  MRH_CHECK_LAUNCH();
+  if (lr != 0.f && bias && gb && mb && vb) {
+    hipLaunchKernelGGL(k_adam_bias, dim3((Cout + 255) / 256), dim3(256), 0, st, bias, gb, mb, vb, Cout, lr, beta1,
+                       beta2, eps, bc1, bc2);
+    MRH_CHECK_LAUNCH();
+  }
+  return MRH_OK;
+}

@@ -42,7 +42,7 @@ class NLCGEngine:
 
     def __init__(self, plan, y: torch.Tensor, *, problems: int, frames: int, regm: Optional[RegManager],
                  direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5, ls_max_iter: int = 20,
-                 rel_tol: float = 0.0, max_hist: int = 256):
+                 rel_tol: float = 0.0, max_hist: int = 256, restart_c: Optional[float] = None):
         if direction not in ("fr", "prplus"):
             raise ValueError("direction must be 'fr' or 'prplus'")
         self.plan = plan
@@ -73,6 +73,9 @@ class NLCGEngine:
         desc.direction = _lib.CG_DIR_FR if direction == "fr" else _lib.CG_DIR_PRPLUS
         desc.c1, desc.ls_beta, desc.ls_max_iter = float(c1), float(ls_beta), int(ls_max_iter)
         desc.rel_tol, desc.max_hist = float(rel_tol or 0.0), int(max_hist)
+        desc.restart_c = -1.0 if restart_c is None else float(restart_c)
+        if restart_c is not None and restart_c < 0:
+            raise ValueError("restart_c must be >= 0 (None keeps the reference behaviour)")
         self.max_hist = int(max_hist)
         self._lib = _lib.load()
         h = ctypes.c_void_p()
@@ -187,6 +190,7 @@ class NLCGEngine:
 @dataclass
 class CGConfig:
     """graspcg/solvers/cg.py:30-52, plus ``rel_tol`` (relative objective-change stop rule),
+    ``restart_c`` (beta = 0 restart when the next direction would not descend; None = reference),
     ``use_graph`` (HIP-graph replay of the iteration) and ``problems`` (independent slices)."""
     devices: Optional[object] = None
     direction: str = "fr"
@@ -204,6 +208,7 @@ class CGConfig:
     record_history: bool = True
     verbose: bool = False
     rel_tol: Optional[float] = None
+    restart_c: Optional[float] = None
     use_graph: bool = True
     problems: int = 1
 
@@ -253,7 +258,8 @@ class CGSolver:
         self.engine = NLCGEngine(plan, y, problems=P, frames=plan.B // P, regm=regm,
                                  direction=self.cfg.direction.lower(), c1=self.cfg.c1,
                                  ls_beta=self.cfg.ls_beta, ls_max_iter=self.cfg.ls_max_iter,
-                                 rel_tol=self.cfg.rel_tol or 0.0, max_hist=max(64, self.cfg.max_iter))
+                                 rel_tol=self.cfg.rel_tol or 0.0, max_hist=max(64, self.cfg.max_iter),
+                                 restart_c=self.cfg.restart_c)
         self.ws = Workspace(self.engine)
         if self.cfg.init_mode == "zero":
             self.engine.x.zero_()

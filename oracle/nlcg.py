@@ -184,11 +184,16 @@ def _norm(v, rdt):
 def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequence[TV],
          max_iter: int = 50, direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5,
          ls_max_iter: int = 20, tol_g: float = 1e-4, tol_step: float = 1e-9,
-         rel_tol: Optional[float] = None, mutate_x: bool = True) -> CGResult:
+         rel_tol: Optional[float] = None, mutate_x: bool = True,
+         restart_c: Optional[float] = None) -> CGResult:
     """Reference-semantics NLCG (graspcg/solvers/cg.py:162-246) with Armijo search.
 
     ``rel_tol`` adds the stop rule |f_k - f_{k-1}| / |f_{k-1}| < rel_tol on the
-    accepted objective values (not a reference rule; used for time-to-tolerance)."""
+    accepted objective values (not a reference rule; used for time-to-tolerance).
+    ``restart_c`` (not a reference rule) sets beta = 0 whenever the new direction
+    -g + beta d would have Re<g, d_new> > -restart_c |g|^2, so every searched
+    direction descends; None keeps the reference behaviour (such a direction
+    ends the run with "not_descent")."""
     x = np.array(x0, copy=True)
     rdt = x.real.dtype
     obj = Objective(A, AH, y, regs, mutate_x=mutate_x)
@@ -237,6 +242,10 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
             g_prev = g.copy()
         else:
             raise ValueError(direction)
+        if restart_c is not None:
+            gd_old = rdt.type(np.vdot(g.reshape(-1), d.reshape(-1)).real)
+            if -rho + beta * gd_old > -restart_c * rho:
+                beta = rdt.type(0.0)
         d = d * rdt.type(beta) - g
     if not res.stop:
         res.stop = "max_iter"

@@ -214,3 +214,27 @@ def test_cgsolver_reference_surface():
     assert solver.ws.get("x").data_ptr() == solver.ws.get("var").data_ptr()
     with pytest.raises(NotImplementedError):
         CGSolver(y, be, regm, CGConfig(ls_name="wolfe"))
+
+
+def test_restart_safeguard_matches_oracle():
+    """The device beta restart (restart_c = 0) follows the oracle's restarted PR+ run to the
+    relative-change stop: same iteration count (+-1), objective within 1e-4."""
+    wl = small_dynamic(seed=21, F=5)
+    be, ora = prepare(wl)
+    regm, oregs = regs_for(wl)
+    x0 = ora.adjoint(wl.y)
+    eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, direction="prplus", rel_tol=1e-4,
+                     restart_c=0.0, max_hist=80)
+    eng.prime(c64(x0))
+    eng.run(60)
+    torch.cuda.synchronize()
+    hist = eng.history()[0]
+    ref = nlcg(ora.forward, ora.adjoint, wl.y.astype(np.complex128), x0.astype(np.complex128), oregs,
+               max_iter=60, direction="prplus", mutate_x=False, rel_tol=1e-4, restart_c=0.0)
+    f_gpu = objective_value(ora.forward, wl.y, eng.x.cpu().numpy().astype(np.complex128), oregs)
+    print(f"[restart] gpu iters={len(hist)} stop={eng.stop_reasons()} ref iters={ref.n_iter} stop={ref.stop} "
+          f"f_gpu={f_gpu:.8e} f_ref={ref.f_final:.8e}")
+    assert eng.stop_reasons()[0] == "rel_tol" and ref.stop == "rel_tol"
+    assert abs(len(hist) - ref.n_iter) <= 1
+    assert abs(f_gpu - ref.f_final) <= 1e-4 * abs(ref.f_final)
+    assert all(h["gdot"] < 0 for h in hist)

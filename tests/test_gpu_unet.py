@@ -1,0 +1,283 @@
+"""U-Net warm-start kernels on the GPU against the CPU autograd oracle (oracle/unet.py).
+
+Bars (BF16 storage, FP32 accumulation):
+* conv forward / backward-data: the BF16 output equals the FP64 convolution of the
+  same BF16 inputs to BF16 rounding (|err| <= 2^-8 |ref| + 1e-3 * RMS(ref) per element);
+* backward-weight / bias: FP32 atomics of exact BF16 products, rel L2 <= 1e-5;
+* whole network (forward, loss, every weight gradient) vs the oracle with BF16
+  rounding of each stored activation and gradient: rel L2 <= 1e-2 per tensor.
+"""
+import numpy as np
+import pytest
+import torch
+
+from mrirecon_amd import _lib, synth
+from mrirecon_amd.warmstart import ObjectiveEval, UNet2DT, select_warm_start
+from oracle import unet as ou
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def bf16_np(a):
+    return torch.from_numpy(np.asarray(a, np.float32)).to(torch.bfloat16).to(torch.float64).numpy()
+
+
+def to_dev_bf16(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to(torch.bfloat16).to(DEV)
+
+
+def st():
+    return torch.cuda.current_stream(DEV).cuda_stream
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm((np.asarray(a) - np.asarray(b)).ravel()) / max(np.linalg.norm(np.asarray(b).ravel()), 1e-300))
+
+
+def oracle_conv(x, W, b, kind, nvol, F, relu=False):
+    xt = ou.to_ncdhw(torch.from_numpy(x), nvol, F)
+    return ou.to_channels_last(ou.conv(xt, torch.from_numpy(W), None if b is None else torch.from_numpy(b),
+                                       kind, relu)).numpy()
+
+
+def check_bf16_close(got, ref):
+    err = np.abs(got - ref)
+    tol = 2.0 ** -8 * np.abs(ref) + 1e-3 * np.sqrt(np.mean(ref ** 2))
+    assert np.all(err <= tol), (float((err - tol).max()), float(np.sqrt(np.mean(ref ** 2))))
+
+
+GEOM = (2, 5, 16, 24)   # volumes, frames, H, W: 3840 pixels (not a multiple of the 128-pixel tile)
+
+
+@pytest.mark.parametrize("kind,cin,cout,relu", [
+    (ou.TAP_3X3, 8, 32, True), (ou.TAP_3X3, 64, 32, False), (ou.TAP_T3, 32, 64, True),
+    (ou.TAP_T3, 64, 128, False), (ou.TAP_1X1, 32, 8, False), (ou.TAP_1X1, 256, 128, True)])
+def test_conv_forward(kind, cin, cout, relu):
+    lib = _lib.load()
+    nvol, F, H, W = GEOM
+    npix = nvol * F * H * W
+    rng = np.random.default_rng(cin + cout + kind)
+    T = ou.NTAP[kind]
+    x = bf16_np(rng.standard_normal((nvol * F, H, W, cin)))
+    Wt = bf16_np(rng.standard_normal((T, cout, cin)) * np.sqrt(2.0 / (T * cin)))
+    b = (rng.standard_normal(cout) * 0.1).astype(np.float32)
+    cpad = (cout + 31) // 32 * 32
+    Wp = np.zeros((T, cpad, cin))
+    Wp[:, :cout] = Wt
+    out = torch.full((npix, cout), 7.0, dtype=torch.bfloat16, device=DEV)
+    bd = torch.from_numpy(b).to(DEV)
+    xd, wd = to_dev_bf16(x), to_dev_bf16(Wp)     # keep every buffer alive while the kernel runs
+    _lib.check(lib.mrh_unet_conv_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(),
+                                     out.data_ptr(), kind, nvol, F, H, W, cin, cout, cpad, int(relu), st()))
+    ref = oracle_conv(x, Wt, b.astype(np.float64), kind, nvol, F, relu).reshape(npix, cout)
+    check_bf16_close(out.float().cpu().numpy().astype(np.float64), ref)
+
+
+@pytest.mark.parametrize("kind,cin,cout", [(ou.TAP_3X3, 64, 32), (ou.TAP_T3, 32, 32), (ou.TAP_1X1, 256, 128),
+                                           (ou.TAP_1X1, 32, 8), (ou.TAP_3X3, 32, 64)])
+def test_conv_backward_data(kind, cin, cout):
+    """dL/dx from the forward kernel with transposed, tap-flipped weights (as the network lays them out)."""
+    lib = _lib.load()
+    nvol, F, H, W = GEOM
+    npix = nvol * F * H * W
+    rng = np.random.default_rng(100 + cin + cout + kind)
+    T = ou.NTAP[kind]
+    x = rng.standard_normal((nvol * F, H, W, cin))
+    Wt = bf16_np(rng.standard_normal((T, cout, cin)) * np.sqrt(2.0 / (T * cin)))
+    g = bf16_np(rng.standard_normal((nvol * F, H, W, cout)))
+    wbd = np.ascontiguousarray(Wt[::-1].transpose(0, 2, 1))        # [T-1-t][ci][co]
+    gx = torch.empty((npix, cin), dtype=torch.bfloat16, device=DEV)
+    gd, wd = to_dev_bf16(g), to_dev_bf16(wbd)
+    _lib.check(lib.mrh_unet_conv_fwd(gd.data_ptr(), wd.data_ptr(), None, gx.data_ptr(),
+                                     kind, nvol, F, H, W, cout, cin, cin, 0, st()))
+    xt = torch.from_numpy(x).requires_grad_(True)
+    y = ou.conv(ou.to_ncdhw(xt, nvol, F), torch.from_numpy(Wt), None, kind)
+    (ou.to_channels_last(y) * torch.from_numpy(g)).sum().backward()
+    check_bf16_close(gx.float().cpu().numpy().astype(np.float64), xt.grad.numpy().reshape(npix, cin))
+
+
+@pytest.mark.parametrize("kind,cin,cout", [(ou.TAP_3X3, 8, 32), (ou.TAP_3X3, 64, 32), (ou.TAP_T3, 32, 64),
+                                           (ou.TAP_1X1, 32, 8), (ou.TAP_T3, 256, 256)])
+def test_conv_backward_weight_and_bias(kind, cin, cout):
+    lib = _lib.load()
+    nvol, F, H, W = GEOM
+    npix = nvol * F * H * W
+    rng = np.random.default_rng(200 + cin + cout + kind)
+    T = ou.NTAP[kind]
+    x = bf16_np(rng.standard_normal((nvol * F, H, W, cin)))
+    g = bf16_np(rng.standard_normal((nvol * F, H, W, cout)))
+    cpad = (cout + 31) // 32 * 32
+    dw = torch.zeros(T * cpad * cin, dtype=torch.float32, device=DEV)
+    db = torch.zeros(cpad, dtype=torch.float32, device=DEV)
+    gd, xd = to_dev_bf16(g), to_dev_bf16(x)
+    args = (gd.data_ptr(), xd.data_ptr(), dw.data_ptr(), db.data_ptr(), kind, nvol, F, H, W, cin, cout, cpad, st())
+    _lib.check(lib.mrh_unet_conv_bwd_weight(*args))
+    Wt = torch.zeros((T, cout, cin), dtype=torch.float64, requires_grad=True)
+    bt = torch.zeros(cout, dtype=torch.float64, requires_grad=True)
+    y = ou.conv(ou.to_ncdhw(torch.from_numpy(x), nvol, F), Wt, bt, kind)
+    (ou.to_channels_last(y) * torch.from_numpy(g)).sum().backward()
+    got = dw.cpu().numpy().reshape(T, cpad, cin)
+    assert rel_l2(got[:, :cout], Wt.grad.numpy()) <= 1e-5
+    assert np.all(got[:, cout:] == 0)
+    assert rel_l2(db.cpu().numpy()[:cout], bt.grad.numpy()) <= 1e-5
+    # the entry point accumulates (+=)
+    _lib.check(lib.mrh_unet_conv_bwd_weight(*args))
+    assert rel_l2(dw.cpu().numpy().reshape(T, cpad, cin)[:, :cout], 2 * Wt.grad.numpy()) <= 1e-5
+
+
+def test_pool_upsample_concat_kernels():
+    lib = _lib.load()
+    nf, H, W, C1, C2 = 3, 8, 12, 32, 16
+    rng = np.random.default_rng(5)
+    e = bf16_np(rng.standard_normal((nf, H, W, C1)))
+    p = torch.empty((nf, H // 2, W // 2, C1), dtype=torch.bfloat16, device=DEV)
+    ed = to_dev_bf16(e)
+    _lib.check(lib.mrh_unet_pool2(ed.data_ptr(), p.data_ptr(), nf, H, W, C1, st()))
+    ref = e.reshape(nf, H // 2, 2, W // 2, 2, C1).mean(axis=(2, 4))
+    check_bf16_close(p.float().cpu().numpy(), ref)
+    # adjoint with an additive term
+    gy = bf16_np(rng.standard_normal((nf, H // 2, W // 2, C1)))
+    add = bf16_np(rng.standard_normal((nf, H, W, C1)))
+    gx = torch.empty((nf, H, W, C1), dtype=torch.bfloat16, device=DEV)
+    gyd, addd = to_dev_bf16(gy), to_dev_bf16(add)
+    _lib.check(lib.mrh_unet_pool2_bwd(gyd.data_ptr(), addd.data_ptr(), gx.data_ptr(), nf, H, W, C1, st()))
+    check_bf16_close(gx.float().cpu().numpy(), 0.25 * gy.repeat(2, axis=1).repeat(2, axis=2) + add)
+    # up2 + concat and its adjoint
+    s = bf16_np(rng.standard_normal((nf, H // 2, W // 2, C1)))
+    sk = bf16_np(rng.standard_normal((nf, H, W, C2)))
+    u = torch.empty((nf, H, W, C1 + C2), dtype=torch.bfloat16, device=DEV)
+    sd, skd = to_dev_bf16(s), to_dev_bf16(sk)
+    _lib.check(lib.mrh_unet_up2_concat(sd.data_ptr(), skd.data_ptr(), u.data_ptr(), nf, H, W, C1, C2, st()))
+    ref_u = np.concatenate([s.repeat(2, axis=1).repeat(2, axis=2), sk], axis=-1)
+    assert np.array_equal(u.float().cpu().numpy(), ref_u)
+    gu = bf16_np(rng.standard_normal((nf, H, W, C1 + C2)))
+    gs = torch.empty((nf, H // 2, W // 2, C1), dtype=torch.bfloat16, device=DEV)
+    gk = torch.empty((nf, H, W, C2), dtype=torch.bfloat16, device=DEV)
+    gud = to_dev_bf16(gu)
+    _lib.check(lib.mrh_unet_up2_concat_bwd(gud.data_ptr(), gs.data_ptr(), gk.data_ptr(), nf, H, W, C1, C2, st()))
+    check_bf16_close(gs.float().cpu().numpy(), gu[..., :C1].reshape(nf, H // 2, 2, W // 2, 2, C1).sum(axis=(2, 4)))
+    assert np.array_equal(gk.float().cpu().numpy(), gu[..., C1:])
+
+
+def _random_net(levels, seed):
+    net = UNet2DT(levels=levels, base=32, device=DEV, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    params = net.parameters()
+    for name, (W, b) in params.items():
+        if name == "out":
+            W[:] = rng.standard_normal(W.shape) * 0.2
+        if name != "enc0_s":
+            b[:] = rng.standard_normal(b.shape) * 0.05
+    net.set_parameters(params)
+    return net
+
+
+def _oracle_params(net):
+    out = {}
+    for name, (W, b) in net.parameters().items():
+        out[name] = (torch.tensor(bf16_np(W), dtype=torch.float64, requires_grad=True),
+                     torch.tensor(b.astype(np.float64), requires_grad=True))
+    return out
+
+
+def _smooth_pair(nvol, F, H, W, seed):
+    x_t = np.concatenate([synth.dynamic_phantom(H, F, seed=seed + v, n_ellipses=5) for v in range(nvol)])
+    rng = np.random.default_rng(seed)
+    noise = (rng.standard_normal(x_t.shape) + 1j * rng.standard_normal(x_t.shape)) * 0.2
+    x_adj = (0.8 * x_t + noise).astype(np.complex64)
+    return x_adj, x_t.astype(np.complex64)
+
+
+def test_network_forward_loss_and_gradients_match_oracle():
+    levels, nvol, F, H, W = 3, 2, 4, 16, 16
+    net = _random_net(levels, seed=3)
+    x_adj, x_t = _smooth_pair(nvol, F, H, W, seed=40)
+    xa, xt = torch.from_numpy(x_adj).to(DEV), torch.from_numpy(x_t).to(DEV)
+    s = 1.3
+    loss = float(net.loss_and_grad(xa, xt, nvol=nvol, scales=[s, s]).item())
+    xw = net._workspace(nvol, F, H, W, train=True)["x_warm"].cpu().numpy()
+    params = _oracle_params(net)
+    xa64, xt64 = torch.from_numpy(x_adj.astype(np.complex128)), torch.from_numpy(x_t.astype(np.complex128))
+    out = ou.forward(ou.bf16_round(ou.pack_input(xa64, s)), params, nvol, F, levels, 32, round_bf16=True)
+    ref_loss = ou.mse_loss(out, xa64, xt64, s)
+    ref_loss.backward()
+    ref_xw = ou.warm_start(out.detach(), xa64, s).numpy()
+    errs = {"x_warm": rel_l2(xw - x_adj, ref_xw - x_adj), "loss": abs(loss - ref_loss.item()) / ref_loss.item()}
+    grads = net.grads
+    for L in net.layers:
+        gw = L.bufs["gw"].reshape(L.ntap, L.cout_pad, L.cin)[:, :L.cout].cpu().numpy()
+        gb = L.bufs["gb"][:L.cout].cpu().numpy()
+        W, b = params[L.name]
+        errs[L.name + ".w"] = rel_l2(gw, W.grad.numpy())
+        errs[L.name + ".b"] = rel_l2(gb, b.grad.numpy())
+    assert grads.abs().sum().item() > 0
+    print("network errors:", {k: f"{v:.2e}" for k, v in errs.items()})
+    bad = {k: v for k, v in errs.items() if not v <= 1e-2}
+    assert not bad, bad
+
+
+def test_training_reduces_loss_and_inference_is_deterministic(tmp_path):
+    levels, nvol, F, H, W = 3, 1, 6, 32, 32
+    net = UNet2DT(levels=levels, base=32, device=DEV, seed=9)
+    x_adj, x_t = _smooth_pair(nvol, F, H, W, seed=50)
+    xa, xt = torch.from_numpy(x_adj).to(DEV), torch.from_numpy(x_t).to(DEV)
+    # untrained: zero output layer -> identity warm start
+    assert torch.equal(net.infer(xa).clone(), xa)
+    losses = [float(net.train_step(xa, xt, lr=2e-3).item()) for _ in range(60)]
+    assert losses[-1] < 0.3 * losses[0], (losses[0], losses[-1])
+    a = net.infer(xa).clone()
+    b = net.infer(xa).clone()
+    assert torch.equal(a, b)
+    p = tmp_path / "w.npz"
+    net.save(p)
+    net2 = UNet2DT.load(p, device=DEV)
+    assert torch.equal(net2.infer(xa).clone(), a)
+    assert net2.step == net.step == 60
+
+
+def test_shape_errors():
+    net = UNet2DT(levels=3, base=32, device=DEV)
+    with pytest.raises(ValueError):
+        net.infer(torch.zeros((4, 18, 18), dtype=torch.complex64, device=DEV))   # 18 not divisible by 4
+    with pytest.raises(ValueError):
+        net.infer(torch.zeros((5, 16, 16), dtype=torch.complex64, device=DEV), nvol=2)
+    with pytest.raises(ValueError):
+        UNet2DT(levels=3, base=16, device=DEV)
+    lib = _lib.load()
+    with pytest.raises(ValueError):
+        _lib.check(lib.mrh_unet_conv_fwd(1, 1, None, 1, 3, 1, 1, 8, 8, 8, 32, 32, 0, st()))
+    with pytest.raises(ValueError):
+        _lib.check(lib.mrh_unet_conv_fwd(1, 1, None, 1, 1, 1, 1, 8, 8, 12, 32, 32, 0, st()))
+
+
+def test_fallback_rejects_a_worse_warm_start():
+    from mrirecon_amd import RegManager, TVND, TVParams
+    from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+    P, F, S, C = 2, 3, 64, 4
+    traj, dcf, off = synth.radial_frames([13] * (P * F), 128)
+    x_true = np.concatenate([synth.dynamic_phantom(S, F, seed=60 + p, n_ellipses=5) for p in range(P)])
+    maps = synth.gaussian_maps(C, S)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=6, osf=2.0)
+    be.prepare((C, S, S), torch.from_numpy(traj), torch.from_numpy(dcf),
+               torch.from_numpy(maps.astype(np.complex64)).to(DEV), device=DEV, k_offsets=off)
+    plan = be.plan
+    xt = torch.from_numpy(x_true).to(DEV)
+    y = torch.empty(plan.n_k, dtype=torch.complex64, device=DEV)
+    plan.forward(xt, y)
+    x_adj = torch.empty_like(xt)
+    plan.adjoint(y, x_adj)
+    regm = RegManager([TVND("t", TVParams(weight=0.02, axes="temporal")),
+                       TVND("s", TVParams(weight=0.002, axes="spatial"))])
+    obj = ObjectiveEval(plan, y, regm, problems=P, frames=F)
+    # problem 0 gets the truth (better than the adjoint), problem 1 a blown-up image (worse)
+    x_warm = x_adj.clone()
+    x_warm[:F] = xt[:F]
+    x_warm[F:] = 5.0 * x_adj[F:]
+    x0, acc, f_w, f_a = select_warm_start(obj, x_adj, x_warm)
+    assert acc.tolist() == [True, False]
+    assert f_w[0] < f_a[0] and f_w[1] > f_a[1]
+    assert torch.equal(x0[:F], xt[:F]) and torch.equal(x0[F:], x_adj[F:])
+    # objective value = 1/2 |A x - y|^2 + TV: zero data term at the truth
+    e = regm.energy_and_grad_tensors(xt[:F].contiguous(), None).cpu().numpy()
+    assert abs(f_w[0] - e[0]) <= 1e-6 * abs(e[0])

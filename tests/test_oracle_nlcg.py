@@ -166,3 +166,31 @@ def test_objective_value_matches_f_at_zero():
     obj = Objective(A, AH, y, regs, mutate_x=False)
     obj.begin_linesearch(x, np.zeros_like(x))
     assert abs(obj.f_g(x, np.zeros_like(x), 0.0)[0] - objective_value(A, y, x, regs)) < 1e-10
+
+
+def test_restart_safeguard_keeps_every_direction_descending():
+    """PR+ with the reference's stop on a non-descent direction ends early on a small GRASP-like
+    problem; with restart_c = 0 (beta = 0 whenever -g + beta d would not descend) the same run
+    continues to the relative-change stop, every searched direction descends and f ends lower."""
+    from mrirecon_amd import synth
+    from oracle.kbnufft import KBNufft
+    F, S, C = 5, 64, 4
+    traj, dcf, off = synth.radial_frames([13] * F, 128)
+    x = synth.dynamic_phantom(S, F, seed=21, n_ellipses=6)
+    wl = synth.Workload("dyn", S, 1, F, C, 128, 6, 2.0, [13] * F, synth.gaussian_maps(C, S), traj, dcf, off, x,
+                        noise_seed=22)
+    ora = KBNufft(wl.maps, wl.traj, wl.k_offsets, wl.dcf, wl.W, wl.osf, dtype=np.float32)
+    synth.measure(wl, ora.forward)
+    regs = [TV(0.02, (0,)), TV(0.002, (1, 2))]
+    x0 = ora.adjoint(wl.y)
+    ref = nlcg(ora.forward, ora.adjoint, wl.y, x0, regs, max_iter=60, direction="prplus", rel_tol=1e-4,
+               mutate_x=False)
+    rst = nlcg(ora.forward, ora.adjoint, wl.y, x0, regs, max_iter=60, direction="prplus", rel_tol=1e-4,
+               mutate_x=False, restart_c=0.0)
+    assert ref.stop == "not_descent" and ref.n_iter < 5
+    assert rst.stop == "rel_tol" and rst.n_iter > 10
+    assert all(h["gdot"] < 0 for h in rst.history)
+    assert rst.f_final < 0.5 * ref.f_final
+    # identical until the first direction the reference rejects
+    for a, b in zip(ref.history, rst.history):
+        assert a["step"] == b["step"] and a["f"] == b["f"]
