@@ -8,9 +8,10 @@ A step is one NUFFT pair over the whole workload: forward A of all 34 frames
 into 12-coil k-space, then adjoint A^H back (408 coil-frames each way), with
 inputs resident in HBM.  ``value`` = coil-frame NUFFT pairs per second.
 Also reported: per-kernel HIP-event times and the roofline of the dominant
-kernel, the plain GPU NLCG time to a 1e-4 relative objective change from the
-adjoint start (temporal TV 0.02 + spatial TV 0.002), and the CPU oracle timed
-on a bounded sample of the same workload.
+kernel, the GPU NLCG time to a 1e-4 relative objective change from the
+adjoint start (plain) and from the learned U-Net warm start on held-out
+volumes (temporal TV 0.02 + spatial TV 0.002), the U-Net's BF16 MFMA rate, and
+the CPU oracle timed on a bounded sample of the same workload.
 
 Usage: python bench.py [--gpus 1] [--steps 20] [--warmup 3]
 """
@@ -28,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+BF16_PEAK_TFLOPS = 2500.0     # MI355X dense BF16 MFMA
 METRIC = "time-to-1e-4 objective per volume: warm-start vs plain NLCG; NUFFT pairs/s"
 
 
@@ -94,6 +96,74 @@ def cpu_baseline(wl, frames=8, reps=4):
             "seconds_per_pair_sample": t}
 
 
+def time_to_tolerance(args, dev):
+    """Plain vs warm-started NLCG on held-out GRASP volumes, plus the U-Net's MFMA rate.
+
+    Both runs: PR+ with the beta = 0 descent restart, Armijo (c1 1e-4, halving
+    from t = 1, 20 trials), temporal TV 0.02 + spatial TV 0.002, inputs in HBM.
+    plain = adjoint + NLCG to |f_k - f_(k-1)| <= 1e-4 |f_(k-1)|; warm = adjoint +
+    U-Net + fallback check + NLCG to the same rule; matched = the warm pipeline run
+    until it reaches the plain run's final objective (within 1e-4)."""
+    import numpy as np
+    import torch
+
+    from mrirecon_amd.train_warmstart import GraspProblem, _phantom, evaluate, run_warm
+    from mrirecon_amd.warmstart import UNet2DT
+
+    weights = os.path.join(ROOT, "mrirecon_amd", "weights", "unet_grasp.npz")
+    print(f"bench: time-to-1e-4 on {args.tol_volumes} held-out GRASP volumes (plain vs warm start)",
+          file=sys.stderr, flush=True)
+    pb = GraspProblem(max_iter=args.tol_iters)
+    seeds = list(range(520_000, 520_000 + args.tol_volumes))
+    phantoms = [_phantom(s) for s in seeds]
+    net = UNet2DT.load(weights, device=dev)
+    pb.measure(phantoms[0], noise_seed=seeds[0] + 7)      # warm-up: workspaces, graphs
+    run_warm(pb, net)
+    res = evaluate(pb, net, seeds, phantoms, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    rows = res.pop("rows")
+    ttt = {"volumes": len(rows), "seeds": f"{seeds[0]}..{seeds[-1]} (not in the training set)",
+           "solver": "PR+ with beta=0 descent restart, Armijo c1=1e-4 halving from t=1 (20 trials), "
+                     "temporal TV 0.02 + spatial TV 0.002 (eps 1e-3), FP64 reductions, HIP-graph iteration",
+           "stop_rule": "relative objective change <= 1e-4",
+           "plain_ms_median": round(res["median_plain_s"] * 1e3, 3),
+           "plain_iterations_median": res["median_plain_iters"],
+           "warm_ms_median": round(res["median_warm_s"] * 1e3, 3),
+           "warm_iterations_median": res["median_warm_iters"],
+           "speedup_median": round(res["speedup_median_of_ratios"], 3),
+           "matched_objective_ms_median": round(res["median_matched_s"] * 1e3, 3),
+           "matched_objective_iterations_median": res["median_matched_iters"],
+           "matched_objective_speedup_median": round(res["matched_speedup_median_of_ratios"], 3),
+           "matched_objective_reached_fraction": res["matched_reached_frac"],
+           "fallback_rate": res["fallback_rate"],
+           "warm_final_objective_rel_to_plain_median": round(res["median_rel_f_gap"], 6),
+           "warm_start_objective_over_plain_final_median": round(res["median_f_warm_start_over_plain_final"], 4),
+           "network_ms_median": round(float(np.median([r["network_ms"] for r in rows])), 3),
+           "fallback_check_ms_median": round(float(np.median([r["select_ms"] for r in rows])), 3),
+           "weights": "mrirecon_amd/weights/unet_grasp.npz", "per_volume": [
+               {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows]}
+    # U-Net forward alone: dense MFMA FLOPs over HIP-event time on the launch stream
+    x = pb.x_adj
+    for _ in range(3):
+        net.infer(x, scales=[1.0])
+    reps = 10
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    torch.cuda.synchronize()
+    ev[0].record()
+    for _ in range(reps):
+        net.infer(x, scales=[1.0])
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    flops = net.flops_per_volume(x.shape[0], x.shape[1], x.shape[2])
+    unet = {"flops_per_volume": flops, "forward_ms": round(ms, 4),
+            "achieved_tflops": round(flops / (ms * 1e-3) / 1e12, 2), "peak_tflops": BF16_PEAK_TFLOPS,
+            "mfma_frac": round(flops / (ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 4),
+            "shape": f"{x.shape[0]} frames x {x.shape[1]}x{x.shape[2]}, levels {net.levels}, base {net.base}, "
+                     f"{sum(L.n_weight for L in net.layers)} weights (BF16, FP32 accumulation)",
+            "train_steps": net.step}
+    return ttt, unet
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,7 +172,8 @@ def main():
     ap.add_argument("--strip-cols", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance NLCG run")
-    ap.add_argument("--tol-iters", type=int, default=150)
+    ap.add_argument("--tol-iters", type=int, default=200)
+    ap.add_argument("--tol-volumes", type=int, default=8, help="held-out volumes of the time-to-tolerance runs")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (reference for PMC byte counters)")
     args = ap.parse_args()
@@ -114,7 +185,7 @@ def main():
     import numpy as np
     import torch
 
-    from mrirecon_amd import NLCGEngine, RegManager, TVND, TVParams, synth
+    from mrirecon_amd import synth
     from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
 
     dev = torch.device("cuda:0")
@@ -187,38 +258,9 @@ def main():
                 "survey_model_frac": round(survey / (pair_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "whole_problem_pairs_per_s": round(1e3 / ms_per_step, 2)}
 
-    ttt = None
+    ttt, unet = None, None
     if not args.no_tol:
-        print("bench: plain NLCG time-to-1e-4 from the adjoint start", file=sys.stderr, flush=True)
-        ynp = y.cpu().numpy()
-        sigma = 0.01 * float(np.abs(ynp).max())
-        plan.forward(x, y)
-        torch.cuda.synchronize()
-        ymeas = y + torch.from_numpy(synth.complex_noise(ynp.shape, sigma, wl.noise_seed)).to(dev)
-        regm = RegManager([TVND("tv_t", TVParams(weight=wl.lam_t, axes="temporal")),
-                           TVND("tv_s", TVParams(weight=wl.lam_s, axes="spatial"))])
-        # PR+ (Polak-Ribiere): the reference default FR stalls on this workload after one step
-        # (its first beta = 1 yields an ascent direction; the oracle reproduces the stop)
-        eng = NLCGEngine(plan, ymeas, problems=1, frames=wl.B, regm=regm, rel_tol=1e-4, direction="prplus",
-                         max_hist=args.tol_iters + 1)
-        x0 = torch.empty_like(x)
-        # warm the graph (capture) outside the timed region
-        plan.adjoint(ymeas, x0)
-        eng.prime(x0)
-        eng.capture()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        plan.adjoint(ymeas, x0)
-        eng.prime(x0)
-        n = eng.run(args.tol_iters, use_graph=True, check_every=1)
-        torch.cuda.synchronize()
-        t_plain = time.perf_counter() - t0
-        h = eng.history()[0]
-        ttt = {"direction": "prplus", "rel_tol": 1e-4, "plain_seconds": round(t_plain, 4),
-               "plain_iterations": len(h), "stop": eng.stop_reasons()[0],
-               "ms_per_iteration": round(t_plain / max(1, n) * 1e3, 3),
-               "f_first": h[0]["f"] if h else None, "f_last": h[-1]["f_step"] if h else None,
-               "warm_start": "not built in this tree (no learned warm start yet)"}
+        ttt, unet = time_to_tolerance(args, dev)
 
     cpu = None
     if not args.no_cpu:
@@ -235,7 +277,7 @@ def main():
                    "coil_frames": ncf, "samples_per_frame": int(wl.uniform_K), "strip_cols": info["strip_cols"],
                    "kb_terms": info["kb_terms"], "parallelism": "1 GPU"},
         "roofline": roofline, "kernels": per_kernel, "pipeline": pipeline,
-        "time_to_tol": ttt, "cpu_baseline": cpu,
+        "time_to_tol": ttt, "unet": unet, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

@@ -154,6 +154,7 @@ typedef struct mrh_cg mrh_cg;
 #define MRH_CG_LS_FAILED 1
 #define MRH_CG_NOT_DESCENT 2
 #define MRH_CG_REL_TOL 3
+#define MRH_CG_TARGET 4       /* objective reached the target set by mrh_cg_set_targets */
 
 typedef struct {
   mrh_tv_desc tv;           /* image layout + regularisers                      */
@@ -223,6 +224,12 @@ int mrh_cg_gradient(mrh_cg* eng, const void* x, void* g, const void* d, const vo
  * graspcg/solvers/cg.py:183-184) */
 int mrh_cg_direction(mrh_cg* eng, void* d, const void* g, void* g_prev, int prime, void* stream);
 
+/* Per-problem objective targets (host double[P]; NULL or NaN entries clear them):
+ * a problem stops with MRH_CG_TARGET once an accepted step has f <= target.
+ * Used to time how long a run takes to reach a given objective (e.g. another
+ * run's final value); not a reference stop rule.  Synchronises `stream`. */
+int mrh_cg_set_targets(mrh_cg* eng, const double* targets, void* stream);
+
 /* Copy status (int[P]) and iteration counts (int[P]) to host (synchronising `stream`). */
 int mrh_cg_read_status(mrh_cg* eng, int* status, int* iters, void* stream);
 /* Copy history (host mrh_cg_hist[n_iter * P], iteration-major) to host. */
@@ -272,9 +279,12 @@ int mrh_unet_pack_input(const void* img, void* out, int64_t npix, int C, float s
 int mrh_unet_unpack_residual(const void* xin, const void* net, void* xout, int64_t npix, int Cn,
                              float inv_scale, void* stream);
 /* loss += weight * sum |(xout - xt) scale|^2 (FP64, device scalar) and
- * gnet[:, 0:2] = 2 weight (xout - xt) scale, gnet[:, 2:] = 0 (BF16 (npix, Cn)). */
-int mrh_unet_mse_grad(const void* xout, const void* xt, void* gnet, double* loss, int64_t npix,
-                      int Cn, float scale, float weight, void* stream);
+ * gnet[:, 0:2] = 2 weight (xout - xt) scale + gx_coef (Re gx, Im gx),
+ * gnet[:, 2:] = 0 (BF16 (npix, Cn)).  gx (complex64, may be NULL) is the
+ * gradient of an extra loss term with respect to xout, e.g. the NLCG
+ * objective's A^H (A xout - y) + grad TV(xout). */
+int mrh_unet_mse_grad(const void* xout, const void* xt, const void* gx, float gx_coef, void* gnet,
+                      double* loss, int64_t npix, int Cn, float scale, float weight, void* stream);
 /* p[0:n] = value (FP32); used to clear gradient accumulators. */
 int mrh_unet_fill(float* p, int64_t n, float value, void* stream);
 /* Adam step (bias-corrected, step >= 1) on the FP32 master weights wm and bias,

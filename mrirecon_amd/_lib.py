@@ -20,7 +20,7 @@ ABI_VERSION = 2
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
 CG_DIR_FR, CG_DIR_PRPLUS = 0, 1
-CG_ACTIVE, CG_LS_FAILED, CG_NOT_DESCENT, CG_REL_TOL = 0, 1, 2, 3
+CG_ACTIVE, CG_LS_FAILED, CG_NOT_DESCENT, CG_REL_TOL, CG_TARGET = 0, 1, 2, 3, 4
 
 
 class NufftDesc(Structure):
@@ -85,6 +85,7 @@ _SIGS = {
     "mrh_cg_accept": (c_int, [c_void_p] + [c_void_p] * 6 + [c_void_p]),
     "mrh_cg_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "mrh_cg_direction": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "mrh_cg_set_targets": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
     "mrh_cg_read_status": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), c_void_p]),
     "mrh_cg_read_history": (c_int, [c_void_p, POINTER(CGHist), c_int, c_void_p]),
     "mrh_unet_conv_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 9 + [c_void_p]),
@@ -96,8 +97,8 @@ _SIGS = {
     "mrh_unet_up2_concat_bwd": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 5 + [c_void_p]),
     "mrh_unet_pack_input": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),
     "mrh_unet_unpack_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),
-    "mrh_unet_mse_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_float,
-                                  c_void_p]),
+    "mrh_unet_mse_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int64, c_int,
+                                  c_float, c_float, c_void_p]),
     "mrh_unet_fill": (c_int, [c_void_p, c_int64, c_float, c_void_p]),
     "mrh_unet_adam": (c_int, [c_void_p] * 10 + [c_int] * 4 + [c_float] * 4 + [c_int, c_void_p]),
 }

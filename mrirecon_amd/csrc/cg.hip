@@ -287,6 +287,7 @@ struct mrh_cg {
   double* d_part_k;   // [P][nb_k][PK_STRIDE]
   ProbState* d_st;
   mrh_cg_hist* d_hist;  // [max_hist][P]
+  double* d_target;     // [P] objective targets (-inf: none)
 };
 
 __global__ void k_cg_reset(ProbState* st, int P) {
@@ -503,7 +504,8 @@ __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, fl
 // Finish gradient reductions: beta, stop rule, history, iteration count.
 __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* part_i, int nb_i, int prime,
                                                 int direction, double rel_tol, double restart_c,
-                                                mrh_cg_hist* hist, int max_hist, int P) {
+                                                const double* __restrict__ target, mrh_cg_hist* hist,
+                                                int max_hist, int P) {
   const int p = blockIdx.x;
   __shared__ double vals[5];
   for (int v = 0; v < 5; ++v) {
@@ -543,6 +545,7 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
   s.iter += 1;
   if (rel_tol > 0.0 && s.have_f_prev && fabs(s.ft - s.f_prev) <= rel_tol * fabs(s.f_prev))
     s.status = MRH_CG_REL_TOL;
+  if (s.ft <= target[p]) s.status = MRH_CG_TARGET;
   s.f_prev = s.ft;
   s.have_f_prev = 1;
 }
@@ -606,7 +609,13 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   ALLOC(e->d_part_k, sizeof(double) * (size_t)e->P * e->nb_k * PK_STRIDE);
   ALLOC(e->d_st, sizeof(ProbState) * e->P);
   ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
+  ALLOC(e->d_target, sizeof(double) * e->P);
 #undef ALLOC
+  {
+    std::vector<double> none(e->P, -HUGE_VAL);
+    err = hipMemcpy(e->d_target, none.data(), sizeof(double) * e->P, hipMemcpyHostToDevice);
+    if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: copy failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
+  }
   err = hipMemcpy(e->d_kprob, d->k_prob, sizeof(int64_t) * (e->P + 1), hipMemcpyHostToDevice);
   if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: copy failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
   hipLaunchKernelGGL(k_cg_reset, dim3((e->P + 63) / 64), dim3(64), 0, 0, e->d_st, e->P);
@@ -618,7 +627,7 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
 
 extern "C" int mrh_cg_destroy(mrh_cg* e) {
   if (!e) return MRH_OK;
-  void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist};
+  void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist, e->d_target};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete e;
   return MRH_OK;
@@ -676,7 +685,7 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
                      (const float2*)d, (const float2*)g_prev, e->tv, e->npix_p, e->d_st, prime, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
-                     e->direction, e->rel_tol, e->restart_c, e->d_hist, e->max_hist, e->P);
+                     e->direction, e->rel_tol, e->restart_c, e->d_target, e->d_hist, e->max_hist, e->P);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -686,6 +695,16 @@ extern "C" int mrh_cg_direction(mrh_cg* e, void* d, const void* g, void* g_prev,
   hipLaunchKernelGGL(k_cg_direction, dim3(e->nb_i, e->P), dim3(RT), 0, (hipStream_t)stream, (float2*)d,
                      (const float2*)g, (float2*)g_prev, e->npix_p, e->d_st, prime);
   MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_set_targets(mrh_cg* e, const double* targets, void* stream) {
+  MRH_REQUIRE(e, MRH_EINVAL, "cg_set_targets: null engine");
+  std::vector<double> t(e->P, -HUGE_VAL);
+  if (targets)
+    for (int p = 0; p < e->P; ++p) t[p] = std::isnan(targets[p]) ? -HUGE_VAL : targets[p];
+  MRH_HIP(hipStreamSynchronize((hipStream_t)stream));
+  MRH_HIP(hipMemcpy(e->d_target, t.data(), sizeof(double) * e->P, hipMemcpyHostToDevice));
   return MRH_OK;
 }
 

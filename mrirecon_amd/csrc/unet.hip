@@ -334,6 +334,7 @@ __global__ void k_unpack_residual(const float2* __restrict__ xin, const u16* __r
 }
 
 __global__ __launch_bounds__(256) void k_mse_grad(const float2* __restrict__ xout, const float2* __restrict__ xt,
+                                                  const float2* __restrict__ gx, float gx_coef,
                                                   u16* __restrict__ gnet, int64_t npix, int Cn, float scale,
                                                   float w, double* __restrict__ loss) {
   double s = 0.0;
@@ -341,9 +342,15 @@ __global__ __launch_bounds__(256) void k_mse_grad(const float2* __restrict__ xou
     const float2 a = xout[i], b = xt[i];
     const float ex = (a.x - b.x) * scale, ey = (a.y - b.y) * scale;
     s += (double)(ex * ex + ey * ey) * w;
+    float gr = 2.f * w * ex, gi = 2.f * w * ey;
+    if (gx) {
+      const float2 q = gx[i];
+      gr = fmaf(gx_coef, q.x, gr);
+      gi = fmaf(gx_coef, q.y, gi);
+    }
     u16* o = gnet + i * Cn;
-    o[0] = f2bf(2.f * w * ex);
-    o[1] = f2bf(2.f * w * ey);
+    o[0] = f2bf(gr);
+    o[1] = f2bf(gi);
     for (int c = 2; c < Cn; ++c) o[c] = 0;
   }
   s = warp_sum_d(s);
@@ -523,11 +530,12 @@ extern "C" int mrh_unet_unpack_residual(const void* xin, const void* net, void* 
   return MRH_OK;
 }
 
-extern "C" int mrh_unet_mse_grad(const void* xout, const void* xt, void* gnet, double* loss, int64_t npix, int Cn,
-                                 float scale, float weight, void* stream) {
+extern "C" int mrh_unet_mse_grad(const void* xout, const void* xt, const void* gx, float gx_coef, void* gnet,
+                                 double* loss, int64_t npix, int Cn, float scale, float weight, void* stream) {
   MRH_REQUIRE(xout && xt && gnet && loss && npix >= 1 && Cn >= 2, MRH_EINVAL, "unet_mse_grad: bad arguments");
   hipLaunchKernelGGL(k_mse_grad, dim3(std::min<unsigned>(grid_for(npix), 2048)), dim3(256), 0, (hipStream_t)stream,
-                     (const float2*)xout, (const float2*)xt, (u16*)gnet, npix, Cn, scale, weight, loss);
+                     (const float2*)xout, (const float2*)xt, (const float2*)gx, gx_coef, (u16*)gnet, npix, Cn,
+                     scale, weight, loss);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

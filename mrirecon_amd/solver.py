@@ -34,7 +34,7 @@ import torch
 from . import _lib
 from .regularization import RegManager
 
-_STATUS = {0: "active", 1: "ls_failed", 2: "not_descent", 3: "rel_tol"}
+_STATUS = {0: "active", 1: "ls_failed", 2: "not_descent", 3: "rel_tol", 4: "target"}
 
 
 class NLCGEngine:
@@ -141,6 +141,17 @@ class NLCGEngine:
                 if not self.active().any():
                     break
         return k
+
+    def set_targets(self, targets: Optional[Sequence[float]]) -> None:
+        """Stop problem p once an accepted step reaches f <= targets[p] (None clears)."""
+        if targets is None:
+            _lib.check(self._lib.mrh_cg_set_targets(self.handle, None, self._st()))
+            return
+        t = np.ascontiguousarray(np.asarray(targets, dtype=np.float64).reshape(-1))
+        if t.size != self.P:
+            raise ValueError(f"need {self.P} targets, got {t.size}")
+        _lib.check(self._lib.mrh_cg_set_targets(self.handle, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                                self._st()))
 
     # ------------------------------------------------------------------ readout
     def status(self):

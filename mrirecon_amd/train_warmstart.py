@@ -6,16 +6,18 @@
 Data: seeded dynamic phantoms of the GRASP generator (256^2 x 34 frames,
 12 coils, 21 golden-angle spokes x 512 samples per frame, complex Gaussian noise
 of 1 % of max |y|).  Network input: the density-compensated adjoint image
-A^H y.  Target: plain PR+ NLCG from that adjoint start to a relative objective
-change below 1e-4 (temporal TV 0.02 + spatial TV 0.002), with the beta = 0
-restart whenever the next direction would not descend (without it PR+ stops
-on a non-descent direction after a few iterations, far from the tolerance) --
-the reconstruction the warm start is meant to shortcut.  Training: Adam with a cosine learning
-rate, one volume per step, MSE in the per-volume scaled domain, every pass on
-the library's BF16 MFMA kernels.  Evaluation on held-out seeds: wall time and
-iterations of NLCG to the same stop rule from the adjoint start (plain) and
-from the fallback-checked warm start (adjoint + network + objective check
-included in the warm time).
+A^H y.  Target: the reconstruction of a fixed-length PR+ NLCG run (150
+iterations by default) from that adjoint start (temporal TV 0.02 + spatial TV
+0.002), with the beta = 0 restart whenever the next direction would not descend
+(without it PR+ stops on a non-descent direction after a few iterations).
+Training: Adam with a cosine learning rate, one volume per step, MSE in the
+per-volume scaled domain plus, optionally, the NLCG objective at the network
+output relative to the target's (its gradient A^H (A x - y) + grad TV comes from
+the same kernels as the solver's), every pass on the library's BF16 MFMA kernels.  Evaluation on held-out seeds: wall time and
+iterations of NLCG to the relative-change stop (1e-4) from the adjoint start
+(plain) and from the fallback-checked warm start (adjoint + network + objective
+check included in the warm time), and the time the warm start needs to reach
+the plain run's final objective (within 1e-4).
 """
 from __future__ import annotations
 
@@ -41,7 +43,7 @@ class GraspProblem:
     """One GRASP volume on the device: plan, measurement buffer, NLCG engine, objective."""
 
     def __init__(self, device="cuda:0", rel_tol: float = 1e-4, max_iter: int = 200,
-                 restart_c: Optional[float] = 0.0):
+                 restart_c: Optional[float] = 0.0, long_iters: int = 150):
         import torch
 
         from . import NLCGEngine, RegManager, TVND, TVParams, synth
@@ -64,12 +66,18 @@ class GraspProblem:
         self.regm = RegManager([TVND("tv_t", TVParams(weight=g["lam_t"], axes="temporal")),
                                 TVND("tv_s", TVParams(weight=g["lam_s"], axes="spatial"))])
         self.max_iter = int(max_iter)
+        self.long_iters = int(long_iters)
+        # stop rule run (relative objective change) and fixed-length / target run (no change rule)
         self.eng = NLCGEngine(self.plan, self.y, problems=1, frames=g["F"], regm=self.regm, rel_tol=rel_tol,
                               direction="prplus", max_hist=self.max_iter + 1, restart_c=restart_c)
+        self.eng_long = NLCGEngine(self.plan, self.y, problems=1, frames=g["F"], regm=self.regm, rel_tol=0.0,
+                                   direction="prplus", max_hist=max(self.long_iters, self.max_iter) + 1,
+                                   restart_c=restart_c)
         self.objective = ObjectiveEval(self.plan, self.y, self.regm, problems=1, frames=g["F"])
         self.x0 = torch.zeros_like(self.x_true)
-        self.eng.prime(self.x_adj)
-        self.eng.capture()
+        for e in (self.eng, self.eng_long):
+            e.prime(self.x_adj)
+            e.capture()
 
     def measure(self, x_true: np.ndarray, noise_seed: int) -> None:
         """y = A x_true + CN(0, sigma^2), sigma = 1 % of max |A x_true| (seeded)."""
@@ -87,15 +95,15 @@ class GraspProblem:
         self.plan.adjoint(self.y, out)
         return out
 
-    def solve(self, x0) -> Dict[str, object]:
-        """NLCG from x0 to the stop rule; the engine's x holds the result."""
-        eng = self.eng
+    def solve(self, x0, long: bool = False, iters: Optional[int] = None) -> int:
+        """NLCG from x0: to the relative-change stop (``long`` False) or for ``iters``
+        iterations / until the engine's objective target (``long`` True)."""
+        eng = self.eng_long if long else self.eng
         eng.prime(x0)
-        n = eng.run(self.max_iter, use_graph=True, check_every=1)
-        return {"steps": n}
+        return eng.run(iters or (self.long_iters if long else self.max_iter), use_graph=True, check_every=1)
 
-    def history(self):
-        return self.eng.history()[0]
+    def history(self, long: bool = False):
+        return (self.eng_long if long else self.eng).history()[0]
 
 
 def _sync(torch):
@@ -116,9 +124,15 @@ def run_plain(pb: GraspProblem) -> Dict[str, float]:
             "stop": pb.eng.stop_reasons()[0]}
 
 
-def run_warm(pb: GraspProblem, net) -> Dict[str, float]:
+def run_warm(pb: GraspProblem, net, target: Optional[float] = None) -> Dict[str, float]:
+    """Adjoint -> network -> fallback check -> NLCG, all inside the timed region.
+
+    ``target`` None: NLCG to the relative-change stop.  Otherwise: NLCG without
+    the change rule until f <= target (or the long-run iteration cap)."""
     from .warmstart import select_warm_start
     torch = pb.torch
+    eng = pb.eng_long if target is not None else pb.eng
+    eng.set_targets(None if target is None else [target])
     _sync(torch)
     t0 = time.perf_counter()
     pb.adjoint()
@@ -129,13 +143,15 @@ def run_warm(pb: GraspProblem, net) -> Dict[str, float]:
     t_net = time.perf_counter()
     x0, acc, f_w, f_a = select_warm_start(pb.objective, pb.x_adj, x_warm, out=pb.x0)
     t_sel = time.perf_counter()
-    pb.solve(x0)
+    pb.solve(x0, long=target is not None, iters=pb.max_iter if target is not None else None)
     _sync(torch)
     t = time.perf_counter() - t0
-    h = pb.history()
+    h = pb.history(long=target is not None)
+    stop = eng.stop_reasons()[0]
+    eng.set_targets(None)
     return {"seconds": t, "iterations": len(h), "f_final": h[-1]["f_step"] if h else float("nan"),
             "f_start": float(f_w[0]), "f_adjoint": float(f_a[0]), "accepted": bool(acc[0]),
-            "f_path": [r["f_step"] for r in h], "stop": pb.eng.stop_reasons()[0],
+            "f_path": [r["f_step"] for r in h], "stop": stop,
             "adjoint_ms": (t_adj - t0) * 1e3, "network_ms": (t_net - t_adj) * 1e3,
             "select_ms": (t_sel - t_net) * 1e3}
 
@@ -155,22 +171,35 @@ def evaluate(pb: GraspProblem, net, seeds: Sequence[int], phantoms, log=print) -
         p = run_plain(pb)
         w = run_warm(pb, net)
         target = p["f_final"] * (1 + 1e-4)
+        m = run_warm(pb, net, target=target)
         rows.append({"seed": int(seed), "plain_s": p["seconds"], "plain_iters": p["iterations"],
                      "warm_s": w["seconds"], "warm_iters": w["iterations"], "accepted": w["accepted"],
+                     "matched_s": m["seconds"], "matched_iters": m["iterations"],
+                     "matched_reached": m["stop"] == "target",
                      "f_plain_final": p["f_final"], "f_warm_final": w["f_final"], "f_adjoint": w["f_adjoint"],
                      "f_warm_start": w["f_start"], "network_ms": w["network_ms"], "select_ms": w["select_ms"],
                      "adjoint_ms": w["adjoint_ms"], "warm_iters_to_plain_f": iterations_to_reach(w["f_path"], target),
                      "plain_stop": p["stop"], "warm_stop": w["stop"]})
         r = rows[-1]
         log(f"eval seed {seed}: plain {r['plain_iters']} it {r['plain_s'] * 1e3:.1f} ms | warm {r['warm_iters']} it "
-            f"{r['warm_s'] * 1e3:.1f} ms (net {r['network_ms']:.2f} ms, accepted {r['accepted']}) | "
-            f"f {r['f_plain_final']:.6g} vs {r['f_warm_final']:.6g}")
+            f"{r['warm_s'] * 1e3:.1f} ms (net {r['network_ms']:.2f} ms, accepted {r['accepted']}) | matched "
+            f"{r['matched_iters']} it {r['matched_s'] * 1e3:.1f} ms ({'reached' if r['matched_reached'] else 'NOT reached'}) | "
+            f"f plain {r['f_plain_final']:.6g} warm {r['f_warm_final']:.6g} warm-start {r['f_warm_start']:.6g} "
+            f"adjoint {r['f_adjoint']:.6g}")
     ps = np.array([r["plain_s"] for r in rows])
     ws = np.array([r["warm_s"] for r in rows])
+    ms = np.array([r["matched_s"] for r in rows])
     return {
         "n": len(rows), "median_plain_s": float(np.median(ps)), "median_warm_s": float(np.median(ws)),
         "speedup_median_of_ratios": float(np.median(ps / ws)),
         "speedup_total": float(ps.sum() / ws.sum()),
+        "median_matched_s": float(np.median(ms)),
+        "matched_speedup_median_of_ratios": float(np.median(ps / ms)),
+        "matched_speedup_total": float(ps.sum() / ms.sum()),
+        "matched_reached_frac": float(np.mean([r["matched_reached"] for r in rows])),
+        "median_matched_iters": float(np.median([r["matched_iters"] for r in rows])),
+        "median_f_warm_start_over_plain_final": float(np.median([r["f_warm_start"] / r["f_plain_final"]
+                                                                 for r in rows])),
         "median_plain_iters": float(np.median([r["plain_iters"] for r in rows])),
         "median_warm_iters": float(np.median([r["warm_iters"] for r in rows])),
         "fallback_rate": float(np.mean([not r["accepted"] for r in rows])),
@@ -198,6 +227,10 @@ def main(argv=None) -> int:
     ap.add_argument("--log", default=None, help="JSON log")
     ap.add_argument("--levels", type=int, default=4)
     ap.add_argument("--base", type=int, default=32)
+    ap.add_argument("--obj-weight", type=float, default=0.0,
+                    help="weight of the objective term f(x_warm) / f(target) in the loss (0: MSE only)")
+    ap.add_argument("--target-iters", type=int, default=150,
+                    help="NLCG iterations (no change rule) of each training target")
     args = ap.parse_args(argv)
 
     def log(*a):
@@ -207,17 +240,18 @@ def main(argv=None) -> int:
     pool = mp.get_context("fork").Pool(max(1, args.workers))
     import torch
 
-    from .warmstart import UNet2DT
+    from .warmstart import ObjectiveGrad, UNet2DT
 
-    pb = GraspProblem()
+    pb = GraspProblem(long_iters=args.target_iters)
     g = GRASP
     train_seeds = list(range(args.seed0, args.seed0 + args.train))
     test_seeds = list(range(args.seed0 + 500_000, args.seed0 + 500_000 + args.test))
     shape = (g["F"], g["S"], g["S"])
     X_adj = torch.empty((args.train,) + shape, dtype=torch.complex64, device=pb.dev)
     X_tgt = torch.empty_like(X_adj)
+    Y = torch.empty((args.train, pb.plan.n_k), dtype=torch.complex64, device=pb.dev) if args.obj_weight else None
+    f_norm = np.zeros(args.train)
     t0 = time.perf_counter()
-    plain_iters, plain_s = [], []
     chunks = _chunks(train_seeds, 48)
     pending = pool.map_async(_phantom, chunks[0]) if chunks else None
     i = 0
@@ -227,14 +261,16 @@ def main(argv=None) -> int:
             pending = pool.map_async(_phantom, chunks[c + 1])
         for seed, x_true in zip(chunk, data):
             pb.measure(x_true, noise_seed=seed + 7)
-            r = run_plain(pb)
+            pb.adjoint()
+            pb.solve(pb.x_adj, long=True)
             X_adj[i].copy_(pb.x_adj)
-            X_tgt[i].copy_(pb.eng.x)
-            plain_iters.append(r["iterations"])
-            plain_s.append(r["seconds"])
+            X_tgt[i].copy_(pb.eng_long.x)
+            f_norm[i] = pb.history(long=True)[-1]["f_step"]
+            if Y is not None:
+                Y[i].copy_(pb.y)
             i += 1
-        log(f"targets {i}/{args.train}: plain NLCG median {np.median(plain_iters):.0f} it, "
-            f"{np.median(plain_s) * 1e3:.1f} ms ({time.perf_counter() - t0:.0f} s)")
+        log(f"targets {i}/{args.train}: {pb.long_iters}-iteration NLCG reconstructions "
+            f"({time.perf_counter() - t0:.0f} s)")
     t_data = time.perf_counter() - t0
 
     net = UNet2DT.load(args.init, device=pb.dev) if args.init else UNet2DT(args.levels, args.base, device=pb.dev,
@@ -243,24 +279,32 @@ def main(argv=None) -> int:
     rng = np.random.default_rng(2)
     order = np.concatenate([rng.permutation(args.train) for _ in range(int(math.ceil(args.epochs)) + 1)])[:n_steps]
     warm = min(100, max(1, n_steps // 20))
-    losses = []
+    losses, gaps = [], []
+    objective = ObjectiveGrad(pb.plan, pb.regm, g["F"]) if args.obj_weight else None
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     pend = []
     for s, j in enumerate(order):
+        j = int(j)
         lr = args.lr * min(1.0, (s + 1) / warm) * 0.5 * (1 + math.cos(math.pi * s / max(1, n_steps)))
-        pend.append(net.train_step(X_adj[int(j)], X_tgt[int(j)], lr=lr).clone())
+        if objective is not None:
+            pend.append(net.train_step(X_adj[j], X_tgt[j], lr=lr, objective=objective, y=Y[j],
+                                       obj_weight=args.obj_weight, f_norm=[f_norm[j]]).clone())
+            gaps.append(objective.value() / f_norm[j] - 1.0)
+        else:
+            pend.append(net.train_step(X_adj[j], X_tgt[j], lr=lr).clone())
         if (s + 1) % 50 == 0 or s + 1 == n_steps:
             vals = [float(v.item()) for v in pend]
             pend = []
             losses.extend(vals)
-            log(f"train step {s + 1}/{n_steps}: loss {np.mean(vals):.5f} lr {lr:.2e} "
+            gap = f" objective gap {np.mean(gaps[-len(vals):]):.4f}" if gaps else ""
+            log(f"train step {s + 1}/{n_steps}: mse {np.mean(vals):.5f}{gap} lr {lr:.2e} "
                 f"({(time.perf_counter() - t1) / (s + 1) * 1e3:.1f} ms/step)")
     torch.cuda.synchronize()
     t_train = time.perf_counter() - t1
     net.save(args.out)
     log(f"saved {args.out}")
-    del X_adj, X_tgt
+    del X_adj, X_tgt, Y
     torch.cuda.empty_cache()
 
     # held-out evaluation
@@ -270,11 +314,14 @@ def main(argv=None) -> int:
         res = evaluate(pb, net, test_seeds, test_ph, log=log)
         log(f"held-out: plain median {res['median_plain_s'] * 1e3:.1f} ms / {res['median_plain_iters']:.0f} it, warm "
             f"median {res['median_warm_s'] * 1e3:.1f} ms / {res['median_warm_iters']:.0f} it, speedup "
-            f"{res['speedup_median_of_ratios']:.2f}x, fallback {res['fallback_rate']:.2%}")
+            f"{res['speedup_median_of_ratios']:.2f}x, fallback {res['fallback_rate']:.2%}; to the plain objective: "
+            f"median {res['median_matched_s'] * 1e3:.1f} ms / {res['median_matched_iters']:.0f} it, speedup "
+            f"{res['matched_speedup_median_of_ratios']:.2f}x, reached {res['matched_reached_frac']:.0%}")
     pool.close()
     out = {"train": args.train, "test": args.test, "epochs": args.epochs, "lr": args.lr, "steps": n_steps,
            "data_seconds": t_data, "train_seconds": t_train, "ms_per_step": t_train / max(1, n_steps) * 1e3,
-           "plain_train_median_iters": float(np.median(plain_iters)) if plain_iters else None,
+           "target_iters": args.target_iters, "obj_weight": args.obj_weight,
+           "gap_curve": [float(np.mean(gaps[k:k + 50])) for k in range(0, len(gaps), 50)],
            "loss_curve": [float(np.mean(losses[k:k + 50])) for k in range(0, len(losses), 50)],
            "eval": res}
     if args.log:

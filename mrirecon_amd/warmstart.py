@@ -370,9 +370,16 @@ class UNet2DT:
 
     @torch.no_grad()
     def loss_and_grad(self, x_adj: torch.Tensor, x_target: torch.Tensor, nvol: int = 1,
-                      scales: Optional[Sequence[float]] = None) -> torch.Tensor:
-        """Forward + backward of mean |(x_warm - x_target) scale|^2; gradients in ``self.grads``.
-        Returns the loss as a device FP64 scalar (no host synchronisation)."""
+                      scales: Optional[Sequence[float]] = None, objective: Optional["ObjectiveGrad"] = None,
+                      y: Optional[torch.Tensor] = None, obj_weight: float = 0.0,
+                      f_norm: Optional[Sequence[float]] = None) -> torch.Tensor:
+        """Forward + backward of
+
+            L = mean |(x_warm - x_target) scale|^2  +  obj_weight * sum_v f_v(x_warm) / f_norm[v]
+
+        (f = the NLCG objective of k-space ``y``, evaluated by ``objective``; the second
+        term is optional).  Gradients land in ``self.grads``; returns the MSE term as a
+        device FP64 scalar (no host synchronisation; ``objective.value()`` reads f)."""
         F, H, W = self._check_image(x_adj, nvol)
         if x_target.shape != x_adj.shape or x_target.dtype != torch.complex64 or not x_target.is_contiguous():
             raise ValueError("x_target must match x_adj (contiguous complex64)")
@@ -382,20 +389,29 @@ class UNet2DT:
         self._pack(ws, x_adj, scales, F)
         self._forward(ws, nvol, F, H, W)
         self._unpack(ws, x_adj, ws["x_warm"], scales, F)
+        gx = None
+        if objective is not None and obj_weight:
+            if y is None or f_norm is None or len(f_norm) != nvol:
+                raise ValueError("the objective term needs y and one f_norm per volume")
+            gx = objective(ws["x_warm"], y)
         _lib.check(self.lib.mrh_unet_fill(ws["loss"].data_ptr(), 2, 0.0, self._st()))   # 2 floats = 1 double
         per = F * H * W
         npix = nvol * per
         for v, s in enumerate(scales):
+            coef = float(obj_weight / (f_norm[v] * s)) if gx is not None else 0.0
             _lib.check(self.lib.mrh_unet_mse_grad(ws["x_warm"].reshape(-1)[v * per:].data_ptr(),
                                                   x_target.reshape(-1)[v * per:].data_ptr(),
-                                                  ws["g_out"][v * per:].data_ptr(), ws["loss"].data_ptr(), per,
-                                                  self.out_ch, float(s), float(1.0 / npix), self._st()))
+                                                  None if gx is None else gx.reshape(-1)[v * per:].data_ptr(),
+                                                  coef, ws["g_out"][v * per:].data_ptr(), ws["loss"].data_ptr(),
+                                                  per, self.out_ch, float(s), float(1.0 / npix), self._st()))
         self._backward(ws, nvol, F, H, W)
         return ws["loss"]
 
     @torch.no_grad()
-    def train_step(self, x_adj: torch.Tensor, x_target: torch.Tensor, lr: float, nvol: int = 1) -> torch.Tensor:
-        loss = self.loss_and_grad(x_adj, x_target, nvol)
+    def train_step(self, x_adj: torch.Tensor, x_target: torch.Tensor, lr: float, nvol: int = 1,
+                   **objective_term) -> torch.Tensor:
+        """One Adam step on :meth:`loss_and_grad` (keyword arguments select the objective term)."""
+        loss = self.loss_and_grad(x_adj, x_target, nvol, **objective_term)
         self.step += 1
         self._adam(lr)
         return loss
@@ -434,6 +450,43 @@ class ObjectiveEval:
         _lib.check(self.lib.mrh_tv_energy_grad(ctypes.byref(self.tv), x.data_ptr(), None, self.work.data_ptr(),
                                                self.e_tv.data_ptr(), st))
         return 0.5 * self.e_data.cpu().numpy() + self.e_tv.cpu().numpy()
+
+
+class ObjectiveGrad:
+    """g = A^H (A x - y) + grad TV(x) and f = 1/2 |A x - y|^2 + TV(x) for one problem, on the
+    device -- the gradient the NLCG engine computes (graspcg/ops/objective.py:184-291 at t = 0),
+    used as the gradient of the objective term of the warm-start training loss."""
+
+    def __init__(self, plan, regm, frames: int):
+        from .regularization import RegManager
+        self.plan, self.F = plan, int(frames)
+        self.lib = _lib.load()
+        dev = plan.maps.device
+        S0, S1 = plan.S
+        self.Ax = torch.empty(plan.n_k, dtype=torch.complex64, device=dev)
+        self.r = torch.empty_like(self.Ax)
+        self.g = torch.empty((plan.B, S0, S1), dtype=torch.complex64, device=dev)
+        self.work = torch.zeros(int(self.lib.mrh_reduce_workspace_len()), dtype=torch.float64, device=dev)
+        self.energy = torch.zeros(2, dtype=torch.float64, device=dev)
+        if plan.B != self.F:
+            raise ValueError("ObjectiveGrad handles one problem (plan frames == frames)")
+        self.tv = (regm or RegManager([])).tv_desc(1, self.F, S0, S1)
+
+    def __call__(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        y = y.reshape(-1)
+        self.plan.forward(x, self.Ax)
+        _lib.check(self.lib.mrh_residual_energy(self.r.data_ptr(), self.Ax.data_ptr(), self.Ax.data_ptr(),
+                                                y.data_ptr(), 0.0, y.numel(), self.work.data_ptr(),
+                                                self.energy[0:1].data_ptr(), st))
+        self.plan.adjoint(self.r, self.g)
+        _lib.check(self.lib.mrh_tv_energy_grad(ctypes.byref(self.tv), x.data_ptr(), self.g.data_ptr(),
+                                               self.work.data_ptr(), self.energy[1:2].data_ptr(), st))
+        return self.g
+
+    def value(self) -> float:
+        e = self.energy.cpu().numpy()
+        return float(0.5 * e[0] + e[1])
 
 
 def select_warm_start(objective: ObjectiveEval, x_adj: torch.Tensor, x_warm: torch.Tensor,

@@ -238,3 +238,29 @@ def test_restart_safeguard_matches_oracle():
     assert abs(len(hist) - ref.n_iter) <= 1
     assert abs(f_gpu - ref.f_final) <= 1e-4 * abs(ref.f_final)
     assert all(h["gdot"] < 0 for h in hist)
+
+
+def test_objective_target_stop():
+    """mrh_cg_set_targets: a run stops at the first accepted step with f <= target, i.e. the
+    iteration where an identical run without a target first reached that value."""
+    wl = small_dynamic(seed=23, F=5)
+    be, ora = prepare(wl)
+    regm, _ = regs_for(wl)
+    x0 = c64(ora.adjoint(wl.y))
+    eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, direction="prplus",
+                     restart_c=0.0, max_hist=40)
+    eng.prime(x0)
+    eng.run(12)
+    h = eng.history()[0]
+    assert eng.stop_reasons() == ["active"] and len(h) == 12
+    target = h[6]["f_step"]
+    eng.set_targets([target])
+    eng.prime(x0)
+    eng.run(12)
+    h2 = eng.history()[0]
+    assert eng.stop_reasons() == ["target"] and len(h2) == 7
+    assert h2[-1]["f_step"] == target
+    eng.set_targets(None)
+    eng.prime(x0)
+    eng.run(12)
+    assert eng.stop_reasons() == ["active"]

@@ -281,3 +281,48 @@ def test_fallback_rejects_a_worse_warm_start():
     # objective value = 1/2 |A x - y|^2 + TV: zero data term at the truth
     e = regm.energy_and_grad_tensors(xt[:F].contiguous(), None).cpu().numpy()
     assert abs(f_w[0] - e[0]) <= 1e-6 * abs(e[0])
+
+
+def test_objective_gradient_and_loss_term():
+    """ObjectiveGrad returns the engine's gradient A^H (A x - y) + grad TV and f; the loss
+    kernel adds coef * (Re g, Im g) to the MSE gradient."""
+    from mrirecon_amd import NLCGEngine, RegManager, TVND, TVParams
+    from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+    from mrirecon_amd.warmstart import ObjectiveGrad
+    F, S, C = 4, 64, 4
+    traj, dcf, off = synth.radial_frames([13] * F, 128)
+    x_true = synth.dynamic_phantom(S, F, seed=70, n_ellipses=5)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=6, osf=2.0)
+    be.prepare((C, S, S), torch.from_numpy(traj), torch.from_numpy(dcf),
+               torch.from_numpy(synth.gaussian_maps(C, S).astype(np.complex64)).to(DEV), device=DEV, k_offsets=off)
+    plan = be.plan
+    xt = torch.from_numpy(x_true).to(DEV)
+    y = torch.empty(plan.n_k, dtype=torch.complex64, device=DEV)
+    plan.forward(xt, y)
+    y += 0.01 * y.abs().max() * torch.randn(y.shape, dtype=torch.complex64, device=DEV, generator=torch.Generator(DEV).manual_seed(1))
+    x = torch.empty_like(xt)
+    plan.adjoint(y, x)
+    regm = RegManager([TVND("t", TVParams(weight=0.02, axes="temporal")), TVND("s", TVParams(weight=0.002, axes="spatial"))])
+    og = ObjectiveGrad(plan, regm, F)
+    g = og(x, y).clone()
+    eng = NLCGEngine(plan, y, problems=1, frames=F, regm=regm, direction="prplus")
+    eng.prime(x)
+    torch.cuda.synchronize()
+    assert rel_l2(g.cpu().numpy(), eng.g.cpu().numpy()) <= 1e-6
+    f = og.value()
+    obj = ObjectiveEval(plan, y, regm, problems=1, frames=F)
+    assert abs(f - obj(x)[0]) <= 1e-9 * abs(f)
+    # loss kernel with the extra gradient term
+    lib = _lib.load()
+    xt_img = torch.zeros_like(x)
+    gnet = torch.empty((x.numel(), 8), dtype=torch.bfloat16, device=DEV)
+    loss = torch.zeros(1, dtype=torch.float64, device=DEV)
+    s, w, coef = 0.7, 1.0 / x.numel(), 0.3
+    _lib.check(lib.mrh_unet_mse_grad(x.data_ptr(), xt_img.data_ptr(), g.data_ptr(), coef, gnet.data_ptr(),
+                                     loss.data_ptr(), x.numel(), 8, s, w, st()))
+    xn, gn = x.cpu().numpy().reshape(-1), g.cpu().numpy().reshape(-1)
+    ref = np.stack([2 * w * xn.real * s + coef * gn.real, 2 * w * xn.imag * s + coef * gn.imag], axis=1)
+    got = gnet.float().cpu().numpy()
+    check_bf16_close(got[:, :2].astype(np.float64), ref.astype(np.float64))
+    assert np.all(got[:, 2:] == 0)
+    assert abs(loss.item() - w * s * s * np.sum(np.abs(xn) ** 2)) <= 1e-5 * loss.item()
