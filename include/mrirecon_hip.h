@@ -61,6 +61,7 @@ typedef struct {
   int G0, G1, W, strip_cols, n_strips, kb_terms;
   double beta;
   int64_t n_samples, n_entries, max_strip_entries, workspace_bytes;
+  int coil_chunk;           /* coils per strip workgroup (mrh_nufft_set_coil_chunk)    */
 } mrh_nufft_info;
 
 /* Build a plan: footprints, strip binning, KB coefficients, deapodisation,
@@ -69,6 +70,14 @@ typedef struct {
 int mrh_nufft_plan_create(const mrh_nufft_desc* desc, mrh_nufft_plan** plan);
 int mrh_nufft_plan_destroy(mrh_nufft_plan* plan);
 int mrh_nufft_plan_info(const mrh_nufft_plan* plan, mrh_nufft_info* info);
+
+/* Work split of the strip kernels: each workgroup handles one (frame, strip)
+ * and `coils_per_block` coils (default C: all coils, one workgroup per
+ * (frame, strip)).  Smaller chunks give more workgroups for batches of few
+ * frames; results are bit-identical for every value.  Chosen per plan by the
+ * autotuner (NufftPlan.autotune, which times each candidate).  No reference
+ * counterpart (the reference NUFFT is a third-party call, torchkb.py:249,309). */
+int mrh_nufft_set_coil_chunk(mrh_nufft_plan* plan, int coils_per_block);
 
 /* Forward A: x (B*like, S0, S1) coil-combined image (or (B*like, C, S0, S1)
  * coil images when x_has_coils) -> y, frame b / like l block at

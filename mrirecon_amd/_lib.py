@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -37,7 +37,7 @@ class NufftInfo(Structure):
     _fields_ = [
         ("G0", c_int), ("G1", c_int), ("W", c_int), ("strip_cols", c_int), ("n_strips", c_int),
         ("kb_terms", c_int), ("beta", c_double), ("n_samples", c_int64), ("n_entries", c_int64),
-        ("max_strip_entries", c_int64), ("workspace_bytes", c_int64),
+        ("max_strip_entries", c_int64), ("workspace_bytes", c_int64), ("coil_chunk", c_int),
     ]
 
 
@@ -68,6 +68,7 @@ _SIGS = {
     "mrh_nufft_plan_create": (c_int, [POINTER(NufftDesc), POINTER(c_void_p)]),
     "mrh_nufft_plan_destroy": (c_int, [c_void_p]),
     "mrh_nufft_plan_info": (c_int, [c_void_p, POINTER(NufftInfo)]),
+    "mrh_nufft_set_coil_chunk": (c_int, [c_void_p, c_int]),
     "mrh_nufft_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "mrh_nufft_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "mrh_nufft_forward_stage": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),

@@ -62,6 +62,7 @@ __device__ __forceinline__ float kb_weight(const KBPoly& kb, float d) {
 // ---------------------------------------------------------------- plan
 struct mrh_nufft_plan {
   int S0, S1, G0, G1, W, C, B, like_max, Tc, nstrips, np;
+  int cchunk;   // coils per strip workgroup (autotuned; C = one workgroup per (frame, strip))
   double beta;
   int64_t Ktot, n_entries;
   int max_strip_entries;
@@ -156,6 +157,7 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   std::memset(p, 0, sizeof(*p));
   p->S0 = d->S0; p->S1 = d->S1; p->G0 = d->G0; p->G1 = d->G1; p->W = d->W;
   p->C = d->C; p->B = d->B; p->like_max = d->like; p->Tc = Tc; p->nstrips = d->G1 / Tc;
+  p->cchunk = d->C;
   (void)hipGetDevice(&p->device);
 
   // KB parameters
@@ -286,12 +288,21 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
 
 extern "C" int mrh_nufft_plan_destroy(mrh_nufft_plan* p) { return free_plan(p); }
 
+extern "C" int mrh_nufft_set_coil_chunk(mrh_nufft_plan* p, int coils_per_block) {
+  MRH_REQUIRE(p, MRH_EINVAL, "set_coil_chunk: null plan");
+  MRH_REQUIRE(coils_per_block >= 1 && coils_per_block <= p->C, MRH_EINVAL,
+              "coils per strip workgroup must be in [1, %d] (got %d)", p->C, coils_per_block);
+  p->cchunk = coils_per_block;
+  return MRH_OK;
+}
+
 extern "C" int mrh_nufft_plan_info(const mrh_nufft_plan* p, mrh_nufft_info* info) {
   MRH_REQUIRE(p && info, MRH_EINVAL, "plan_info: null argument");
   info->G0 = p->G0; info->G1 = p->G1; info->W = p->W; info->strip_cols = p->Tc;
   info->n_strips = p->nstrips; info->kb_terms = p->np; info->beta = p->beta;
   info->n_samples = p->Ktot; info->n_entries = p->n_entries;
   info->max_strip_entries = p->max_strip_entries;
+  info->coil_chunk = p->cchunk;
   info->workspace_bytes = (int64_t)(p->T_elems * sizeof(float2));
   return MRH_OK;
 }
@@ -439,10 +450,11 @@ template <int G0, int WT>
 __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
     const float2* __restrict__ T, float2* __restrict__ y, const int2* __restrict__ em,
     const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt) {
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk) {
   constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
+  const int cb = blockIdx.z * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
   __shared__ float2 smem2[2 * TC * CS];   // double buffer: coil c fills one half while c-1 is read
   __shared__ float2 stw[G0];
   const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
@@ -475,8 +487,8 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
       pre[t] = e < ndata ? Tb[e] : make_float2(0.f, 0.f);
     }
   };
-  fetch(0);
-  for (int c = 0; c < C; ++c) {
+  fetch(cb);
+  for (int c = cb; c < ce; ++c) {
     float2* smem = smem2 + (c & 1) * TC * CS;
 #pragma unroll
     for (int t = 0; t < DPT; ++t) {
@@ -496,7 +508,7 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
       const int r = S0 - h0 + e / TC, j = e - (e / TC) * TC;
       smem[j * CS + PIDX(r)] = make_float2(0.f, 0.f);
     }
-    if (c + 1 < C) fetch(c + 1);
+    if (c + 1 < ce) fetch(c + 1);
     __syncthreads();                       // strip filled (and every thread is past coil c-1)
     fft_lds_wave<G0, false>(smem + col * CS, stw, lane);
     __syncthreads();                       // all columns transformed
@@ -567,10 +579,11 @@ template <int G0, int WT>
 __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
     const float2* __restrict__ y, float2* __restrict__ T, const int2* __restrict__ em,
     const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt) {
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk) {
   constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
+  const int cb = blockIdx.z * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
   __shared__ unsigned long long acc[2 * TC * CS];
   __shared__ float2 stw[G0];
   __shared__ float wmax[NTH / 64];
@@ -601,9 +614,9 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
     for (int q = 0; q < STRIP_EPT; ++q)
       vn[q] = F[q].k >= 0 ? cscale(yc[F[q].k], F[q].wAH) : make_float2(0.f, 0.f);
   };
-  fetch(0);
+  fetch(cb);
   for (int i = tid; i < 2 * TC * CS; i += NTH) acc[i] = 0ull;
-  for (int c = 0; c < C; ++c) {
+  for (int c = cb; c < ce; ++c) {
     const float2* yc = ybase + (int64_t)c * Kb;
     float2 v[STRIP_EPT];
     float m = 0.f;
@@ -612,7 +625,7 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
       v[q] = vn[q];
       m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
     }
-    if (c + 1 < C) fetch(c + 1);
+    if (c + 1 < ce) fetch(c + 1);
     for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
       const float2 vo = cscale(yc[em[e].y], ef[e].w);
       m = fmaxf(m, fmaxf(fabsf(vo.x), fabsf(vo.y)));
@@ -766,13 +779,13 @@ static int launch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStream
 template <int G, int WT>
 static int launch_strips(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
                          int nb, int like, hipStream_t st) {
-  dim3 grid(p->nstrips, nb), block(STRIP_THREADS);
+  dim3 grid(p->nstrips, nb, (p->C + p->cchunk - 1) / p->cchunk), block(STRIP_THREADS);
   if (adjoint)
     hipLaunchKernelGGL((k_adj_strips<G, WT>), grid, block, 0, st, yin, p->d_T, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W);
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk);
   else
     hipLaunchKernelGGL((k_fwd_strips<G, WT>), grid, block, 0, st, p->d_T, yout, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W);
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

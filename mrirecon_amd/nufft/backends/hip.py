@@ -71,6 +71,7 @@ class NufftPlan:
         with torch.cuda.device(maps.device):
             _lib.check(lib.mrh_nufft_plan_create(ctypes.byref(d), ctypes.byref(h)))
         self.handle = h
+        self.tuning = None
 
     def info(self) -> Dict[str, float]:
         inf = _lib.NufftInfo()
@@ -96,6 +97,48 @@ class NufftPlan:
     def adjoint_stage(self, stage: int, y: torch.Tensor, x: torch.Tensor, like: int = 1) -> None:
         _lib.check(self._lib.mrh_nufft_adjoint_stage(self.handle, int(stage), y.data_ptr(), x.data_ptr(),
                                                      int(like), _lib.stream_of(y)))
+
+    @property
+    def coil_chunk(self) -> int:
+        return int(self.info()["coil_chunk"])
+
+    def set_coil_chunk(self, coils_per_block: int) -> None:
+        _lib.check(self._lib.mrh_nufft_set_coil_chunk(self.handle, int(coils_per_block)))
+
+    @torch.no_grad()
+    def autotune(self, reps: int = 3, like: int = 1, candidates: Optional[Sequence[int]] = None) -> Dict[int, float]:
+        """Pick the strip kernels' coils per workgroup by timing forward + adjoint pairs.
+
+        Candidates are ceil(C / k) for k = 1..C (one workgroup per (frame, strip) down to one
+        per coil).  Each is timed on scratch data with HIP events on the current stream
+        (one warm-up pair, then the median of ``reps`` pairs); the fastest is kept and
+        ``self.tuning`` records every time (ms per pair).  Results do not depend on the
+        choice."""
+        S0, S1 = self.S
+        dev = self.maps.device
+        x = torch.ones((self.B * like, S0, S1), dtype=torch.complex64, device=dev)
+        y = torch.empty(self.n_k * like, dtype=torch.complex64, device=dev)
+        cands = sorted({-(-self.C // k) for k in range(1, self.C + 1)} if candidates is None
+                       else {int(c) for c in candidates}, reverse=True)
+        times: Dict[int, float] = {}
+        for c in cands:
+            self.set_coil_chunk(c)
+            self.forward(x, y, like)
+            self.adjoint(y, x, like)
+            ts = []
+            for _ in range(max(1, reps)):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                self.forward(x, y, like)
+                self.adjoint(y, x, like)
+                e1.record()
+                e1.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            times[c] = float(sorted(ts)[len(ts) // 2])
+        best = min(times, key=times.get)
+        self.set_coil_chunk(best)
+        self.tuning = {"coil_chunk": best, "ms_per_pair": times}
+        return times
 
     def close(self) -> None:
         if getattr(self, "handle", None) is not None and self.handle.value:
@@ -125,7 +168,8 @@ class HipNUFFTAdapter:
     apply_dcf_in_adj: bool = True
     kb_width: int = 6          # torchkb numpoints=6 (torchkb.py:139)
     osf: float = 2.0           # torchkb default 2x grid
-    strip_cols: int = 0        # 0 = chosen by the plan (autotuner may override)
+    strip_cols: int = 0        # 0 = chosen by the plan
+    autotune: bool = True      # time the strip kernels' work split per plan (NufftPlan.autotune)
 
     _im_shape: Optional[Tuple[int, ...]] = None
     _maps: Optional[torch.Tensor] = None
@@ -223,6 +267,8 @@ class HipNUFFTAdapter:
                           dcf_flat=self._dcf_flat, dcf_mode=self.dcf_mode,
                           dcf_in_fwd=self.apply_dcf_in_fwd, dcf_in_adj=self.apply_dcf_in_adj,
                           maps=self._maps, like=like, strip_cols=self.strip_cols)
+            if self.autotune:
+                p.autotune(like=like)
             self._plans[like] = p
         return p
 

@@ -59,7 +59,7 @@ int main(void) {
          sizeof(mrh_nufft_desc), sizeof(mrh_nufft_info), sizeof(mrh_tv_term), sizeof(mrh_tv_desc),
          sizeof(mrh_cg_desc), sizeof(mrh_cg_hist));
   P(mrh_nufft_desc, beta) P(mrh_nufft_desc, k_offsets) P(mrh_nufft_desc, maps) P(mrh_nufft_desc, strip_cols)
-  P(mrh_nufft_info, beta) P(mrh_nufft_info, workspace_bytes)
+  P(mrh_nufft_info, beta) P(mrh_nufft_info, workspace_bytes) P(mrh_nufft_info, coil_chunk)
   P(mrh_tv_desc, term) P(mrh_cg_desc, k_prob) P(mrh_cg_desc, c1) P(mrh_cg_desc, max_hist) P(mrh_cg_desc, restart_c)
   return 0;
 }

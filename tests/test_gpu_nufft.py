@@ -238,3 +238,31 @@ def test_diag_AHA_scalar_known_answer():
     expect = float(np.conj(wl.maps[0, ctr[0], ctr[1]]).real) * float(wl.dcf.astype(np.float64).sum()) / sos
     print(f"[c1] diag_AHA_scalar = {alpha:.5f} (NUDFT {expect:.5f})")
     assert abs(alpha - expect) <= 1e-2 * abs(expect)
+
+
+def test_coil_chunk_autotune_keeps_results_identical():
+    """The strip kernels' work split (coils per workgroup) changes only the launch: forward and
+    adjoint are bit-identical for every split, and the autotuner records a time per candidate."""
+    wl = synth.config_c1()
+    be = adapter_for(wl)
+    plan = be.plan
+    assert plan.tuning is not None and set(plan.tuning["ms_per_pair"]) == {8, 4, 3, 2, 1}
+    assert plan.coil_chunk == plan.tuning["coil_chunk"]
+    x = c64(wl.x_true)
+    outs = []
+    for c in (8, 3, 1):
+        plan.set_coil_chunk(c)
+        y = torch.empty(plan.n_k, dtype=torch.complex64, device=DEV)
+        plan.forward(x, y)
+        xa = torch.empty_like(x)
+        plan.adjoint(y, xa)
+        torch.cuda.synchronize()
+        outs.append((y.cpu().numpy(), xa.cpu().numpy()))
+    for y, xa in outs[1:]:
+        assert np.array_equal(y, outs[0][0]) and np.array_equal(xa, outs[0][1])
+    ora = oracle_for(wl)
+    assert rel_l2(outs[-1][0], ora.forward(wl.x_true)) <= 1e-5
+    with pytest.raises(ValueError):
+        plan.set_coil_chunk(0)
+    with pytest.raises(ValueError):
+        plan.set_coil_chunk(9)
