@@ -229,8 +229,8 @@ def batched_uneven(args, dev):
     eng.prime(xa)
     eng.capture()
 
-    def solve(x0):
-        eng.prime(x0)
+    def solve(x0, Ax0=None):
+        eng.prime(x0, Ax0)
         return eng.run(args.tol_iters, use_graph=True, check_every=1)
 
     torch.cuda.synchronize()
@@ -256,7 +256,7 @@ def batched_uneven(args, dev):
     plan.adjoint(y, xa)
     xw = net.infer(xa, nvol=P)
     x0, acc, f_w, f_a = select_warm_start(obj, xa, xw, out=x0)
-    solve(x0)
+    solve(x0, obj.Ax if acc.all() else None)
     torch.cuda.synchronize()
     t_warm = time.perf_counter() - t0
     st_w, it_w = eng.status()

@@ -6,10 +6,10 @@
 // (dx+1)), 3-tap temporal (t = df+1, never crossing a volume's first or last
 // frame) and 1x1.  Out-of-range neighbours read as zero (zero padding).
 //
-// Convolutions are implicit GEMMs on v_mfma_f32_16x16x32_bf16 (D = A B, A
-// 16x32, B 32x16, FP32 accumulate).  Lane l holds A row (l % 16) and B
-// column (l % 16) with the eight K indices 8 (l/16) .. +7, and D rows
-// 4 (l/16) .. +3 of column (l % 16).
+// Convolutions are implicit GEMMs on BF16 MFMA with FP32 accumulation:
+// v_mfma_f32_16x16x32_bf16 (lane l holds A row l % 16 and B column l % 16 with
+// the eight K indices 8 (l/16) .. +7) and v_mfma_f32_16x16x16_bf16 (four K
+// indices 4 (l/16) .. +3); D rows 4 (l/16) .. +3 of column l % 16.
 //   forward          D[cout][pixel] = sum_(tap, cin) W[tap][cout][cin] x[pixel + off(tap)][cin]
 //                    epilogue: + bias, optional ReLU, round to BF16.
 //   backward-data    the same kernel applied to the output gradient with the
@@ -125,6 +125,116 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, con
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         float v = acc[a][h][j] + ((bias && cb + j < Cout) ? bias[cb + j] : 0.f);
+        if (relu) v = fmaxf(v, 0.f);
+        o[j] = f2bf(v);
+      }
+      if (cb + 3 < Cout) {
+        *reinterpret_cast<uint2*>(out + (size_t)p * Cout + cb) =
+            make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16), (unsigned)o[2] | ((unsigned)o[3] << 16));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (cb + j < Cout) out[(size_t)p * Cout + cb + j] = o[j];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ forward, LDS-tiled (Cin % 16 == 0)
+// Block: 4 waves along pixels; wave tile (16 MT) couts x (16 NT) pixels, block
+// tile BC = 16 MT couts x BP = 64 NT consecutive (flattened) pixels.  Per chunk
+// of 16 input channels the block stages into LDS
+//   * the input rows the taps touch: for each tap row offset r (dy for 3x3, df
+//     for temporal, none for 1x1) the pixels [p0 + r*RS - 1, p0 + r*RS + BP + 1)
+//     (RS = W or H*W), 16 channels each, out-of-range pixels as zero;
+//   * the weights [tap][BC couts][16 channels];
+// then runs one 16x16x16 MFMA K-step per tap.  Each staged input element serves
+// every tap and every cout tile of the block (the register-only kernel above
+// re-reads it per tap); boundary masks are applied per (pixel, tap) at read time.
+template <int MT, int NT>
+__global__ __launch_bounds__(256) void k_conv_fwd_lds(const u16* __restrict__ x, const u16* __restrict__ w,
+                                                      const float* __restrict__ bias, u16* __restrict__ out,
+                                                      TapGeom g, int npix, int Cin, int Cout, int Cout_pad,
+                                                      int relu) {
+  typedef short s4 __attribute__((ext_vector_type(4)));
+  constexpr int BP = 64 * NT, BC = 16 * MT, KC = 16, RW = BP + 2;
+  __shared__ __align__(16) u16 sIn[3 * RW * KC];
+  __shared__ __align__(16) u16 sW[9 * BC * KC];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  const int p0 = blockIdx.x * BP, co0 = blockIdx.y * BC;
+  const int HW = g.H * g.W;
+  const int nrows = g.kind == MRH_UNET_TAP_1X1 ? 1 : 3;
+  const int RS = g.kind == MRH_UNET_TAP_3X3 ? g.W : (g.kind == MRH_UNET_TAP_T3 ? HW : 0);
+  int pxx[NT], py[NT], pfr[NT];
+  bool pv[NT];
+#pragma unroll
+  for (int h = 0; h < NT; ++h) {
+    const int p = p0 + wave * 16 * NT + 16 * h + r;
+    pv[h] = p < npix;
+    const int pp = pv[h] ? p : 0;
+    pxx[h] = pp % g.W;
+    const int t1 = pp / g.W;
+    py[h] = t1 % g.H;
+    pfr[h] = (t1 / g.H) % g.F;
+  }
+  f32x4 acc[MT][NT];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int h = 0; h < NT; ++h) acc[m][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+  for (int ci0 = 0; ci0 < Cin; ci0 += KC) {
+    __syncthreads();   // the previous chunk's fragments have been read
+    for (int i = tid; i < nrows * RW * 2; i += 256) {
+      const int row = i / (RW * 2), rem = i - row * (RW * 2);
+      const int q = rem >> 1, half = rem & 1;
+      const int rr = nrows == 1 ? 0 : row - 1;
+      const long long src = (long long)p0 + (long long)rr * RS - 1 + q;
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (src >= 0 && src < npix) v = *reinterpret_cast<const uint4*>(x + src * Cin + ci0 + half * 8);
+      *reinterpret_cast<uint4*>(sIn + (row * RW + q) * KC + half * 8) = v;
+    }
+    for (int i = tid; i < g.ntap * BC * 2; i += 256) {
+      const int t = i / (BC * 2), rem = i - t * (BC * 2);
+      const int co = rem >> 1, half = rem & 1;
+      *reinterpret_cast<uint4*>(sW + (t * BC + co) * KC + half * 8) =
+          *reinterpret_cast<const uint4*>(w + ((size_t)t * Cout_pad + co0 + co) * Cin + ci0 + half * 8);
+    }
+    __syncthreads();
+    for (int t = 0; t < g.ntap; ++t) {
+      int df, dy, dx;
+      tap_offset(g, t, df, dy, dx);
+      const int row = nrows == 1 ? 0 : (g.kind == MRH_UNET_TAP_3X3 ? dy : df) + 1;
+      s4 a[MT], b[NT];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) a[m] = *reinterpret_cast<const s4*>(sW + (t * BC + 16 * m + r) * KC + 4 * kg);
+#pragma unroll
+      for (int h = 0; h < NT; ++h) {
+        const bool ok = pv[h] && (unsigned)(pfr[h] + df) < (unsigned)g.F &&
+                        (unsigned)(py[h] + dy) < (unsigned)g.H && (unsigned)(pxx[h] + dx) < (unsigned)g.W;
+        const int lp = wave * 16 * NT + 16 * h + r + 1 + dx;
+        b[h] = ok ? *reinterpret_cast<const s4*>(sIn + (row * RW + lp) * KC + 4 * kg) : s4{0, 0, 0, 0};
+      }
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int h = 0; h < NT; ++h)
+          acc[m][h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[h], acc[m][h], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < NT; ++h) {
+    const int p = p0 + wave * 16 * NT + 16 * h + r;
+    if (p >= npix) continue;
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int cb = co0 + 16 * m + 4 * kg;
+      if (cb >= Cout) continue;
+      u16 o[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float v = acc[m][h][j] + ((bias && cb + j < Cout) ? bias[cb + j] : 0.f);
         if (relu) v = fmaxf(v, 0.f);
         o[j] = f2bf(v);
       }
@@ -426,9 +536,20 @@ extern "C" int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias
   const int64_t npix = (int64_t)nvol * F * H * W;
   MRH_REQUIRE(nvol >= 1 && npix < (1ll << 31) - 256, MRH_EINVAL, "unet_conv_fwd: %lld pixels out of range",
               (long long)npix);
-  dim3 grid((unsigned)((npix + 127) / 128), Cout_pad / 32);
-  hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, (hipStream_t)stream, (const u16*)x, (const u16*)w, bias,
-                     (u16*)out, g, (int)npix, Cin, Cout, Cout_pad, relu);
+  hipStream_t st = (hipStream_t)stream;
+  if (Cin % 16 == 0 && Cout_pad % 64 == 0) {          // 64 couts x 256 pixels per workgroup
+    dim3 grid((unsigned)((npix + 255) / 256), Cout_pad / 64);
+    hipLaunchKernelGGL((k_conv_fwd_lds<4, 4>), grid, dim3(256), 0, st, (const u16*)x, (const u16*)w, bias,
+                       (u16*)out, g, (int)npix, Cin, Cout, Cout_pad, relu);
+  } else if (Cin % 16 == 0) {                         // 32 couts x 512 pixels per workgroup
+    dim3 grid((unsigned)((npix + 511) / 512), Cout_pad / 32);
+    hipLaunchKernelGGL((k_conv_fwd_lds<2, 8>), grid, dim3(256), 0, st, (const u16*)x, (const u16*)w, bias,
+                       (u16*)out, g, (int)npix, Cin, Cout, Cout_pad, relu);
+  } else {                                            // Cin = 8: register-only kernel
+    dim3 grid((unsigned)((npix + 127) / 128), Cout_pad / 32);
+    hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, st, (const u16*)x, (const u16*)w, bias, (u16*)out, g,
+                       (int)npix, Cin, Cout, Cout_pad, relu);
+  }
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

@@ -89,13 +89,17 @@ class NLCGEngine:
     def _st(self) -> int:
         return _lib.stream_of(self.x)
 
-    def prime(self, x0: Optional[torch.Tensor] = None) -> None:
-        """Start a run at x0 (or the current x): g = grad f(x0), d = -g, Ad = A d."""
+    def prime(self, x0: Optional[torch.Tensor] = None, Ax0: Optional[torch.Tensor] = None) -> None:
+        """Start a run at x0 (or the current x): g = grad f(x0), d = -g, Ad = A d.
+        ``Ax0`` (= A x0, already computed by the caller) saves the forward NUFFT."""
         lib, e = self._lib, self.handle
         if x0 is not None:
             self.x.copy_(x0.reshape(self.x.shape))
         _lib.check(lib.mrh_cg_reset(e, self._st()))
-        self.plan.forward(self.x, self.Ax)
+        if Ax0 is not None:
+            self.Ax.copy_(Ax0.reshape(self.Ax.shape))
+        else:
+            self.plan.forward(self.x, self.Ax)
         _lib.check(lib.mrh_cg_residual(e, self.Ax.data_ptr(), self.y.data_ptr(), self.r.data_ptr(), self._st()))
         self.plan.adjoint(self.r, self.g)
         _lib.check(lib.mrh_cg_gradient(e, self.x.data_ptr(), self.g.data_ptr(), 0, _lib.ptr(self.g_prev), 1,

@@ -95,11 +95,11 @@ class GraspProblem:
         self.plan.adjoint(self.y, out)
         return out
 
-    def solve(self, x0, long: bool = False, iters: Optional[int] = None) -> int:
+    def solve(self, x0, long: bool = False, iters: Optional[int] = None, Ax0=None) -> int:
         """NLCG from x0: to the relative-change stop (``long`` False) or for ``iters``
         iterations / until the engine's objective target (``long`` True)."""
         eng = self.eng_long if long else self.eng
-        eng.prime(x0)
+        eng.prime(x0, Ax0)
         return eng.run(iters or (self.long_iters if long else self.max_iter), use_graph=True, check_every=1)
 
     def history(self, long: bool = False):
@@ -143,7 +143,8 @@ def run_warm(pb: GraspProblem, net, target: Optional[float] = None) -> Dict[str,
     t_net = time.perf_counter()
     x0, acc, f_w, f_a = select_warm_start(pb.objective, pb.x_adj, x_warm, out=pb.x0)
     t_sel = time.perf_counter()
-    pb.solve(x0, long=target is not None, iters=pb.max_iter if target is not None else None)
+    pb.solve(x0, long=target is not None, iters=pb.max_iter if target is not None else None,
+             Ax0=pb.objective.Ax if acc.all() else None)
     _sync(torch)
     t = time.perf_counter() - t0
     h = pb.history(long=target is not None)

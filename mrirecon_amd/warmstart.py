@@ -493,9 +493,10 @@ def select_warm_start(objective: ObjectiveEval, x_adj: torch.Tensor, x_warm: tor
                       out: Optional[torch.Tensor] = None) -> Tuple[torch.Tensor, np.ndarray, np.ndarray, np.ndarray]:
     """Per problem, keep x_warm unless f(x_warm) > f(x_adj) (then the adjoint start).
 
-    Returns (x0, accepted[P] bool, f_warm[P], f_adj[P])."""
-    f_w = objective(x_warm)
+    Returns (x0, accepted[P] bool, f_warm[P], f_adj[P]).  x_warm is evaluated last, so
+    ``objective.Ax`` holds A x0 when every problem accepted (``accepted.all()``)."""
     f_a = objective(x_adj)
+    f_w = objective(x_warm)
     acc = f_w <= f_a
     x0 = x_warm if out is None else out.copy_(x_warm)
     if not acc.all():

@@ -264,3 +264,22 @@ def test_objective_target_stop():
     eng.prime(x0)
     eng.run(12)
     assert eng.stop_reasons() == ["active"]
+
+
+def test_prime_with_precomputed_forward():
+    """prime(x0, Ax0=A x0) runs exactly the iterations prime(x0) runs."""
+    wl = small_dynamic(seed=27, F=4)
+    be, ora = prepare(wl)
+    regm, _ = regs_for(wl)
+    x0 = c64(ora.adjoint(wl.y))
+    eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, direction="prplus", restart_c=0.0)
+    eng.prime(x0)
+    eng.run(6)
+    h1 = eng.history()[0]
+    Ax0 = torch.empty_like(eng.Ax)
+    be.plan.forward(x0, Ax0)
+    eng.prime(x0, Ax0)
+    eng.run(6)
+    h2 = eng.history()[0]
+    assert [r["step"] for r in h1] == [r["step"] for r in h2]
+    assert all(abs(a["f_step"] - b["f_step"]) <= 1e-9 * abs(a["f_step"]) for a, b in zip(h1, h2))

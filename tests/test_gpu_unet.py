@@ -50,12 +50,17 @@ def check_bf16_close(got, ref):
 GEOM = (2, 5, 16, 24)   # volumes, frames, H, W: 3840 pixels (not a multiple of the 128-pixel tile)
 
 
+@pytest.mark.parametrize("geom", [GEOM, (1, 3, 10, 18)])
 @pytest.mark.parametrize("kind,cin,cout,relu", [
     (ou.TAP_3X3, 8, 32, True), (ou.TAP_3X3, 64, 32, False), (ou.TAP_T3, 32, 64, True),
-    (ou.TAP_T3, 64, 128, False), (ou.TAP_1X1, 32, 8, False), (ou.TAP_1X1, 256, 128, True)])
-def test_conv_forward(kind, cin, cout, relu):
+    (ou.TAP_T3, 64, 128, False), (ou.TAP_1X1, 32, 8, False), (ou.TAP_1X1, 256, 128, True),
+    (ou.TAP_3X3, 16, 64, True), (ou.TAP_3X3, 48, 96, False)])
+def test_conv_forward(kind, cin, cout, relu, geom):
+    """Every kernel variant: Cin = 8 (register tiles), Cin % 16 == 0 with Cout_pad % 64 == 0
+    (64 x 256 LDS tiles) or not (32 x 512 LDS tiles); tiles that end inside a row and in the
+    middle of the last tile (540 pixels)."""
     lib = _lib.load()
-    nvol, F, H, W = GEOM
+    nvol, F, H, W = geom
     npix = nvol * F * H * W
     rng = np.random.default_rng(cin + cout + kind)
     T = ou.NTAP[kind]
