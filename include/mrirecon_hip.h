@@ -270,10 +270,10 @@ int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias, void* out
  * (db may be NULL).  FP32 atomics: the result is reproducible to rounding only. */
 int mrh_unet_conv_bwd_weight(const void* gout, const void* x, float* dw, float* db, int kind, int nvol,
                              int F, int H, int W, int Cin, int Cout, int Cout_pad, void* stream);
-/* g[i] = 0 where y[i] <= 0 (y = the ReLU output), n elements. */
+/* g[i] = 0 where y[i] <= 0 (y = the ReLU output), n elements (n % 8 == 0). */
 int mrh_unet_relu_bwd(void* g, const void* y, int64_t n, void* stream);
 /* 2 x 2 average pooling (nf, H, W, C) -> (nf, H/2, W/2, C) and its adjoint
- * gx = pool^T gy (+ add, which may be NULL). */
+ * gx = pool^T gy (+ add, which may be NULL).  C % 8 == 0 (as for C1, C2 below). */
 int mrh_unet_pool2(const void* x, void* y, int nf, int H, int W, int C, void* stream);
 int mrh_unet_pool2_bwd(const void* gy, const void* add, void* gx, int nf, int H, int W, int C, void* stream);
 /* dst (nf, H, W, C1 + C2) = concat(nearest 2x upsampling of src (nf, H/2, W/2, C1),

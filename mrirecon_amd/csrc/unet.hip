@@ -151,15 +151,16 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const u16* __restrict__ x, con
 // then runs one 16x16x16 MFMA K-step per tap.  Each staged input element serves
 // every tap and every cout tile of the block (the register-only kernel above
 // re-reads it per tap); boundary masks are applied per (pixel, tap) at read time.
-template <int MT, int NT>
+template <int MT, int NT, int KC>
 __global__ __launch_bounds__(256) void k_conv_fwd_lds(const u16* __restrict__ x, const u16* __restrict__ w,
                                                       const float* __restrict__ bias, u16* __restrict__ out,
                                                       TapGeom g, int npix, int Cin, int Cout, int Cout_pad,
                                                       int relu) {
   typedef short s4 __attribute__((ext_vector_type(4)));
-  constexpr int BP = 64 * NT, BC = 16 * MT, KC = 16, RW = BP + 2;
+  constexpr int BP = 64 * NT, BC = 16 * MT, RW = BP + 2, NCH = KC / 8;   // NCH 16-byte groups per pixel
   __shared__ __align__(16) u16 sIn[3 * RW * KC];
   __shared__ __align__(16) u16 sW[9 * BC * KC];
+  static_assert(KC == 16 || KC % 32 == 0, "channel chunk");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r = lane & 15, kg = lane >> 4;
   const int p0 = blockIdx.x * BP, co0 = blockIdx.y * BC;
@@ -186,18 +187,18 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const u16* __restrict__ x,
 
   for (int ci0 = 0; ci0 < Cin; ci0 += KC) {
     __syncthreads();   // the previous chunk's fragments have been read
-    for (int i = tid; i < nrows * RW * 2; i += 256) {
-      const int row = i / (RW * 2), rem = i - row * (RW * 2);
-      const int q = rem >> 1, half = rem & 1;
+    for (int i = tid; i < nrows * RW * NCH; i += 256) {
+      const int row = i / (RW * NCH), rem = i - row * (RW * NCH);
+      const int q = rem / NCH, half = rem - q * NCH;
       const int rr = nrows == 1 ? 0 : row - 1;
       const long long src = (long long)p0 + (long long)rr * RS - 1 + q;
       uint4 v = make_uint4(0, 0, 0, 0);
       if (src >= 0 && src < npix) v = *reinterpret_cast<const uint4*>(x + src * Cin + ci0 + half * 8);
       *reinterpret_cast<uint4*>(sIn + (row * RW + q) * KC + half * 8) = v;
     }
-    for (int i = tid; i < g.ntap * BC * 2; i += 256) {
-      const int t = i / (BC * 2), rem = i - t * (BC * 2);
-      const int co = rem >> 1, half = rem & 1;
+    for (int i = tid; i < g.ntap * BC * NCH; i += 256) {
+      const int t = i / (BC * NCH), rem = i - t * (BC * NCH);
+      const int co = rem / NCH, half = rem - co * NCH;
       *reinterpret_cast<uint4*>(sW + (t * BC + co) * KC + half * 8) =
           *reinterpret_cast<const uint4*>(w + ((size_t)t * Cout_pad + co0 + co) * Cin + ci0 + half * 8);
     }
@@ -206,21 +207,40 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const u16* __restrict__ x,
       int df, dy, dx;
       tap_offset(g, t, df, dy, dx);
       const int row = nrows == 1 ? 0 : (g.kind == MRH_UNET_TAP_3X3 ? dy : df) + 1;
-      s4 a[MT], b[NT];
-#pragma unroll
-      for (int m = 0; m < MT; ++m) a[m] = *reinterpret_cast<const s4*>(sW + (t * BC + 16 * m + r) * KC + 4 * kg);
+      bool ok[NT];
+      int lp[NT];
 #pragma unroll
       for (int h = 0; h < NT; ++h) {
-        const bool ok = pv[h] && (unsigned)(pfr[h] + df) < (unsigned)g.F &&
-                        (unsigned)(py[h] + dy) < (unsigned)g.H && (unsigned)(pxx[h] + dx) < (unsigned)g.W;
-        const int lp = wave * 16 * NT + 16 * h + r + 1 + dx;
-        b[h] = ok ? *reinterpret_cast<const s4*>(sIn + (row * RW + lp) * KC + 4 * kg) : s4{0, 0, 0, 0};
+        ok[h] = pv[h] && (unsigned)(pfr[h] + df) < (unsigned)g.F && (unsigned)(py[h] + dy) < (unsigned)g.H &&
+                (unsigned)(pxx[h] + dx) < (unsigned)g.W;
+        lp[h] = (row * RW + wave * 16 * NT + 16 * h + r + 1 + dx) * KC;
       }
+      if constexpr (KC == 16) {            // one 16x16x16 step: lane holds channels 4 kg .. +3
+        s4 a[MT], b[NT];
 #pragma unroll
-      for (int m = 0; m < MT; ++m)
+        for (int m = 0; m < MT; ++m) a[m] = *reinterpret_cast<const s4*>(sW + (t * BC + 16 * m + r) * KC + 4 * kg);
 #pragma unroll
-        for (int h = 0; h < NT; ++h)
-          acc[m][h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[h], acc[m][h], 0, 0, 0);
+        for (int h = 0; h < NT; ++h) b[h] = ok[h] ? *reinterpret_cast<const s4*>(sIn + lp[h] + 4 * kg) : s4{0, 0, 0, 0};
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int h = 0; h < NT; ++h)
+            acc[m][h] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[m], b[h], acc[m][h], 0, 0, 0);
+      } else {                             // 16x16x32 steps: lane holds channels 32 s + 8 kg .. +7
+#pragma unroll
+        for (int s = 0; s < KC / 32; ++s) {
+          bf16x8 a[MT], b[NT];
+#pragma unroll
+          for (int m = 0; m < MT; ++m) a[m] = load8(sW + (t * BC + 16 * m + r) * KC + 32 * s + 8 * kg);
+#pragma unroll
+          for (int h = 0; h < NT; ++h) b[h] = ok[h] ? load8(sIn + lp[h] + 32 * s + 8 * kg) : zero8();
+#pragma unroll
+          for (int m = 0; m < MT; ++m)
+#pragma unroll
+            for (int h = 0; h < NT; ++h)
+              acc[m][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[h], acc[m][h], 0, 0, 0);
+        }
+      }
     }
   }
 #pragma unroll
@@ -350,76 +370,135 @@ __global__ __launch_bounds__(256) void k_bias_grad(const u16* __restrict__ gout,
 #define GRID_STRIDE(i, n) \
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < (n); i += (int64_t)gridDim.x * blockDim.x)
 
-__global__ void k_relu_bwd(u16* __restrict__ g, const u16* __restrict__ y, int64_t n) {
-  GRID_STRIDE(i, n) if (!(bf2f(y[i]) > 0.f)) g[i] = 0;
+// Elementwise kernels move 8 channels (16 bytes) per thread; every channel count is a multiple of 8.
+__device__ __forceinline__ void unpack8(uint4 v, float* f) {
+  const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    f[2 * k] = __uint_as_float(w[k] << 16);
+    f[2 * k + 1] = __uint_as_float(w[k] & 0xffff0000u);
+  }
+}
+__device__ __forceinline__ uint4 pack8(const float* f) {
+  unsigned w[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = (unsigned)f2bf(f[2 * k]) | ((unsigned)f2bf(f[2 * k + 1]) << 16);
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+__device__ __forceinline__ uint4 ld8(const u16* p) { return *reinterpret_cast<const uint4*>(p); }
+__device__ __forceinline__ void st8(u16* p, uint4 v) { *reinterpret_cast<uint4*>(p) = v; }
+
+// g = 0 where the ReLU output y <= 0 (n elements, n % 8 == 0)
+__global__ void k_relu_bwd(u16* __restrict__ g, const u16* __restrict__ y, int64_t n8) {
+  GRID_STRIDE(i, n8) {
+    float fy[8], fg[8];
+    unpack8(ld8(y + 8 * i), fy);
+    unpack8(ld8(g + 8 * i), fg);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) if (!(fy[k] > 0.f)) fg[k] = 0.f;
+    st8(g + 8 * i, pack8(fg));
+  }
 }
 
 __global__ void k_pool_fwd(const u16* __restrict__ x, u16* __restrict__ y, int nf, int H, int W, int C) {
-  const int Ho = H / 2, Wo = W / 2;
-  GRID_STRIDE(i, (int64_t)nf * Ho * Wo * C) {
-    const int c = i % C;
-    int64_t q = i / C;
+  const int Ho = H / 2, Wo = W / 2, CG = C / 8;
+  GRID_STRIDE(i, (int64_t)nf * Ho * Wo * CG) {
+    const int c = (int)(i % CG) * 8;
+    int64_t q = i / CG;
     const int xo = q % Wo; q /= Wo;
     const int yo = q % Ho;
     const int64_t f = q / Ho;
     const size_t b = (((size_t)f * H + 2 * yo) * W + 2 * xo) * C + c;
     const size_t row = (size_t)W * C;
-    y[i] = f2bf(0.25f * (bf2f(x[b]) + bf2f(x[b + C]) + bf2f(x[b + row]) + bf2f(x[b + row + C])));
+    float a[8], t[8];
+    unpack8(ld8(x + b), a);
+    unpack8(ld8(x + b + C), t);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] += t[k];
+    unpack8(ld8(x + b + row), t);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] += t[k];
+    unpack8(ld8(x + b + row + C), t);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a[k] = 0.25f * (a[k] + t[k]);
+    st8(y + 8 * i, pack8(a));
   }
 }
 
 // gx = pool^T gy (+ add)
 __global__ void k_pool_bwd(const u16* __restrict__ gy, const u16* __restrict__ add, u16* __restrict__ gx, int nf,
                            int H, int W, int C) {
-  GRID_STRIDE(i, (int64_t)nf * H * W * C) {
-    const int c = i % C;
-    int64_t q = i / C;
+  const int CG = C / 8;
+  GRID_STRIDE(i, (int64_t)nf * H * W * CG) {
+    const int c = (int)(i % CG) * 8;
+    int64_t q = i / CG;
     const int xx = q % W; q /= W;
     const int yy = q % H;
     const int64_t f = q / H;
-    float v = 0.25f * bf2f(gy[(((size_t)f * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c]);
-    if (add) v += bf2f(add[i]);
-    gx[i] = f2bf(v);
+    float v[8];
+    unpack8(ld8(gy + (((size_t)f * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C + c), v);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] *= 0.25f;
+    if (add) {
+      float a[8];
+      unpack8(ld8(add + 8 * i), a);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] += a[k];
+    }
+    st8(gx + 8 * i, pack8(v));
   }
 }
 
 // dst = concat(nearest_up2(src) [C1 channels], skip [C2 channels]) at H x W
 __global__ void k_up_concat(const u16* __restrict__ src, const u16* __restrict__ skip, u16* __restrict__ dst,
                             int nf, int H, int W, int C1, int C2) {
-  const int C = C1 + C2;
-  GRID_STRIDE(i, (int64_t)nf * H * W * C) {
-    const int c = i % C;
-    const int64_t pix = i / C;
-    if (c < C1) {
+  const int CG1 = C1 / 8, CG = (C1 + C2) / 8;
+  GRID_STRIDE(i, (int64_t)nf * H * W * CG) {
+    const int cg = (int)(i % CG);
+    const int64_t pix = i / CG;
+    uint4 v;
+    if (cg < CG1) {
       const int xx = pix % W;
       const int64_t q = pix / W;
       const int yy = q % H;
       const int64_t f = q / H;
-      dst[i] = src[(((size_t)f * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C1 + c];
+      v = ld8(src + (((size_t)f * (H / 2) + yy / 2) * (W / 2) + xx / 2) * C1 + 8 * cg);
     } else {
-      dst[i] = skip[(size_t)pix * C2 + (c - C1)];
+      v = ld8(skip + (size_t)pix * C2 + 8 * (cg - CG1));
     }
+    st8(dst + 8 * i, v);
   }
 }
 
 // gsrc = up2^T gdst[:, :C1] (H/2 x W/2), gskip = gdst[:, C1:]
 __global__ void k_up_concat_bwd(const u16* __restrict__ gdst, u16* __restrict__ gsrc, u16* __restrict__ gskip,
                                 int nf, int H, int W, int C1, int C2) {
-  const int C = C1 + C2, Ho = H / 2, Wo = W / 2;
-  const int64_t nsrc = (int64_t)nf * Ho * Wo * C1, nsk = (int64_t)nf * H * W * C2;
+  const int C = C1 + C2, Ho = H / 2, Wo = W / 2, CG1 = C1 / 8, CG2 = C2 / 8;
+  const int64_t nsrc = (int64_t)nf * Ho * Wo * CG1, nsk = (int64_t)nf * H * W * CG2;
   GRID_STRIDE(i, nsrc + nsk) {
     if (i < nsrc) {
-      const int c = i % C1;
-      int64_t q = i / C1;
+      const int c = (int)(i % CG1) * 8;
+      int64_t q = i / CG1;
       const int xo = q % Wo; q /= Wo;
       const int yo = q % Ho;
       const int64_t f = q / Ho;
       const size_t b = (((size_t)f * H + 2 * yo) * W + 2 * xo) * C + c;
       const size_t row = (size_t)W * C;
-      gsrc[i] = f2bf(bf2f(gdst[b]) + bf2f(gdst[b + C]) + bf2f(gdst[b + row]) + bf2f(gdst[b + row + C]));
+      float a[8], t[8];
+      unpack8(ld8(gdst + b), a);
+      unpack8(ld8(gdst + b + C), t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] += t[k];
+      unpack8(ld8(gdst + b + row), t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] += t[k];
+      unpack8(ld8(gdst + b + row + C), t);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) a[k] += t[k];
+      st8(gsrc + 8 * i, pack8(a));
     } else {
       const int64_t j = i - nsrc;
-      gskip[j] = gdst[(j / C2) * C + C1 + j % C2];
+      st8(gskip + 8 * j, ld8(gdst + (j / CG2) * C + C1 + 8 * (j % CG2)));
     }
   }
 }
@@ -537,19 +616,28 @@ extern "C" int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias
   MRH_REQUIRE(nvol >= 1 && npix < (1ll << 31) - 256, MRH_EINVAL, "unet_conv_fwd: %lld pixels out of range",
               (long long)npix);
   hipStream_t st = (hipStream_t)stream;
-  if (Cin % 16 == 0 && Cout_pad % 64 == 0) {          // 64 couts x 256 pixels per workgroup
-    dim3 grid((unsigned)((npix + 255) / 256), Cout_pad / 64);
-    hipLaunchKernelGGL((k_conv_fwd_lds<4, 4>), grid, dim3(256), 0, st, (const u16*)x, (const u16*)w, bias,
-                       (u16*)out, g, (int)npix, Cin, Cout, Cout_pad, relu);
-  } else if (Cin % 16 == 0) {                         // 32 couts x 512 pixels per workgroup
-    dim3 grid((unsigned)((npix + 511) / 512), Cout_pad / 32);
-    hipLaunchKernelGGL((k_conv_fwd_lds<2, 8>), grid, dim3(256), 0, st, (const u16*)x, (const u16*)w, bias,
-                       (u16*)out, g, (int)npix, Cin, Cout, Cout_pad, relu);
-  } else {                                            // Cin = 8: register-only kernel
-    dim3 grid((unsigned)((npix + 127) / 128), Cout_pad / 32);
-    hipLaunchKernelGGL(k_conv_fwd, grid, dim3(256), 0, st, (const u16*)x, (const u16*)w, bias, (u16*)out, g,
-                       (int)npix, Cin, Cout, Cout_pad, relu);
+  // Tile choice measured per layer shape on MI355X (34 x 256^2 volume, every layer of the network):
+  // 1x1 convolutions, Cin = 8 and 32-cout temporal layers are fastest on the register tiles; the
+  // 3x3 / temporal layers with >= 64 couts on 64 x 128 LDS tiles (64 x 256 for 3x3 with Cin >= 128);
+  // 3x3 with 32 couts on 32 x 128 tiles with 32-channel chunks.
+  const u16 *xs = (const u16*)x, *ws = (const u16*)w;
+  u16* os = (u16*)out;
+#define MRH_LDS_CONV(MT, NT, KC)                                                                              \
+  hipLaunchKernelGGL((k_conv_fwd_lds<MT, NT, KC>),                                                        \
+                     dim3((unsigned)((npix + 64 * NT - 1) / (64 * NT)), Cout_pad / (16 * MT)), dim3(256), 0, st, \
+                     xs, ws, bias, os, g, (int)npix, Cin, Cout, Cout_pad, relu)
+  if (Cin % 16 != 0 || kind == MRH_UNET_TAP_1X1 || (kind == MRH_UNET_TAP_T3 && Cout_pad == 32)) {
+    hipLaunchKernelGGL(k_conv_fwd, dim3((unsigned)((npix + 127) / 128), Cout_pad / 32), dim3(256), 0, st, xs, ws,
+                       bias, os, g, (int)npix, Cin, Cout, Cout_pad, relu);
+  } else if (Cout_pad % 64 != 0) {
+    if (Cin % 32 == 0) MRH_LDS_CONV(2, 2, 32);
+    else MRH_LDS_CONV(2, 4, 16);
+  } else if (kind == MRH_UNET_TAP_3X3 && Cin >= 128) {
+    MRH_LDS_CONV(4, 4, 16);
+  } else {
+    MRH_LDS_CONV(4, 2, 16);
   }
+#undef MRH_LDS_CONV
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -586,17 +674,18 @@ extern "C" int mrh_unet_conv_bwd_weight(const void* gout, const void* x, float* 
 }
 
 extern "C" int mrh_unet_relu_bwd(void* g, const void* y, int64_t n, void* stream) {
-  MRH_REQUIRE(g && y && n >= 0, MRH_EINVAL, "unet_relu_bwd: bad arguments");
+  MRH_REQUIRE(g && y && n >= 0 && n % 8 == 0, MRH_EINVAL, "unet_relu_bwd: n must be a multiple of 8");
   if (n == 0) return MRH_OK;
-  hipLaunchKernelGGL(k_relu_bwd, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (u16*)g, (const u16*)y, n);
+  hipLaunchKernelGGL(k_relu_bwd, dim3(grid_for(n / 8)), dim3(256), 0, (hipStream_t)stream, (u16*)g, (const u16*)y,
+                     n / 8);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
 
 extern "C" int mrh_unet_pool2(const void* x, void* y, int nf, int H, int W, int C, void* stream) {
-  MRH_REQUIRE(x && y && nf >= 1 && C >= 1 && H >= 2 && W >= 2 && H % 2 == 0 && W % 2 == 0, MRH_EINVAL,
-              "unet_pool2: need even H=%d, W=%d", H, W);
-  hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for((int64_t)nf * H * W * C / 4)), dim3(256), 0, (hipStream_t)stream,
+  MRH_REQUIRE(x && y && nf >= 1 && C >= 8 && C % 8 == 0 && H >= 2 && W >= 2 && H % 2 == 0 && W % 2 == 0,
+              MRH_EINVAL, "unet_pool2: need even H=%d, W=%d and C=%d a multiple of 8", H, W, C);
+  hipLaunchKernelGGL(k_pool_fwd, dim3(grid_for((int64_t)nf * H * W * C / 32)), dim3(256), 0, (hipStream_t)stream,
                      (const u16*)x, (u16*)y, nf, H, W, C);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
@@ -604,9 +693,9 @@ extern "C" int mrh_unet_pool2(const void* x, void* y, int nf, int H, int W, int 
 
 extern "C" int mrh_unet_pool2_bwd(const void* gy, const void* add, void* gx, int nf, int H, int W, int C,
                                   void* stream) {
-  MRH_REQUIRE(gy && gx && nf >= 1 && C >= 1 && H % 2 == 0 && W % 2 == 0 && H >= 2 && W >= 2, MRH_EINVAL,
-              "unet_pool2_bwd: need even H=%d, W=%d", H, W);
-  hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for((int64_t)nf * H * W * C)), dim3(256), 0, (hipStream_t)stream,
+  MRH_REQUIRE(gy && gx && nf >= 1 && C >= 8 && C % 8 == 0 && H % 2 == 0 && W % 2 == 0 && H >= 2 && W >= 2,
+              MRH_EINVAL, "unet_pool2_bwd: need even H=%d, W=%d and C=%d a multiple of 8", H, W, C);
+  hipLaunchKernelGGL(k_pool_bwd, dim3(grid_for((int64_t)nf * H * W * C / 8)), dim3(256), 0, (hipStream_t)stream,
                      (const u16*)gy, (const u16*)add, (u16*)gx, nf, H, W, C);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
@@ -614,9 +703,10 @@ extern "C" int mrh_unet_pool2_bwd(const void* gy, const void* add, void* gx, int
 
 extern "C" int mrh_unet_up2_concat(const void* src, const void* skip, void* dst, int nf, int H, int W, int C1,
                                    int C2, void* stream) {
-  MRH_REQUIRE(src && skip && dst && nf >= 1 && C1 >= 1 && C2 >= 1 && H % 2 == 0 && W % 2 == 0 && H >= 2 && W >= 2,
+  MRH_REQUIRE(src && skip && dst && nf >= 1 && C1 >= 8 && C2 >= 8 && C1 % 8 == 0 && C2 % 8 == 0 && H % 2 == 0 &&
+                  W % 2 == 0 && H >= 2 && W >= 2,
               MRH_EINVAL, "unet_up2_concat: bad arguments (H=%d W=%d C1=%d C2=%d)", H, W, C1, C2);
-  hipLaunchKernelGGL(k_up_concat, dim3(grid_for((int64_t)nf * H * W * (C1 + C2))), dim3(256), 0,
+  hipLaunchKernelGGL(k_up_concat, dim3(grid_for((int64_t)nf * H * W * (C1 + C2) / 8)), dim3(256), 0,
                      (hipStream_t)stream, (const u16*)src, (const u16*)skip, (u16*)dst, nf, H, W, C1, C2);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
@@ -624,10 +714,10 @@ extern "C" int mrh_unet_up2_concat(const void* src, const void* skip, void* dst,
 
 extern "C" int mrh_unet_up2_concat_bwd(const void* gdst, void* gsrc, void* gskip, int nf, int H, int W, int C1,
                                        int C2, void* stream) {
-  MRH_REQUIRE(gdst && gsrc && gskip && nf >= 1 && C1 >= 1 && C2 >= 1 && H % 2 == 0 && W % 2 == 0 && H >= 2 &&
-                  W >= 2,
+  MRH_REQUIRE(gdst && gsrc && gskip && nf >= 1 && C1 >= 8 && C2 >= 8 && C1 % 8 == 0 && C2 % 8 == 0 && H % 2 == 0 &&
+                  W % 2 == 0 && H >= 2 && W >= 2,
               MRH_EINVAL, "unet_up2_concat_bwd: bad arguments (H=%d W=%d C1=%d C2=%d)", H, W, C1, C2);
-  const int64_t n = (int64_t)nf * (H / 2) * (W / 2) * C1 + (int64_t)nf * H * W * C2;
+  const int64_t n = ((int64_t)nf * (H / 2) * (W / 2) * C1 + (int64_t)nf * H * W * C2) / 8;
   hipLaunchKernelGGL(k_up_concat_bwd, dim3(grid_for(n)), dim3(256), 0, (hipStream_t)stream, (const u16*)gdst,
                      (u16*)gsrc, (u16*)gskip, nf, H, W, C1, C2);
   MRH_CHECK_LAUNCH();

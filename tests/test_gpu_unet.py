@@ -54,11 +54,12 @@ GEOM = (2, 5, 16, 24)   # volumes, frames, H, W: 3840 pixels (not a multiple of 
 @pytest.mark.parametrize("kind,cin,cout,relu", [
     (ou.TAP_3X3, 8, 32, True), (ou.TAP_3X3, 64, 32, False), (ou.TAP_T3, 32, 64, True),
     (ou.TAP_T3, 64, 128, False), (ou.TAP_1X1, 32, 8, False), (ou.TAP_1X1, 256, 128, True),
-    (ou.TAP_3X3, 16, 64, True), (ou.TAP_3X3, 48, 96, False)])
+    (ou.TAP_3X3, 16, 64, True), (ou.TAP_3X3, 48, 96, False), (ou.TAP_3X3, 128, 64, True),
+    (ou.TAP_T3, 32, 32, False)])
 def test_conv_forward(kind, cin, cout, relu, geom):
-    """Every kernel variant: Cin = 8 (register tiles), Cin % 16 == 0 with Cout_pad % 64 == 0
-    (64 x 256 LDS tiles) or not (32 x 512 LDS tiles); tiles that end inside a row and in the
-    middle of the last tile (540 pixels)."""
+    """Every kernel variant: register tiles (Cin = 8, 1x1, 32-cout temporal), LDS tiles 64 x 128
+    (16-channel chunks), 64 x 256 (3x3, Cin >= 128), 32 x 128 (32-channel chunks) and 32 x 256;
+    tiles that end inside a row and in the middle of the last tile (540 pixels)."""
     lib = _lib.load()
     nvol, F, H, W = geom
     npix = nvol * F * H * W
