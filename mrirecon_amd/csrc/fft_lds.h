@@ -186,3 +186,36 @@ __device__ __forceinline__ void fft_lds_wave(float2* buf, const float2* tw, int 
   fft_sync<true>();
   FFTStages<N, 1, INV, 64, true>::run(buf, tw, lane);
 }
+
+// Same transform of a column whose rows [a, b) are zero (the zero padding of a
+// centred, zero-padded image column): the first pass reads only rows outside
+// [a, b), so the caller neither writes nor clears the padding.
+template <int N, bool INV>
+__device__ __forceinline__ void fft_lds_wave_padded(float2* buf, const float2* tw, int lane, int a, int b) {
+  constexpr int R = fft_next_radix(N), NBF = N / R, NB = (NBF + 63) / 64;
+  fft_sync<true>();
+  float2 v[NB][R];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int j = lane + q * 64;
+    if (NBF % 64 == 0 || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int row = j + r * NBF;
+        v[q][r] = (row < a || row >= b) ? buf[PIDX(row)] : make_float2(0.f, 0.f);
+      }
+      DFT<R, INV>::run(v[q]);
+    }
+  }
+  fft_sync<true>();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int j = lane + q * 64;
+    if (NBF % 64 == 0 || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[PIDX(j * R + r)] = v[q][r];
+    }
+  }
+  fft_sync<true>();
+  FFTStages<N, R, INV, 64, true>::run(buf, tw, lane);
+}

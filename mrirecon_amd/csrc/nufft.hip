@@ -5,16 +5,17 @@
 //                     along the second image axis for every image row (LDS
 //                     Stockham), written to the strip-major intermediate T
 //                     (cf, strip, S0, Tc).  Also clears this coil-frame's y.
-//   2. k_fwd_strips : one workgroup per (coil-frame, strip of Tc grid
-//                     columns): loads the S0 x Tc block of T, zero-pads to G0,
-//                     FFTs the Tc columns in LDS, then interpolates every
+//   2. k_fwd_strips : one workgroup per (frame, strip of Tc grid columns,
+//                     chunk of coils): loads the S0 x Tc block of T, FFTs the
+//                     Tc columns in LDS (the first pass skips the zero
+//                     padding to G0 rows), then interpolates every
 //                     sample whose KB footprint touches the strip straight out
 //                     of LDS.  A footprint spans at most two strips, so each
 //                     sample receives at most two partial sums (atomic adds of
 //                     two terms onto zero commute exactly).
 // Adjoint  A^H: the same two passes reversed: k_adj_strips spreads the strip's
-//   samples into an LDS-resident strip (LDS float atomics), inverse-FFTs the
-//   columns and writes only the S0 cropped rows; k_adj_rows inverse-FFTs each
+//   samples into an LDS-resident strip (exact int64 fixed-point LDS atomics),
+//   inverse-FFTs the columns and writes only the S0 cropped rows; k_adj_rows inverse-FFTs each
 //   row of every coil, crops, deapodises and accumulates conj(map_c) over the
 //   coils in registers.
 // The G0 x G1 oversampled grid never exists in HBM.
@@ -504,13 +505,10 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
       const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
       smem[j * CS + PIDX(r)] = Tbb[c * cstride + e];
     }
-    for (int e = tid; e < (G0 - S0) * TC; e += NTH) {      // zero-padded rows
-      const int r = S0 - h0 + e / TC, j = e - (e / TC) * TC;
-      smem[j * CS + PIDX(r)] = make_float2(0.f, 0.f);
-    }
     if (c + 1 < ce) fetch(c + 1);
     __syncthreads();                       // strip filled (and every thread is past coil c-1)
-    fft_lds_wave<G0, false>(smem + col * CS, stw, lane);
+    // rows [S0 - h0, G0 - h0) are the zero padding: never written, skipped by the first pass
+    fft_lds_wave_padded<G0, false>(smem + col * CS, stw, lane, S0 - h0, G0 - h0);
     __syncthreads();                       // all columns transformed
     float2* yc = ybase + (int64_t)c * Kb;
 #pragma unroll
