@@ -50,6 +50,20 @@ __device__ __forceinline__ bf16x8 load8(const u16* p) {
 }
 __device__ __forceinline__ bf16x8 zero8() { return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0)); }
 
+// Workgroups are dealt to the 8 XCDs round-robin in dispatch order.  Remap the
+// linear workgroup index so that each XCD runs a contiguous range of tiles
+// (pixel tiles fastest): the halo rows a tile shares with its neighbours are
+// then fetched into one XCD's L2 instead of several.  A bijection of
+// [0, gx*gy) for any grid size.
+__device__ __forceinline__ void xcd_tile(int& bx, int& by) {
+  const int gx = gridDim.x, n = gx * gridDim.y;
+  const int b = blockIdx.x + blockIdx.y * gx;
+  const int q = n / 8, rem = n % 8, x = b % 8, k = b / 8;
+  const int t = x * q + min(x, rem) + k;
+  bx = t % gx;
+  by = t / gx;
+}
+
 // ------------------------------------------------------------------ forward / backward-data
 // Grid (ceil(npix / 128), Cout_pad / 32), 256 threads; wave w computes couts
 // [32 by, +32) x pixels [128 bx + 32 w, +32) as 2 x 2 MFMA tiles.
@@ -163,7 +177,12 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const u16* __restrict__ x,
   static_assert(KC == 16 || KC % 32 == 0, "channel chunk");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int r = lane & 15, kg = lane >> 4;
-  const int p0 = blockIdx.x * BP, co0 = blockIdx.y * BC;
+  // 3x3 taps: neighbouring tiles share halo rows, so each XCD takes a contiguous run of
+  // tiles.  Temporal taps reuse rows a whole frame away, which the plain dispatch order
+  // already keeps on one XCD (and the contiguous runs would evict before reuse).
+  int bx = blockIdx.x, by = blockIdx.y;
+  if (g.kind == MRH_UNET_TAP_3X3) xcd_tile(bx, by);
+  const int p0 = bx * BP, co0 = by * BC;
   const int HW = g.H * g.W;
   const int nrows = g.kind == MRH_UNET_TAP_1X1 ? 1 : 3;
   const int RS = g.kind == MRH_UNET_TAP_3X3 ? g.W : (g.kind == MRH_UNET_TAP_T3 ? HW : 0);
