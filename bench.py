@@ -246,7 +246,9 @@ def batched_uneven(args, dev):
                        f"temporal TV 0.02", "nufft": rate, "plain_seconds": round(t_plain, 4),
            "plain_iterations_median": float(np.median(it_p)), "plain_iterations_max": int(it_p.max()),
            "plain_slices_per_second": round(P / t_plain, 2)}
-    weights = os.path.join(ROOT, "mrirecon_amd", "weights", "unet_grasp.npz")
+    wdir = os.path.join(ROOT, "mrirecon_amd", "weights")
+    own = os.path.exists(os.path.join(wdir, "unet_batched.npz"))
+    weights = os.path.join(wdir, "unet_batched.npz" if own else "unet_grasp.npz")
     net = UNet2DT.load(weights, device=dev)
     obj = ObjectiveEval(plan, y, regm, problems=P, frames=F)
     x0 = torch.empty_like(x)
@@ -266,8 +268,11 @@ def batched_uneven(args, dev):
                 "warm_iterations_max": int(it_w.max()), "warm_slices_per_second": round(P / t_warm, 2),
                 "speedup": round(t_plain / t_warm, 3), "fallback_rate": float(np.mean(~acc)),
                 "warm_final_objective_rel_to_plain_median": round(float(np.median((f_warm - f_plain) / f_plain)), 6),
-                "note": "U-Net trained on the GRASP workload (34 frames, 12 coils, 21 spokes, spatial+temporal TV); "
-                        "applied unchanged here, with the per-slice fallback"})
+                "weights": os.path.relpath(weights, ROOT),
+                "note": ("U-Net trained on single slices of this workload (20 frames, 8 coils, 13..55 spokes per "
+                         "frame, temporal TV) with spoke patterns and phantoms disjoint from these slices"
+                         if own else "U-Net trained on the GRASP workload (34 frames, 12 coils, 21 spokes, "
+                                     "spatial+temporal TV), applied unchanged, with the per-slice fallback")})
     return out
 
 

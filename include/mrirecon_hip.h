@@ -92,6 +92,18 @@ int mrh_nufft_forward(mrh_nufft_plan* plan, const void* x, void* y, int like, in
  * and NUFFT.AH (graspcg/nufft/api.py:223-270). */
 int mrh_nufft_adjoint(mrh_nufft_plan* plan, const void* y, void* x, int like, void* stream);
 
+/* Forward / adjoint restricted to the frames of active problems: frame b
+ * belongs to problem b / frames_per_problem, and its workgroups run only when
+ * problem_active[problem] != 0 (device int array, e.g. mrh_cg_active_flags);
+ * the k-space (forward) or image (adjoint) entries of skipped frames keep their
+ * previous values.  Lets a batch of slices that stop at different iterations
+ * stop paying for the NUFFT of the finished ones.  No reference counterpart
+ * (the reference has no batched stop). */
+int mrh_nufft_forward_masked(mrh_nufft_plan* plan, const void* x, void* y, int like, int x_has_coils,
+                             const int* problem_active, int frames_per_problem, void* stream);
+int mrh_nufft_adjoint_masked(mrh_nufft_plan* plan, const void* y, void* x, int like,
+                             const int* problem_active, int frames_per_problem, void* stream);
+
 /* The two kernels of each direction, launched separately so that a caller can
  * time them with events on `stream`.  Forward: stage 0 = row pass
  * (maps * deapod * zero-pad * row FFT, clears y), stage 1 = strip pass (column
@@ -238,6 +250,10 @@ int mrh_cg_direction(mrh_cg* eng, void* d, const void* g, void* g_prev, int prim
  * Used to time how long a run takes to reach a given objective (e.g. another
  * run's final value); not a reference stop rule.  Synchronises `stream`. */
 int mrh_cg_set_targets(mrh_cg* eng, const double* targets, void* stream);
+
+/* Device int[P]: 1 while problem p iterates, 0 once it stopped (updated by the
+ * engine's kernels; reset to 1 by mrh_cg_reset).  Owned by the engine. */
+int mrh_cg_active_flags(mrh_cg* eng, const int** flags);
 
 /* Copy status (int[P]) and iteration counts (int[P]) to host (synchronising `stream`). */
 int mrh_cg_read_status(mrh_cg* eng, int* status, int* iters, void* stream);

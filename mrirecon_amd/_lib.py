@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -71,6 +71,8 @@ _SIGS = {
     "mrh_nufft_set_coil_chunk": (c_int, [c_void_p, c_int]),
     "mrh_nufft_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "mrh_nufft_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
+    "mrh_nufft_forward_masked": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_void_p, c_int, c_void_p]),
+    "mrh_nufft_adjoint_masked": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "mrh_nufft_forward_stage": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "mrh_nufft_adjoint_stage": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
     "mrh_reduce_workspace_len": (c_int64, []),
@@ -87,6 +89,7 @@ _SIGS = {
     "mrh_cg_gradient": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "mrh_cg_direction": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "mrh_cg_set_targets": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
+    "mrh_cg_active_flags": (c_int, [c_void_p, POINTER(c_void_p)]),
     "mrh_cg_read_status": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), c_void_p]),
     "mrh_cg_read_history": (c_int, [c_void_p, POINTER(CGHist), c_int, c_void_p]),
     "mrh_unet_conv_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 9 + [c_void_p]),

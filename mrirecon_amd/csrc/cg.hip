@@ -288,11 +288,13 @@ struct mrh_cg {
   ProbState* d_st;
   mrh_cg_hist* d_hist;  // [max_hist][P]
   double* d_target;     // [P] objective targets (-inf: none)
+  int* d_active;        // [P] 1 while the problem iterates (read by the masked NUFFT entry points)
 };
 
-__global__ void k_cg_reset(ProbState* st, int P) {
+__global__ void k_cg_reset(ProbState* st, int* active, int P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
+  active[p] = 1;
   ProbState s;
   double* z = (double*)&s;
   for (size_t i = 0; i < sizeof(ProbState) / sizeof(double); ++i) z[i] = 0.0;
@@ -398,7 +400,7 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
 // Finish the line-search reductions and select the step (one block per problem).
 __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* part_i, const double* part_k,
                                                   int nb_i, int nb_k, double c1, double ls_beta, int ls_max,
-                                                  mrh_cg_hist* hist, int max_hist, int P) {
+                                                  mrh_cg_hist* hist, int max_hist, int P, int* active) {
   const int p = blockIdx.x;
   __shared__ double vals[3 + 2 + NT];
   for (int v = 0; v < 3; ++v) {
@@ -421,6 +423,7 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
   s.ft = f0;
   if (!(s.g0d < 0.0) || !isfinite(s.g0d)) {
     s.status = MRH_CG_NOT_DESCENT;
+    active[p] = 0;
     return;
   }
   double t = 1.0;
@@ -430,7 +433,7 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
     if (ft <= f0 + c1 * t * s.g0d) { s.t = t; s.ft = ft; ok = true; break; }
     t *= ls_beta;
   }
-  if (!ok) { s.status = MRH_CG_LS_FAILED; s.t = 0.0; return; }
+  if (!ok) { s.status = MRH_CG_LS_FAILED; s.t = 0.0; active[p] = 0; return; }
   if (s.iter < max_hist) {
     mrh_cg_hist& h = hist[(int64_t)s.iter * P + p];
     h.f0 = f0; h.g0d = s.g0d; h.t = s.t; h.ft = s.ft;
@@ -505,7 +508,7 @@ __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, fl
 __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* part_i, int nb_i, int prime,
                                                 int direction, double rel_tol, double restart_c,
                                                 const double* __restrict__ target, mrh_cg_hist* hist,
-                                                int max_hist, int P) {
+                                                int max_hist, int P, int* active) {
   const int p = blockIdx.x;
   __shared__ double vals[5];
   for (int v = 0; v < 5; ++v) {
@@ -546,6 +549,7 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
   if (rel_tol > 0.0 && s.have_f_prev && fabs(s.ft - s.f_prev) <= rel_tol * fabs(s.f_prev))
     s.status = MRH_CG_REL_TOL;
   if (s.ft <= target[p]) s.status = MRH_CG_TARGET;
+  if (s.status != MRH_CG_ACTIVE) active[p] = 0;
   s.f_prev = s.ft;
   s.have_f_prev = 1;
 }
@@ -610,6 +614,7 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   ALLOC(e->d_st, sizeof(ProbState) * e->P);
   ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
   ALLOC(e->d_target, sizeof(double) * e->P);
+  ALLOC(e->d_active, sizeof(int) * e->P);
 #undef ALLOC
   {
     std::vector<double> none(e->P, -HUGE_VAL);
@@ -618,7 +623,7 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   }
   err = hipMemcpy(e->d_kprob, d->k_prob, sizeof(int64_t) * (e->P + 1), hipMemcpyHostToDevice);
   if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: copy failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
-  hipLaunchKernelGGL(k_cg_reset, dim3((e->P + 63) / 64), dim3(64), 0, 0, e->d_st, e->P);
+  hipLaunchKernelGGL(k_cg_reset, dim3((e->P + 63) / 64), dim3(64), 0, 0, e->d_st, e->d_active, e->P);
   err = hipDeviceSynchronize();
   if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: reset failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
   *out = e;
@@ -627,7 +632,7 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
 
 extern "C" int mrh_cg_destroy(mrh_cg* e) {
   if (!e) return MRH_OK;
-  void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist, e->d_target};
+  void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist, e->d_target, e->d_active};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete e;
   return MRH_OK;
@@ -635,7 +640,8 @@ extern "C" int mrh_cg_destroy(mrh_cg* e) {
 
 extern "C" int mrh_cg_reset(mrh_cg* e, void* stream) {
   MRH_REQUIRE(e, MRH_EINVAL, "cg_reset: null engine");
-  hipLaunchKernelGGL(k_cg_reset, dim3((e->P + 63) / 64), dim3(64), 0, (hipStream_t)stream, e->d_st, e->P);
+  hipLaunchKernelGGL(k_cg_reset, dim3((e->P + 63) / 64), dim3(64), 0, (hipStream_t)stream, e->d_st, e->d_active,
+                     e->P);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -659,7 +665,7 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
                      (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
-                     e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P);
+                     e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -685,7 +691,8 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
                      (const float2*)d, (const float2*)g_prev, e->tv, e->npix_p, e->d_st, prime, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
-                     e->direction, e->rel_tol, e->restart_c, e->d_target, e->d_hist, e->max_hist, e->P);
+                     e->direction, e->rel_tol, e->restart_c, e->d_target, e->d_hist, e->max_hist, e->P,
+                     e->d_active);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -695,6 +702,12 @@ extern "C" int mrh_cg_direction(mrh_cg* e, void* d, const void* g, void* g_prev,
   hipLaunchKernelGGL(k_cg_direction, dim3(e->nb_i, e->P), dim3(RT), 0, (hipStream_t)stream, (float2*)d,
                      (const float2*)g, (float2*)g_prev, e->npix_p, e->d_st, prime);
   MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_active_flags(mrh_cg* e, const int** flags) {
+  MRH_REQUIRE(e && flags, MRH_EINVAL, "cg_active_flags: null argument");
+  *flags = e->d_active;
   return MRH_OK;
 }
 

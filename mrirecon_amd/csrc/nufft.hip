@@ -311,6 +311,18 @@ extern "C" int mrh_nufft_plan_info(const mrh_nufft_plan* p, mrh_nufft_info* info
 // ---------------------------------------------------------------- kernels
 #define ROWS_PER_BLOCK 4
 
+// Frames of stopped problems can be skipped: problem p = (bb / like) / fpp is
+// processed only when m[p] != 0 (m == nullptr: every frame).  Workgroups of
+// skipped frames return before touching memory, so their k-space / image
+// entries keep their previous values.
+struct FrameMask {
+  const int* m;
+  int fpp, like;
+};
+__device__ __forceinline__ bool frame_skipped(const FrameMask& fm, int bb) {
+  return fm.m != nullptr && fm.m[(bb / fm.like) / fm.fpp] == 0;
+}
+
 // Forward row pass.  Grid (ceil(S0/4), nbatch), 256 threads: one wavefront per
 // image row, looping over the coils (the deapodised image row is loaded once;
 // the next coil's map row is fetched while the current row is transformed).
@@ -319,13 +331,14 @@ __global__ __launch_bounds__(256) void k_fwd_rows(
     const float2* __restrict__ x, const float2* __restrict__ maps, const float* __restrict__ dx,
     const float* __restrict__ dy, const float2* __restrict__ tw1, float2* __restrict__ T,
     float2* __restrict__ y, const int64_t* __restrict__ koff, int S0, int S1, int C, int like,
-    int Tc, int x_has_coils) {
+    int Tc, int x_has_coils, FrameMask fm) {
   constexpr int LP = fft_padded(G1);
   constexpr int NQ = (G1 + 63) / 64;
   __shared__ float2 sbuf[ROWS_PER_BLOCK * LP];
   __shared__ float2 stw[G1];
   const int tid = threadIdx.x, lane = tid & 63, wrow = tid >> 6;
   const int bb = blockIdx.y;
+  if (frame_skipped(fm, bb)) return;
   const int ix = blockIdx.x * ROWS_PER_BLOCK + wrow;
   for (int i = tid; i < G1; i += blockDim.x) stw[i] = tw1[i];
   {  // clear this block's share of the frame's k-space (the strip pass accumulates into it)
@@ -451,10 +464,12 @@ template <int G0, int WT>
 __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
     const float2* __restrict__ T, float2* __restrict__ y, const int2* __restrict__ em,
     const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk) {
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk,
+    FrameMask fm) {
   constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
+  if (frame_skipped(fm, blockIdx.y)) return;
   const int cb = blockIdx.z * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
   __shared__ float2 smem2[2 * TC * CS];   // double buffer: coil c fills one half while c-1 is read
   __shared__ float2 stw[G0];
@@ -577,10 +592,12 @@ template <int G0, int WT>
 __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
     const float2* __restrict__ y, float2* __restrict__ T, const int2* __restrict__ em,
     const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk) {
+    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk,
+    FrameMask fm) {
   constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
+  if (frame_skipped(fm, blockIdx.y)) return;
   const int cb = blockIdx.z * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
   __shared__ unsigned long long acc[2 * TC * CS];
   __shared__ float2 stw[G0];
@@ -691,13 +708,14 @@ template <int G1>
 __global__ __launch_bounds__(256) void k_adj_rows(
     const float2* __restrict__ T, const float2* __restrict__ maps, const float* __restrict__ dx,
     const float* __restrict__ dy, const float2* __restrict__ tw1, float2* __restrict__ x,
-    int S0, int S1, int C, int Tc) {
+    int S0, int S1, int C, int Tc, FrameMask fm) {
   constexpr int LP = fft_padded(G1);
   constexpr int NA = (G1 + 63) / 64;
   __shared__ float2 sbuf[ROWS_PER_BLOCK * LP];
   __shared__ float2 stw[G1];
   const int tid = threadIdx.x, lane = tid & 63, wrow = tid >> 6;
   const int bb = blockIdx.y;
+  if (frame_skipped(fm, bb)) return;
   const int ix = blockIdx.x * ROWS_PER_BLOCK + wrow;
   const bool row_ok = ix < S0;
   for (int i = tid; i < G1; i += blockDim.x) stw[i] = tw1[i];
@@ -757,33 +775,33 @@ __global__ __launch_bounds__(256) void k_adj_rows(
 // ---------------------------------------------------------------- dispatch
 template <int G>
 static int launch_rows_fwd(const mrh_nufft_plan* p, const float2* x, float2* y, int nb, int like,
-                           int x_has_coils, hipStream_t st) {
+                           int x_has_coils, FrameMask fm, hipStream_t st) {
   dim3 grid((p->S0 + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK, nb);
   hipLaunchKernelGGL(k_fwd_rows<G>, grid, dim3(256), 0, st, x, p->d_maps, p->d_dx, p->d_dy,
-                     p->d_tw1, p->d_T, y, p->d_koff, p->S0, p->S1, p->C, like, p->Tc, x_has_coils);
+                     p->d_tw1, p->d_T, y, p->d_koff, p->S0, p->S1, p->C, like, p->Tc, x_has_coils, fm);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
 
 template <int G>
-static int launch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStream_t st) {
+static int launch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, FrameMask fm, hipStream_t st) {
   dim3 grid((p->S0 + ROWS_PER_BLOCK - 1) / ROWS_PER_BLOCK, nb);
   hipLaunchKernelGGL(k_adj_rows<G>, grid, dim3(256), 0, st, p->d_T, p->d_maps, p->d_dx, p->d_dy,
-                     p->d_tw1, x, p->S0, p->S1, p->C, p->Tc);
+                     p->d_tw1, x, p->S0, p->S1, p->C, p->Tc, fm);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
 
 template <int G, int WT>
 static int launch_strips(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
-                         int nb, int like, hipStream_t st) {
+                         int nb, int like, FrameMask fm, hipStream_t st) {
   dim3 grid(p->nstrips, nb, (p->C + p->cchunk - 1) / p->cchunk), block(STRIP_THREADS);
   if (adjoint)
     hipLaunchKernelGGL((k_adj_strips<G, WT>), grid, block, 0, st, yin, p->d_T, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk);
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk, fm);
   else
     hipLaunchKernelGGL((k_fwd_strips<G, WT>), grid, block, 0, st, p->d_T, yout, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk);
+                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk, fm);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -791,9 +809,9 @@ static int launch_strips(const mrh_nufft_plan* p, bool adjoint, const float2* yi
 #define MRH_FOR_EACH_G(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)
 
 static int dispatch_rows_fwd(const mrh_nufft_plan* p, const float2* x, float2* y, int nb, int like,
-                             int xc, hipStream_t st) {
+                             int xc, FrameMask fm, hipStream_t st) {
   switch (p->G1) {
-#define X(G) case G: return launch_rows_fwd<G>(p, x, y, nb, like, xc, st);
+#define X(G) case G: return launch_rows_fwd<G>(p, x, y, nb, like, xc, fm, st);
     MRH_FOR_EACH_G(X)
 #undef X
   }
@@ -801,9 +819,9 @@ static int dispatch_rows_fwd(const mrh_nufft_plan* p, const float2* x, float2* y
   return MRH_EUNSUP;
 }
 
-static int dispatch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStream_t st) {
+static int dispatch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, FrameMask fm, hipStream_t st) {
   switch (p->G1) {
-#define X(G) case G: return launch_rows_adj<G>(p, x, nb, st);
+#define X(G) case G: return launch_rows_adj<G>(p, x, nb, fm, st);
     MRH_FOR_EACH_G(X)
 #undef X
   }
@@ -813,9 +831,9 @@ static int dispatch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, hipStre
 
 template <int WT>
 static int dispatch_strips_w(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
-                             int nb, int like, hipStream_t st) {
+                             int nb, int like, FrameMask fm, hipStream_t st) {
   switch (p->G0) {
-#define X(G) case G: return launch_strips<G, WT>(p, adj, yin, yout, nb, like, st);
+#define X(G) case G: return launch_strips<G, WT>(p, adj, yin, yout, nb, like, fm, st);
     MRH_FOR_EACH_G(X)
 #undef X
   }
@@ -824,12 +842,12 @@ static int dispatch_strips_w(const mrh_nufft_plan* p, bool adj, const float2* yi
 }
 
 static int dispatch_strips(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
-                           int nb, int like, hipStream_t st) {
+                           int nb, int like, FrameMask fm, hipStream_t st) {
   switch (p->W) {
-    case 4: return dispatch_strips_w<4>(p, adj, yin, yout, nb, like, st);
-    case 6: return dispatch_strips_w<6>(p, adj, yin, yout, nb, like, st);
-    case 8: return dispatch_strips_w<8>(p, adj, yin, yout, nb, like, st);
-    default: return dispatch_strips_w<0>(p, adj, yin, yout, nb, like, st);
+    case 4: return dispatch_strips_w<4>(p, adj, yin, yout, nb, like, fm, st);
+    case 6: return dispatch_strips_w<6>(p, adj, yin, yout, nb, like, fm, st);
+    case 8: return dispatch_strips_w<8>(p, adj, yin, yout, nb, like, fm, st);
+    default: return dispatch_strips_w<0>(p, adj, yin, yout, nb, like, fm, st);
   }
 }
 
@@ -839,9 +857,10 @@ extern "C" int mrh_nufft_forward(mrh_nufft_plan* p, const void* x, void* y, int 
   MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
   hipStream_t st = (hipStream_t)stream;
   const int nb = p->B * like;
-  int rc = dispatch_rows_fwd(p, (const float2*)x, (float2*)y, nb, like, x_has_coils ? 1 : 0, st);
+  const FrameMask fm{nullptr, 1, like};
+  int rc = dispatch_rows_fwd(p, (const float2*)x, (float2*)y, nb, like, x_has_coils ? 1 : 0, fm, st);
   if (rc) return rc;
-  return dispatch_strips(p, false, nullptr, (float2*)y, nb, like, st);
+  return dispatch_strips(p, false, nullptr, (float2*)y, nb, like, fm, st);
 }
 
 extern "C" int mrh_nufft_adjoint(mrh_nufft_plan* p, const void* y, void* x, int like, void* stream) {
@@ -849,9 +868,38 @@ extern "C" int mrh_nufft_adjoint(mrh_nufft_plan* p, const void* y, void* x, int 
   MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
   hipStream_t st = (hipStream_t)stream;
   const int nb = p->B * like;
-  int rc = dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, st);
+  const FrameMask fm{nullptr, 1, like};
+  int rc = dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, fm, st);
   if (rc) return rc;
-  return dispatch_rows_adj(p, (float2*)x, nb, st);
+  return dispatch_rows_adj(p, (float2*)x, nb, fm, st);
+}
+
+extern "C" int mrh_nufft_forward_masked(mrh_nufft_plan* p, const void* x, void* y, int like, int x_has_coils,
+                                        const int* problem_active, int frames_per_problem, void* stream) {
+  MRH_REQUIRE(p && x && y && problem_active, MRH_EINVAL, "nufft_forward_masked: null argument");
+  MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
+  MRH_REQUIRE(frames_per_problem >= 1 && p->B % frames_per_problem == 0, MRH_EINVAL,
+              "frames_per_problem=%d does not divide the %d frames", frames_per_problem, p->B);
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = p->B * like;
+  const FrameMask fm{problem_active, frames_per_problem, like};
+  int rc = dispatch_rows_fwd(p, (const float2*)x, (float2*)y, nb, like, x_has_coils ? 1 : 0, fm, st);
+  if (rc) return rc;
+  return dispatch_strips(p, false, nullptr, (float2*)y, nb, like, fm, st);
+}
+
+extern "C" int mrh_nufft_adjoint_masked(mrh_nufft_plan* p, const void* y, void* x, int like,
+                                        const int* problem_active, int frames_per_problem, void* stream) {
+  MRH_REQUIRE(p && x && y && problem_active, MRH_EINVAL, "nufft_adjoint_masked: null argument");
+  MRH_REQUIRE(like >= 1 && like <= p->like_max, MRH_EINVAL, "like=%d outside [1, %d]", like, p->like_max);
+  MRH_REQUIRE(frames_per_problem >= 1 && p->B % frames_per_problem == 0, MRH_EINVAL,
+              "frames_per_problem=%d does not divide the %d frames", frames_per_problem, p->B);
+  hipStream_t st = (hipStream_t)stream;
+  const int nb = p->B * like;
+  const FrameMask fm{problem_active, frames_per_problem, like};
+  int rc = dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, fm, st);
+  if (rc) return rc;
+  return dispatch_rows_adj(p, (float2*)x, nb, fm, st);
 }
 
 extern "C" int mrh_nufft_forward_stage(mrh_nufft_plan* p, int stage, const void* x, void* y, int like,
@@ -861,8 +909,9 @@ extern "C" int mrh_nufft_forward_stage(mrh_nufft_plan* p, int stage, const void*
   MRH_REQUIRE(stage == 0 || stage == 1, MRH_EINVAL, "stage must be 0 (rows) or 1 (strips)");
   hipStream_t st = (hipStream_t)stream;
   const int nb = p->B * like;
-  if (stage == 0) return dispatch_rows_fwd(p, (const float2*)x, (float2*)y, nb, like, x_has_coils ? 1 : 0, st);
-  return dispatch_strips(p, false, nullptr, (float2*)y, nb, like, st);
+  const FrameMask fm{nullptr, 1, like};
+  if (stage == 0) return dispatch_rows_fwd(p, (const float2*)x, (float2*)y, nb, like, x_has_coils ? 1 : 0, fm, st);
+  return dispatch_strips(p, false, nullptr, (float2*)y, nb, like, fm, st);
 }
 
 extern "C" int mrh_nufft_adjoint_stage(mrh_nufft_plan* p, int stage, const void* y, void* x, int like,
@@ -872,6 +921,7 @@ extern "C" int mrh_nufft_adjoint_stage(mrh_nufft_plan* p, int stage, const void*
   MRH_REQUIRE(stage == 0 || stage == 1, MRH_EINVAL, "stage must be 0 (strips) or 1 (rows)");
   hipStream_t st = (hipStream_t)stream;
   const int nb = p->B * like;
-  if (stage == 0) return dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, st);
-  return dispatch_rows_adj(p, (float2*)x, nb, st);
+  const FrameMask fm{nullptr, 1, like};
+  if (stage == 0) return dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, fm, st);
+  return dispatch_rows_adj(p, (float2*)x, nb, fm, st);
 }

@@ -90,6 +90,20 @@ class NufftPlan:
         _lib.check(self._lib.mrh_nufft_adjoint(self.handle, y.data_ptr(), x.data_ptr(), int(like),
                                                _lib.stream_of(y)))
 
+    def forward_masked(self, x: torch.Tensor, y: torch.Tensor, problem_active: int, frames_per_problem: int,
+                       like: int = 1) -> None:
+        """Forward over the frames of active problems only (``problem_active``: device int* address)."""
+        _lib.check(self._lib.mrh_nufft_forward_masked(self.handle, x.data_ptr(), y.data_ptr(), int(like), 0,
+                                                      int(problem_active), int(frames_per_problem),
+                                                      _lib.stream_of(x)))
+
+    def adjoint_masked(self, y: torch.Tensor, x: torch.Tensor, problem_active: int, frames_per_problem: int,
+                       like: int = 1) -> None:
+        """Adjoint over the frames of active problems only."""
+        _lib.check(self._lib.mrh_nufft_adjoint_masked(self.handle, y.data_ptr(), x.data_ptr(), int(like),
+                                                      int(problem_active), int(frames_per_problem),
+                                                      _lib.stream_of(y)))
+
     def forward_stage(self, stage: int, x: torch.Tensor, y: torch.Tensor, like: int = 1) -> None:
         _lib.check(self._lib.mrh_nufft_forward_stage(self.handle, int(stage), x.data_ptr(), y.data_ptr(),
                                                      int(like), 0, _lib.stream_of(x)))

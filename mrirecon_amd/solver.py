@@ -42,7 +42,8 @@ class NLCGEngine:
 
     def __init__(self, plan, y: torch.Tensor, *, problems: int, frames: int, regm: Optional[RegManager],
                  direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5, ls_max_iter: int = 20,
-                 rel_tol: float = 0.0, max_hist: int = 256, restart_c: Optional[float] = None):
+                 rel_tol: float = 0.0, max_hist: int = 256, restart_c: Optional[float] = None,
+                 skip_stopped: bool = True):
         if direction not in ("fr", "prplus"):
             raise ValueError("direction must be 'fr' or 'prplus'")
         self.plan = plan
@@ -82,6 +83,11 @@ class NLCGEngine:
         with torch.cuda.device(dev):
             _lib.check(self._lib.mrh_cg_create(ctypes.byref(desc), ctypes.byref(h)))
         self.handle = h
+        flags = ctypes.c_void_p()
+        _lib.check(self._lib.mrh_cg_active_flags(h, ctypes.byref(flags)))
+        self._active = flags.value
+        # the iteration's NUFFTs skip the frames of problems that have stopped
+        self.skip_stopped = bool(skip_stopped)
         self._graph = None
         self.n_steps = 0
 
@@ -116,11 +122,17 @@ class NLCGEngine:
                                          self.Ax.data_ptr(), self.Ad.data_ptr(), self.y.data_ptr(), s))
         _lib.check(lib.mrh_cg_accept(e, self.x.data_ptr(), self.d.data_ptr(), self.Ax.data_ptr(),
                                      self.Ad.data_ptr(), self.y.data_ptr(), self.r.data_ptr(), s))
-        self.plan.adjoint(self.r, self.g)
+        if self.skip_stopped:
+            self.plan.adjoint_masked(self.r, self.g, self._active, self.F)
+        else:
+            self.plan.adjoint(self.r, self.g)
         _lib.check(lib.mrh_cg_gradient(e, self.x.data_ptr(), self.g.data_ptr(), self.d.data_ptr(),
                                        _lib.ptr(self.g_prev), 0, s))
         _lib.check(lib.mrh_cg_direction(e, self.d.data_ptr(), self.g.data_ptr(), _lib.ptr(self.g_prev), 0, s))
-        self.plan.forward(self.d, self.Ad)
+        if self.skip_stopped:
+            self.plan.forward_masked(self.d, self.Ad, self._active, self.F)
+        else:
+            self.plan.forward(self.d, self.Ad)
 
     def capture(self) -> None:
         """Capture :meth:`step` in a HIP graph (replayed by :meth:`run`)."""

@@ -1,11 +1,12 @@
-"""Train and evaluate the U-Net warm start on the GRASP workload.
+"""Train and evaluate the U-Net warm start on the GRASP or the batched uneven workload.
 
     python -m mrirecon_amd.train_warmstart --train 2000 --test 200 --epochs 2 \
-        --out unet_grasp.npz --log unet_grasp.json
+        --out unet_grasp.npz --log unet_grasp.json [--workload batched]
 
 Data: seeded dynamic phantoms of the GRASP generator (256^2 x 34 frames,
-12 coils, 21 golden-angle spokes x 512 samples per frame, complex Gaussian noise
-of 1 % of max |y|).  Network input: the density-compensated adjoint image
+12 coils, 21 golden-angle spokes x 512 samples per frame; or one slice of the
+batched workload: 20 frames, 8 coils, 13..55 spokes per frame drawn from a pool
+of seeded patterns), complex Gaussian noise of 1 % of max |y|.  Network input: the density-compensated adjoint image
 A^H y.  Target: the reconstruction of a fixed-length PR+ NLCG run (150
 iterations by default) from that adjoint start (temporal TV 0.02 + spatial TV
 0.002), with the beta = 0 restart whenever the next direction would not descend
@@ -31,7 +32,14 @@ from typing import Dict, List, Optional, Sequence
 
 import numpy as np
 
-GRASP = dict(S=256, F=34, C=12, R=512, spokes=21, W=6, lam_t=0.02, lam_s=0.002, noise=0.01)
+# GRASP: 256^2 x 34 frames, 12 coils, 21 spokes x 512 per frame, temporal + spatial TV.
+# batched: one slice of the batched uneven workload, 256^2 x 20 frames, 8 coils,
+# 13..55 spokes x 512 per frame (drawn per frame), temporal TV only.
+WORKLOADS = {
+    "grasp": dict(S=256, F=34, C=12, R=512, spokes=(21, 21), W=6, lam_t=0.02, lam_s=0.002, noise=0.01),
+    "batched": dict(S=256, F=20, C=8, R=512, spokes=(13, 55), W=6, lam_t=0.02, lam_s=0.0, noise=0.01),
+}
+GRASP = WORKLOADS["grasp"]
 
 
 def _phantom(seed: int) -> np.ndarray:
@@ -39,23 +47,38 @@ def _phantom(seed: int) -> np.ndarray:
     return synth.dynamic_phantom(GRASP["S"], GRASP["F"], seed=seed)
 
 
-class GraspProblem:
-    """One GRASP volume on the device: plan, measurement buffer, NLCG engine, objective."""
+def _phantom_sf(args) -> np.ndarray:
+    from . import synth
+    S, F, seed = args
+    return synth.dynamic_phantom(S, F, seed=seed)
 
-    def __init__(self, device="cuda:0", rel_tol: float = 1e-4, max_iter: int = 200,
-                 restart_c: Optional[float] = 0.0, long_iters: int = 150):
+
+def spokes_for(spec: dict, seed: int) -> List[int]:
+    """Spokes per frame: fixed, or drawn uniformly from spec['spokes'] (seeded)."""
+    lo, hi = spec["spokes"]
+    if lo == hi:
+        return [lo] * spec["F"]
+    return [int(v) for v in np.random.default_rng(seed).integers(lo, hi + 1, size=spec["F"])]
+
+
+class RadialProblem:
+    """One volume of a radial workload on the device: plan, measurement buffer, NLCG engines, objective."""
+
+    def __init__(self, spec: dict, spokes: Sequence[int], device="cuda:0", rel_tol: float = 1e-4,
+                 max_iter: int = 200, restart_c: Optional[float] = 0.0, long_iters: int = 150,
+                 autotune: bool = True):
         import torch
 
         from . import NLCGEngine, RegManager, TVND, TVParams, synth
         from .nufft.backends.hip import HipNUFFTAdapter
         from .warmstart import ObjectiveEval
 
-        g = GRASP
+        g = self.spec = spec
         self.torch = torch
         self.dev = torch.device(device)
-        traj, dcf, off = synth.radial_frames([g["spokes"]] * g["F"], g["R"])
+        traj, dcf, off = synth.radial_frames(list(spokes), g["R"])
         maps = synth.gaussian_maps(g["C"], g["S"]).astype(np.complex64)
-        be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=g["W"], osf=2.0)
+        be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=g["W"], osf=2.0, autotune=autotune)
         be.prepare((g["C"], g["S"], g["S"]), torch.from_numpy(traj), torch.from_numpy(dcf),
                    torch.from_numpy(maps).to(self.dev), device=self.dev, k_offsets=off)
         self.adapter = be
@@ -63,8 +86,10 @@ class GraspProblem:
         self.y = torch.zeros(self.plan.n_k, dtype=torch.complex64, device=self.dev)
         self.x_true = torch.zeros((g["F"], g["S"], g["S"]), dtype=torch.complex64, device=self.dev)
         self.x_adj = torch.zeros_like(self.x_true)
-        self.regm = RegManager([TVND("tv_t", TVParams(weight=g["lam_t"], axes="temporal")),
-                                TVND("tv_s", TVParams(weight=g["lam_s"], axes="spatial"))])
+        regs = [TVND("tv_t", TVParams(weight=g["lam_t"], axes="temporal"))]
+        if g["lam_s"]:
+            regs.append(TVND("tv_s", TVParams(weight=g["lam_s"], axes="spatial")))
+        self.regm = RegManager(regs)
         self.max_iter = int(max_iter)
         self.long_iters = int(long_iters)
         # stop rule run (relative objective change) and fixed-length / target run (no change rule)
@@ -84,7 +109,7 @@ class GraspProblem:
         torch = self.torch
         self.x_true.copy_(torch.from_numpy(x_true))
         self.plan.forward(self.x_true, self.y)
-        sigma = GRASP["noise"] * float(self.y.abs().max())
+        sigma = self.spec["noise"] * float(self.y.abs().max())
         gen = torch.Generator(device=self.dev)
         gen.manual_seed(int(noise_seed))
         n = torch.randn((2, self.y.numel()), generator=gen, device=self.dev, dtype=torch.float32)
@@ -104,6 +129,15 @@ class GraspProblem:
 
     def history(self, long: bool = False):
         return (self.eng_long if long else self.eng).history()[0]
+
+
+class GraspProblem(RadialProblem):
+    """One GRASP volume (fixed trajectory: 21 spokes in each of 34 frames)."""
+
+    def __init__(self, device="cuda:0", rel_tol: float = 1e-4, max_iter: int = 200,
+                 restart_c: Optional[float] = 0.0, long_iters: int = 150):
+        super().__init__(GRASP, spokes_for(GRASP, 0), device=device, rel_tol=rel_tol, max_iter=max_iter,
+                         restart_c=restart_c, long_iters=long_iters)
 
 
 def _sync(torch):
@@ -164,10 +198,12 @@ def iterations_to_reach(f_path: Sequence[float], target: float) -> Optional[int]
     return None
 
 
-def evaluate(pb: GraspProblem, net, seeds: Sequence[int], phantoms, log=print) -> Dict[str, object]:
-    """Plain vs warm-started NLCG on held-out phantoms (same data, same stop rule)."""
+def evaluate(pb, net, seeds: Sequence[int], phantoms, log=print) -> Dict[str, object]:
+    """Plain vs warm-started NLCG on held-out phantoms (same data, same stop rule).
+    ``pb``: one RadialProblem for every phantom, or a list with one per phantom."""
     rows = []
-    for seed, x_true in zip(seeds, phantoms):
+    pbs = pb if isinstance(pb, (list, tuple)) else [pb] * len(seeds)
+    for pb, seed, x_true in zip(pbs, seeds, phantoms):
         pb.measure(x_true, noise_seed=seed + 7)
         p = run_plain(pb)
         w = run_warm(pb, net)
@@ -232,6 +268,9 @@ def main(argv=None) -> int:
                     help="weight of the objective term f(x_warm) / f(target) in the loss (0: MSE only)")
     ap.add_argument("--target-iters", type=int, default=150,
                     help="NLCG iterations (no change rule) of each training target")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="grasp")
+    ap.add_argument("--patterns", type=int, default=32,
+                    help="random spoke-count patterns cycled through by the training phantoms (varying workloads)")
     args = ap.parse_args(argv)
 
     def log(*a):
@@ -243,32 +282,41 @@ def main(argv=None) -> int:
 
     from .warmstart import ObjectiveGrad, UNet2DT
 
-    pb = GraspProblem(long_iters=args.target_iters)
-    g = GRASP
+    g = WORKLOADS[args.workload]
+    varying = g["spokes"][0] != g["spokes"][1]
+    # training trajectories: one (fixed workloads) or a pool of seeded spoke patterns, each with its
+    # own plan kept for the objective term; held-out phantoms get patterns outside the pool
+    n_pat = args.patterns if varying else 1
+    pbs = [RadialProblem(g, spokes_for(g, 1000 + q), long_iters=args.target_iters)
+           for q in range(n_pat)]
+    pb = pbs[0]
     train_seeds = list(range(args.seed0, args.seed0 + args.train))
     test_seeds = list(range(args.seed0 + 500_000, args.seed0 + 500_000 + args.test))
     shape = (g["F"], g["S"], g["S"])
     X_adj = torch.empty((args.train,) + shape, dtype=torch.complex64, device=pb.dev)
     X_tgt = torch.empty_like(X_adj)
-    Y = torch.empty((args.train, pb.plan.n_k), dtype=torch.complex64, device=pb.dev) if args.obj_weight else None
+    Y: List[object] = [None] * args.train
     f_norm = np.zeros(args.train)
+    pat = np.arange(args.train) % n_pat
     t0 = time.perf_counter()
     chunks = _chunks(train_seeds, 48)
-    pending = pool.map_async(_phantom, chunks[0]) if chunks else None
+    job = lambda ch: pool.map_async(_phantom_sf, [(g["S"], g["F"], s) for s in ch])
+    pending = job(chunks[0]) if chunks else None
     i = 0
     for c, chunk in enumerate(chunks):
         data = pending.get()
         if c + 1 < len(chunks):
-            pending = pool.map_async(_phantom, chunks[c + 1])
+            pending = job(chunks[c + 1])
         for seed, x_true in zip(chunk, data):
+            pb = pbs[pat[i]]
             pb.measure(x_true, noise_seed=seed + 7)
             pb.adjoint()
             pb.solve(pb.x_adj, long=True)
             X_adj[i].copy_(pb.x_adj)
             X_tgt[i].copy_(pb.eng_long.x)
             f_norm[i] = pb.history(long=True)[-1]["f_step"]
-            if Y is not None:
-                Y[i].copy_(pb.y)
+            if args.obj_weight:
+                Y[i] = pb.y.clone()
             i += 1
         log(f"targets {i}/{args.train}: {pb.long_iters}-iteration NLCG reconstructions "
             f"({time.perf_counter() - t0:.0f} s)")
@@ -281,14 +329,15 @@ def main(argv=None) -> int:
     order = np.concatenate([rng.permutation(args.train) for _ in range(int(math.ceil(args.epochs)) + 1)])[:n_steps]
     warm = min(100, max(1, n_steps // 20))
     losses, gaps = [], []
-    objective = ObjectiveGrad(pb.plan, pb.regm, g["F"]) if args.obj_weight else None
+    objectives = [ObjectiveGrad(q.plan, q.regm, g["F"]) for q in pbs] if args.obj_weight else None
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     pend = []
     for s, j in enumerate(order):
         j = int(j)
         lr = args.lr * min(1.0, (s + 1) / warm) * 0.5 * (1 + math.cos(math.pi * s / max(1, n_steps)))
-        if objective is not None:
+        if objectives is not None:
+            objective = objectives[pat[j]]
             pend.append(net.train_step(X_adj[j], X_tgt[j], lr=lr, objective=objective, y=Y[j],
                                        obj_weight=args.obj_weight, f_norm=[f_norm[j]]).clone())
             gaps.append(objective.value() / f_norm[j] - 1.0)
@@ -305,14 +354,20 @@ def main(argv=None) -> int:
     t_train = time.perf_counter() - t1
     net.save(args.out)
     log(f"saved {args.out}")
-    del X_adj, X_tgt, Y
+    del X_adj, X_tgt, Y, objectives
     torch.cuda.empty_cache()
 
     # held-out evaluation
     res = None
     if args.test:
-        test_ph = pool.map(_phantom, test_seeds)
-        res = evaluate(pb, net, test_seeds, test_ph, log=log)
+        test_ph = pool.map(_phantom_sf, [(g["S"], g["F"], s) for s in test_seeds])
+        if varying:
+            del pbs
+            torch.cuda.empty_cache()
+            pbs_test = [RadialProblem(g, spokes_for(g, 900_000 + s)) for s in test_seeds]
+        else:
+            pbs_test = pb
+        res = evaluate(pbs_test, net, test_seeds, test_ph, log=log)
         log(f"held-out: plain median {res['median_plain_s'] * 1e3:.1f} ms / {res['median_plain_iters']:.0f} it, warm "
             f"median {res['median_warm_s'] * 1e3:.1f} ms / {res['median_warm_iters']:.0f} it, speedup "
             f"{res['speedup_median_of_ratios']:.2f}x, fallback {res['fallback_rate']:.2%}; to the plain objective: "
@@ -321,7 +376,8 @@ def main(argv=None) -> int:
     pool.close()
     out = {"train": args.train, "test": args.test, "epochs": args.epochs, "lr": args.lr, "steps": n_steps,
            "data_seconds": t_data, "train_seconds": t_train, "ms_per_step": t_train / max(1, n_steps) * 1e3,
-           "target_iters": args.target_iters, "obj_weight": args.obj_weight,
+           "target_iters": args.target_iters, "obj_weight": args.obj_weight, "workload": args.workload,
+           "patterns": n_pat,
            "gap_curve": [float(np.mean(gaps[k:k + 50])) for k in range(0, len(gaps), 50)],
            "loss_curve": [float(np.mean(losses[k:k + 50])) for k in range(0, len(losses), 50)],
            "eval": res}

@@ -266,3 +266,31 @@ def test_coil_chunk_autotune_keeps_results_identical():
         plan.set_coil_chunk(0)
     with pytest.raises(ValueError):
         plan.set_coil_chunk(9)
+
+
+def test_masked_forward_adjoint_skip_inactive_problems():
+    """Frames of problems whose flag is 0 are neither computed nor written; the others equal the
+    unmasked operator exactly."""
+    S, C, R, F = 48, 3, 96, 3
+    spokes = [13, 21, 17, 30, 11, 25]                 # 2 problems x 3 frames, uneven
+    traj, dcf, off = synth.radial_frames(spokes, R)
+    maps = synth.gaussian_maps(C, S)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=6)
+    be.prepare((C, S, S), torch.from_numpy(traj), torch.from_numpy(dcf), c64(maps), device=DEV, k_offsets=off)
+    plan = be.plan
+    x = c64(crand((2 * F, S, S), 3))
+    y_full = torch.empty(plan.n_k, dtype=torch.complex64, device=DEV)
+    plan.forward(x, y_full)
+    xa_full = torch.empty_like(x)
+    plan.adjoint(y_full, xa_full)
+    active = torch.tensor([0, 1], dtype=torch.int32, device=DEV)
+    y = torch.full_like(y_full, 7.0)
+    plan.forward_masked(x, y, active.data_ptr(), F)
+    xa = torch.full_like(x, 5.0)
+    plan.adjoint_masked(y_full, xa, active.data_ptr(), F)
+    torch.cuda.synchronize()
+    k1 = C * int(off[F])                                # problem 1 owns k-space from frame F on
+    assert torch.equal(y[k1:], y_full[k1:]) and bool((y[:k1] == 7.0).all())
+    assert torch.equal(xa[F:], xa_full[F:]) and bool((xa[:F] == 5.0).all())
+    with pytest.raises(ValueError):
+        plan.forward_masked(x, y, active.data_ptr(), 4)     # 4 does not divide 6 frames

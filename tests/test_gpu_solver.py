@@ -283,3 +283,36 @@ def test_prime_with_precomputed_forward():
     h2 = eng.history()[0]
     assert [r["step"] for r in h1] == [r["step"] for r in h2]
     assert all(abs(a["f_step"] - b["f_step"]) <= 1e-9 * abs(a["f_step"]) for a, b in zip(h1, h2))
+
+
+def test_skipping_stopped_problems_changes_nothing():
+    """With the masked NUFFTs (frames of stopped slices skipped) a batch whose slices stop at
+    different iterations gives the same histories and images as without them."""
+    S, C, R, F = 48, 3, 96, 4
+    spokes = [[13, 21, 17, 30], [25, 14, 40, 19], [33, 12, 28, 16]]
+    traj, dcf, off = synth.radial_frames([s for sp in spokes for s in sp], R)
+    x_true = np.concatenate([synth.dynamic_phantom(S, F, seed=80 + i, n_ellipses=5) for i in range(3)])
+    maps = synth.gaussian_maps(C, S)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=6)
+    be.prepare((C, S, S), torch.from_numpy(traj), torch.from_numpy(dcf), c64(maps), device=DEV, k_offsets=off)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(x_true), y)
+    y += 0.01 * y.abs().max() * torch.randn(y.shape, dtype=torch.complex64, device=DEV,
+                                            generator=torch.Generator(DEV).manual_seed(5))
+    x0 = torch.empty((3 * F, S, S), dtype=torch.complex64, device=DEV)
+    be.plan.adjoint(y, x0)
+    regm = RegManager([TVND("t", TVParams(weight=0.02, axes="temporal"))])
+    out = []
+    for skip in (False, True):
+        eng = NLCGEngine(be.plan, y, problems=3, frames=F, regm=regm, direction="prplus", rel_tol=1e-3,
+                         restart_c=0.0, skip_stopped=skip, max_hist=80)
+        eng.prime(x0)
+        eng.run(60)
+        torch.cuda.synchronize()
+        out.append((eng.history(), eng.x.cpu().numpy(), eng.status()[1]))
+    (h0, x0n, it0), (h1, x1n, it1) = out
+    assert list(it0) == list(it1) and len(set(it0.tolist())) > 1, it0     # slices stop at different iterations
+    for p in range(3):
+        assert [r["step"] for r in h0[p]] == [r["step"] for r in h1[p]]
+        assert [r["f_step"] for r in h0[p]] == [r["f_step"] for r in h1[p]]
+    assert np.array_equal(x0n, x1n)
