@@ -440,6 +440,11 @@ class ObjectiveEval:
         self.k_prob = [C * int(plan.k_offsets[p * self.F]) for p in range(self.P + 1)]
 
     def __call__(self, x: torch.Tensor) -> np.ndarray:
+        self.launch(x)
+        return self.read()
+
+    def launch(self, x: torch.Tensor) -> None:
+        """Enqueue the evaluation of f(x) on the current stream (no host synchronisation)."""
         st = torch.cuda.current_stream(x.device).cuda_stream
         self.plan.forward(x, self.Ax)
         for p in range(self.P):
@@ -449,6 +454,9 @@ class ObjectiveEval:
                                                     self.work.data_ptr(), self.e_data[p:p + 1].data_ptr(), st))
         _lib.check(self.lib.mrh_tv_energy_grad(ctypes.byref(self.tv), x.data_ptr(), None, self.work.data_ptr(),
                                                self.e_tv.data_ptr(), st))
+
+    def read(self) -> np.ndarray:
+        """f per problem of the last launched evaluation (synchronises with its stream)."""
         return 0.5 * self.e_data.cpu().numpy() + self.e_tv.cpu().numpy()
 
 
@@ -504,3 +512,35 @@ def select_warm_start(objective: ObjectiveEval, x_adj: torch.Tensor, x_warm: tor
         for p in np.nonzero(~acc)[0]:
             x0[p * per:(p + 1) * per].copy_(x_adj[p * per:(p + 1) * per])
     return x0, acc, f_w, f_a
+
+
+def warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveEval, x_adj: torch.Tensor,
+               nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None
+               ) -> Tuple[torch.Tensor, np.ndarray, np.ndarray, np.ndarray]:
+    """Network + fallback check with f(x_adj) evaluated on a side stream while the network runs.
+
+    ``objective`` and ``objective_adj`` are two evaluators of the same problem (separate
+    buffers).  The network (MFMA-bound) and the adjoint image's objective (a forward NUFFT,
+    memory / LDS-bound) overlap; f(x_warm) runs after both, since the two NUFFTs share the
+    plan's workspace.  Same result as ``select_warm_start(objective, x_adj, net.infer(...))``,
+    and ``objective.Ax`` holds A x0 when every problem accepted."""
+    main = torch.cuda.current_stream(x_adj.device)
+    side = side_stream or torch.cuda.Stream(device=x_adj.device)
+    ready = torch.cuda.Event()
+    ready.record(main)
+    with torch.cuda.stream(side):
+        side.wait_event(ready)
+        objective_adj.launch(x_adj)
+    x_warm = net.infer(x_adj, nvol=nvol)
+    main.wait_stream(side)
+    objective.launch(x_warm)
+    f_a = objective_adj.read()
+    f_w = objective.read()
+    acc = f_w <= f_a
+    x0 = x_warm if out is None else out.copy_(x_warm)
+    if not acc.all():
+        per = x_adj.shape[0] // objective.P
+        for p in np.nonzero(~acc)[0]:
+            x0[p * per:(p + 1) * per].copy_(x_adj[p * per:(p + 1) * per])
+    return x0, acc, f_w, f_a
+

@@ -352,3 +352,32 @@ def test_trained_warm_start_gain_on_held_out_grasp_volumes():
         assert w["accepted"] and w["f_start"] < 0.2 * w["f_adjoint"]
         assert w["stop"] == "target"
         assert w["iterations"] < p["iterations"]
+
+
+def test_overlapped_warm_start_equals_sequential_selection():
+    """warm_start (network on the main stream, f(A^H y) on a side stream) returns what
+    select_warm_start returns for the same network output."""
+    from mrirecon_amd import RegManager, TVND, TVParams
+    from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+    from mrirecon_amd.warmstart import warm_start
+    P, F, S, C = 2, 4, 64, 4
+    traj, dcf, off = synth.radial_frames([13] * (P * F), 128)
+    x_true = np.concatenate([synth.dynamic_phantom(S, F, seed=90 + p, n_ellipses=5) for p in range(P)])
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=6)
+    be.prepare((C, S, S), torch.from_numpy(traj), torch.from_numpy(dcf),
+               torch.from_numpy(synth.gaussian_maps(C, S).astype(np.complex64)).to(DEV), device=DEV, k_offsets=off)
+    plan = be.plan
+    y = torch.empty(plan.n_k, dtype=torch.complex64, device=DEV)
+    plan.forward(torch.from_numpy(x_true).to(DEV), y)
+    x_adj = torch.empty((P * F, S, S), dtype=torch.complex64, device=DEV)
+    plan.adjoint(y, x_adj)
+    regm = RegManager([TVND("t", TVParams(weight=0.02, axes="temporal"))])
+    net = _random_net(3, seed=11)
+    o1, o2 = ObjectiveEval(plan, y, regm, P, F), ObjectiveEval(plan, y, regm, P, F)
+    x0, acc, f_w, f_a = warm_start(net, o1, o2, x_adj, nvol=P, out=torch.empty_like(x_adj))
+    x0 = x0.clone()
+    xw = net.infer(x_adj, nvol=P).clone()
+    r0, racc, rf_w, rf_a = select_warm_start(ObjectiveEval(plan, y, regm, P, F), x_adj, xw, out=torch.empty_like(x_adj))
+    assert torch.equal(x0, r0) and acc.tolist() == racc.tolist()
+    assert np.allclose(f_w, rf_w, rtol=1e-12) and np.allclose(f_a, rf_a, rtol=1e-12)
+
