@@ -412,6 +412,11 @@ struct Foot {
   float wA, wAH;
 };
 
+// Horner terms evaluated per footprint width: enough for every plan that the
+// dispatch sends to that width (W=4 needs at most 20 terms, W=6 at most 24);
+// the remaining coefficients are zero, so the value equals the full series.
+__host__ __device__ constexpr int kb_terms(int wm) { return wm == 4 ? 20 : wm == 6 ? 24 : MRH_KB_MAXNP; }
+
 template <int WM>
 __device__ __forceinline__ void load_foot(Foot<WM>& F, int2 m, float4 f, int W, int c0, int G1,
                                           const KBPoly& kb) {
@@ -421,19 +426,29 @@ __device__ __forceinline__ void load_foot(Foot<WM>& F, int2 m, float4 f, int W, 
   F.wA = f.z;
   F.wAH = f.w;
 #pragma unroll
-  for (int i = 0; i < WM; ++i) F.wx[i] = (i < W) ? kb_weight<MRH_KB_MAXNP>(kb, f.x - (float)i) : 0.f;
+  for (int i = 0; i < WM; ++i) F.wx[i] = (i < W) ? kb_weight<kb_terms(WM)>(kb, f.x - (float)i) : 0.f;
 #pragma unroll
   for (int j = 0; j < WM; ++j) {
     int cc = F.my + j;
     if (cc >= G1) cc -= G1;
-    F.wy[j] = (j < W && (unsigned)(cc - c0) < (unsigned)STRIP_TC) ? kb_weight<MRH_KB_MAXNP>(kb, f.y - (float)j) : 0.f;
+    F.wy[j] = (j < W && (unsigned)(cc - c0) < (unsigned)STRIP_TC) ? kb_weight<kb_terms(WM)>(kb, f.y - (float)j) : 0.f;
   }
 }
 
 // Forward: sum of the footprint over the LDS strip (column-major, padded rows).
+// Columns outside the strip (weight 0) are skipped, which is nearly uniform
+// over a wavefront (its lanes hold consecutive samples of one spoke); the W
+// reads of a column are issued together.
 template <int WM, int CS>
 __device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* smem, int W, int c0, int G0,
                                              int G1) {
+  int ro[WM];
+#pragma unroll
+  for (int i = 0; i < WM; ++i) {
+    int rr = F.r0 + i;
+    if (rr >= G0) rr -= G0;
+    ro[i] = PIDX(rr);
+  }
   float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
   for (int j = 0; j < WM; ++j) {
@@ -441,15 +456,15 @@ __device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* sm
       int cc = F.my + j;
       if (cc >= G1) cc -= G1;
       const float2* colp = smem + (cc - c0) * CS;
+      float2 g[WM];
+#pragma unroll
+      for (int i = 0; i < WM; ++i) g[i] = colp[ro[i]];
       float2 rs = make_float2(0.f, 0.f);
 #pragma unroll
       for (int i = 0; i < WM; ++i) {
         if (i < W) {
-          int rr = F.r0 + i;
-          if (rr >= G0) rr -= G0;
-          const float2 g = colp[PIDX(rr)];
-          rs.x = fmaf(g.x, F.wx[i], rs.x);
-          rs.y = fmaf(g.y, F.wx[i], rs.y);
+          rs.x = fmaf(g[i].x, F.wx[i], rs.x);
+          rs.y = fmaf(g[i].y, F.wx[i], rs.y);
         }
       }
       acc.x = fmaf(rs.x, F.wy[j], acc.x);
@@ -685,19 +700,23 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
 #pragma unroll
       for (int t = 0; t < NPC; ++t) {
         const int p = lane + 64 * t;
-        if (p < CS) fc[p] = cv[t];
+        if (p < CS) {
+          fc[p] = cv[t];
+          ac[CS + p] = 0ull;
+        }
       }
       fft_lds_wave<G0, true>(fc, stw, lane);
+      for (int r = S0 - h0 + lane; r < G0 - h0; r += 64) fc[PIDX(r)] = make_float2(0.f, 0.f);
     }
     __syncthreads();                       // all columns transformed
     float2* Tb = Tbb + c * cstride;
     for (int e = tid; e < S0 * TC; e += NTH) {
       const int ix = e / TC, j = e - ix * TC;
       const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
-      Tb[e] = fbuf[2 * j * CS + PIDX(r)];
+      float2* sp = fbuf + 2 * j * CS + PIDX(r);
+      Tb[e] = *sp;
+      *sp = make_float2(0.f, 0.f);
     }
-    __syncthreads();                       // strip read out before the next coil clears it
-    for (int i = tid; i < 2 * TC * CS; i += NTH) acc[i] = 0ull;
   }
 }
 
@@ -844,11 +863,16 @@ static int dispatch_strips_w(const mrh_nufft_plan* p, bool adj, const float2* yi
 static int dispatch_strips(const mrh_nufft_plan* p, bool adj, const float2* yin, float2* yout,
                            int nb, int like, FrameMask fm, hipStream_t st) {
   switch (p->W) {
-    case 4: return dispatch_strips_w<4>(p, adj, yin, yout, nb, like, fm, st);
-    case 6: return dispatch_strips_w<6>(p, adj, yin, yout, nb, like, fm, st);
+    case 4:
+      if (p->np <= kb_terms(4)) return dispatch_strips_w<4>(p, adj, yin, yout, nb, like, fm, st);
+      break;
+    case 6:
+      if (p->np <= kb_terms(6)) return dispatch_strips_w<6>(p, adj, yin, yout, nb, like, fm, st);
+      break;
     case 8: return dispatch_strips_w<8>(p, adj, yin, yout, nb, like, fm, st);
-    default: return dispatch_strips_w<0>(p, adj, yin, yout, nb, like, fm, st);
+    default: break;
   }
+  return dispatch_strips_w<0>(p, adj, yin, yout, nb, like, fm, st);   // any W <= 8, full series
 }
 
 extern "C" int mrh_nufft_forward(mrh_nufft_plan* p, const void* x, void* y, int like,
