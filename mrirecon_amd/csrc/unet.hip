@@ -25,6 +25,7 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16;
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(u16 b) { return __uint_as_float((unsigned)b << 16); }
 __device__ __forceinline__ u16 f2bf(float f) {   // round to nearest even (finite inputs)
@@ -284,6 +285,148 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const u16* __restrict__ x,
 #pragma unroll
         for (int j = 0; j < 4; ++j)
           if (cb + j < Cout) out[(size_t)p * Cout + cb + j] = o[j];
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------ 3x3 / temporal forward on tiles
+// Block: 256 output pixels of one image x BC = 16 MT couts; Cin in chunks of
+// 32 (one 16x16x32 K-step per tap).
+//   3x3 (TEMP = false): the tile is TR = 256 / TW rows x TW columns; per chunk
+//     the block stages the (TR + 2) x (TW + 2) halo (zero outside the image).
+//   temporal (TEMP = true): the tile is 256 consecutive pixels of one frame;
+//     per chunk the block stages those pixels in the previous, the same and
+//     the next frame of the volume (zero beyond its first / last frame).
+// The chunk's [tap][BC] weights are staged beside them.  Pixel and cout rows
+// of the LDS tiles are 40 halves (80 B) apart, so the 16 lanes of a fragment
+// read hit distinct banks.  The next chunk's input is fetched into registers
+// while the current one is multiplied.  Wave w owns tile pixels
+// [64 w, 64 w + 64) as four 16-pixel MFMA columns.
+// Needs Cin % 32 == 0 and, for 3x3, W % TW == 0 and H % TR == 0 (temporal:
+// H W % 256 == 0).
+template <int MT, int TW, bool TEMP>
+__global__ __launch_bounds__(256, 2) void k_conv_tile(const u16* __restrict__ x, const u16* __restrict__ w,
+                                                     const float* __restrict__ bias, u16* __restrict__ out, int F,
+                                                     int H, int W, int Cin, int Cout, int Cout_pad, int relu) {
+  constexpr int TR = TEMP ? 1 : 256 / TW, HC = TEMP ? 256 : TW + 2, NHP = (TR + 2) * HC;
+  constexpr int NTAP = TEMP ? 3 : 9, KC = 32, PS = KC + 8, BC = 16 * MT;
+  constexpr int NIN = NHP * 4, IPT = (NIN + 255) / 256;        // 16-byte pieces of the halo, per thread
+  constexpr int NWT = NTAP * BC * 4, WPT = (NWT + 255) / 256;   // and of the weights
+  __shared__ __align__(16) u16 sIn[NHP * PS];
+  __shared__ __align__(16) u16 sW[NTAP * BC * PS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  int bx, by;
+  xcd_tile(bx, by);
+  const int HW = H * W;
+  const int tx_n = TEMP ? 1 : W / TW, tpi = TEMP ? HW / 256 : tx_n * (H / TR);   // tiles per image
+  // temporal: frames fastest, so the three blocks that read a frame's tile run back to back
+  int img, trem;
+  if (TEMP) {
+    const int vol = bx / (F * tpi), vr = bx - vol * (F * tpi);
+    trem = vr / F;
+    img = vol * F + (vr - trem * F);
+  } else {
+    img = bx / tpi;
+    trem = bx - img * tpi;
+  }
+  const int y0 = TEMP ? 0 : (trem / tx_n) * TR, x0 = TEMP ? 0 : (trem % tx_n) * TW;
+  const int p0 = TEMP ? trem * 256 : 0, fv = img % F, co0 = by * BC;
+  const long long ibase = (long long)img * HW;
+  u32x4 pin[IPT], pw[WPT];
+  auto gload = [&](int ci0) {
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int idx = tid + 256 * i;
+      pin[i] = u32x4{0u, 0u, 0u, 0u};
+      if (idx < NIN) {
+        const int hp = idx >> 2, q = idx & 3;
+        const int hy = hp / HC, hx = hp - hy * HC;
+        long long src = -1;
+        if (TEMP) {
+          if ((unsigned)(fv + hy - 1) < (unsigned)F) src = ibase + (long long)(hy - 1) * HW + p0 + hx;
+        } else {
+          const int gy = y0 + hy - 1, gx = x0 + hx - 1;
+          if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) src = ibase + (long long)gy * W + gx;
+        }
+        if (src >= 0) pin[i] = *reinterpret_cast<const u32x4*>(x + src * Cin + ci0 + 8 * q);
+      }
+    }
+  };
+  // halo index of this lane's pixel in each of the wave's four MFMA columns (centre tap)
+  int hp0[4];
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int tp = 64 * wave + 16 * h + r;
+    hp0[h] = TEMP ? HC + tp : (tp / TW + 1) * HC + (tp % TW) + 1;
+  }
+  f32x4 acc[MT][4];
+#pragma unroll
+  for (int m = 0; m < MT; ++m)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) acc[m][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+  gload(0);
+  for (int ci0 = 0; ci0 < Cin; ci0 += KC) {
+    __syncthreads();                       // the previous chunk's fragments have been read
+#pragma unroll
+    for (int i = 0; i < WPT; ++i) {
+      const int idx = tid + 256 * i;
+      if (idx < NWT) {
+        const int t = idx / (BC * 4), rem = idx - t * (BC * 4);
+        pw[i] = *reinterpret_cast<const u32x4*>(w + ((size_t)t * Cout_pad + co0 + (rem >> 2)) * Cin + ci0 +
+                                                8 * (rem & 3));
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < IPT; ++i) {
+      const int idx = tid + 256 * i;
+      if (idx < NIN) *reinterpret_cast<u32x4*>(sIn + (idx >> 2) * PS + 8 * (idx & 3)) = pin[i];
+    }
+#pragma unroll
+    for (int i = 0; i < WPT; ++i) {
+      const int idx = tid + 256 * i;
+      if (idx < NWT) *reinterpret_cast<u32x4*>(sW + (idx >> 2) * PS + 8 * (idx & 3)) = pw[i];
+    }
+    __syncthreads();
+    if (ci0 + KC < Cin) gload(ci0 + KC);
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) {
+      const int off = TEMP ? (t - 1) * HC : (t / 3 - 1) * HC + (t % 3 - 1);
+      bf16x8 a[MT], b[4];
+#pragma unroll
+      for (int m = 0; m < MT; ++m) a[m] = load8(sW + (t * BC + 16 * m + r) * PS + 8 * kg);
+#pragma unroll
+      for (int h = 0; h < 4; ++h) b[h] = load8(sIn + (hp0[h] + off) * PS + 8 * kg);
+#pragma unroll
+      for (int m = 0; m < MT; ++m)
+#pragma unroll
+        for (int h = 0; h < 4; ++h) acc[m][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], b[h], acc[m][h], 0, 0, 0);
+    }
+  }
+  // lane holds couts co0 + 16 m + 4 kg + j of tile pixel 64 w + 16 h + r
+#pragma unroll
+  for (int h = 0; h < 4; ++h) {
+    const int tp = 64 * wave + 16 * h + r;
+    const long long p = ibase + (TEMP ? p0 + tp : (long long)(y0 + tp / TW) * W + x0 + tp % TW);
+#pragma unroll
+    for (int m = 0; m < MT; ++m) {
+      const int cb = co0 + 16 * m + 4 * kg;
+      if (cb >= Cout) continue;
+      u16 o[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        float v = acc[m][h][j] + ((bias && cb + j < Cout) ? bias[cb + j] : 0.f);
+        if (relu) v = fmaxf(v, 0.f);
+        o[j] = f2bf(v);
+      }
+      if (cb + 3 < Cout) {
+        *reinterpret_cast<uint2*>(out + p * Cout + cb) =
+            make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16), (unsigned)o[2] | ((unsigned)o[3] << 16));
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+          if (cb + j < Cout) out[p * Cout + cb + j] = o[j];
       }
     }
   }
@@ -636,16 +779,29 @@ extern "C" int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias
               (long long)npix);
   hipStream_t st = (hipStream_t)stream;
   // Tile choice measured per layer shape on MI355X (34 x 256^2 volume, every layer of the network):
-  // 1x1 convolutions, Cin = 8 and 32-cout temporal layers are fastest on the register tiles; the
-  // 3x3 / temporal layers with >= 64 couts on 64 x 128 LDS tiles (64 x 256 for 3x3 with Cin >= 128);
-  // 3x3 with 32 couts on 32 x 128 tiles with 32-channel chunks.
+  // 3x3 layers with Cin % 32 == 0 on 2-D tiles (4 x 64 or 8 x 32 pixels, 1.7-2.8x the 1-D LDS
+  // tiles), temporal layers with >= 64 couts on 256-pixel tiles; 1x1 convolutions, Cin = 8 and
+  // 32-cout temporal layers on the register tiles; other shapes on the 1-D LDS tiles.
   const u16 *xs = (const u16*)x, *ws = (const u16*)w;
   u16* os = (u16*)out;
 #define MRH_LDS_CONV(MT, NT, KC)                                                                              \
   hipLaunchKernelGGL((k_conv_fwd_lds<MT, NT, KC>),                                                        \
                      dim3((unsigned)((npix + 64 * NT - 1) / (64 * NT)), Cout_pad / (16 * MT)), dim3(256), 0, st, \
                      xs, ws, bias, os, g, (int)npix, Cin, Cout, Cout_pad, relu)
-  if (Cin % 16 != 0 || kind == MRH_UNET_TAP_1X1 || (kind == MRH_UNET_TAP_T3 && Cout_pad == 32)) {
+  const int tw = W >= 64 && W % 64 == 0 ? 64 : (W == 32 ? 32 : 0);
+  const bool tile3 = kind == MRH_UNET_TAP_3X3 && tw && H % (256 / tw) == 0;
+  // temporal tiles pay off with 64-cout blocks; 32-cout temporal layers stay on the register tiles
+  const bool tilet = kind == MRH_UNET_TAP_T3 && (H * W) % 256 == 0 && Cout_pad % 64 == 0;
+  if (Cin % 32 == 0 && (tile3 || tilet)) {
+    const int mt = Cout_pad % 64 == 0 ? 4 : 2;
+    dim3 grid((unsigned)(npix / 256), Cout_pad / (16 * mt));
+#define MRH_TILE_CONV(MT, TW, TEMP) \
+  hipLaunchKernelGGL((k_conv_tile<MT, TW, TEMP>), grid, dim3(256), 0, st, xs, ws, bias, os, F, H, W, Cin, Cout, Cout_pad, relu)
+    if (tilet) MRH_TILE_CONV(4, 256, true);
+    else if (mt == 4) { if (tw == 64) MRH_TILE_CONV(4, 64, false); else MRH_TILE_CONV(4, 32, false); }
+    else { if (tw == 64) MRH_TILE_CONV(2, 64, false); else MRH_TILE_CONV(2, 32, false); }
+#undef MRH_TILE_CONV
+  } else if (Cin % 16 != 0 || kind == MRH_UNET_TAP_1X1 || (kind == MRH_UNET_TAP_T3 && Cout_pad == 32)) {
     hipLaunchKernelGGL(k_conv_fwd, dim3((unsigned)((npix + 127) / 128), Cout_pad / 32), dim3(256), 0, st, xs, ws,
                        bias, os, g, (int)npix, Cin, Cout, Cout_pad, relu);
   } else if (Cout_pad % 64 != 0) {

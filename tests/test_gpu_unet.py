@@ -80,6 +80,34 @@ def test_conv_forward(kind, cin, cout, relu, geom):
     check_bf16_close(out.float().cpu().numpy().astype(np.float64), ref)
 
 
+@pytest.mark.parametrize("geom", [(1, 2, 8, 64), (2, 2, 16, 32), (1, 1, 4, 128), (2, 4, 8, 32)])
+@pytest.mark.parametrize("kind", [ou.TAP_3X3, ou.TAP_T3])
+@pytest.mark.parametrize("cin,cout,relu", [(32, 64, True), (64, 32, False), (128, 64, True), (64, 96, False),
+                                           (32, 48, True), (256, 128, False)])
+def test_conv_tiles(kind, cin, cout, relu, geom):
+    """3x3 convolutions on 2-D tiles (4 x 64 or 8 x 32 pixels) and temporal ones on 256-pixel
+    tiles (Cin % 32 == 0): halo rows and columns at every image border, first and last frames
+    of each volume, 32- and 64-cout tiles, couts that end inside a tile."""
+    lib = _lib.load()
+    nvol, F, H, W = geom
+    npix = nvol * F * H * W
+    T = ou.NTAP[kind]
+    rng = np.random.default_rng(cin * 7 + cout + W + kind)
+    x = bf16_np(rng.standard_normal((nvol * F, H, W, cin)))
+    Wt = bf16_np(rng.standard_normal((T, cout, cin)) * np.sqrt(2.0 / (T * cin)))
+    b = (rng.standard_normal(cout) * 0.1).astype(np.float32)
+    cpad = (cout + 31) // 32 * 32
+    Wp = np.zeros((T, cpad, cin))
+    Wp[:, :cout] = Wt
+    out = torch.full((npix, cout), 7.0, dtype=torch.bfloat16, device=DEV)
+    bd = torch.from_numpy(b).to(DEV)
+    xd, wd = to_dev_bf16(x), to_dev_bf16(Wp)
+    _lib.check(lib.mrh_unet_conv_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr(),
+                                     out.data_ptr(), kind, nvol, F, H, W, cin, cout, cpad, int(relu), st()))
+    ref = oracle_conv(x, Wt, b.astype(np.float64), kind, nvol, F, relu).reshape(npix, cout)
+    check_bf16_close(out.float().cpu().numpy().astype(np.float64), ref)
+
+
 @pytest.mark.parametrize("kind,cin,cout", [(ou.TAP_3X3, 64, 32), (ou.TAP_T3, 32, 32), (ou.TAP_1X1, 256, 128),
                                            (ou.TAP_1X1, 32, 8), (ou.TAP_3X3, 32, 64)])
 def test_conv_backward_data(kind, cin, cout):
