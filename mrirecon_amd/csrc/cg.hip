@@ -21,6 +21,7 @@
 #define RT 256            // threads per reduction block
 #define RED_BLOCKS 1024   // blocks of the standalone reductions
 #define NT 21             // line-search candidates: t = 0, 1, beta, ..., beta^19
+#define NT_A 6            // candidates of the first image pass: t = 0, 1, ..., beta^4
 #define PI_STRIDE 32      // doubles per block partial (image pass)
 #define PK_STRIDE 4       // doubles per block partial (k-space pass)
 
@@ -274,6 +275,7 @@ struct ProbState {
   double tv[NT];
   double f0, t, ft, rho, rho_prev, beta, gpg, freg, gdot_step, f_prev;
   int status, iter, have_rho_prev, have_f_prev;
+  int ls_pending;   // the first line-search pass found no step: the second pass evaluates t <= beta^5
 };
 
 struct mrh_cg {
@@ -331,14 +333,21 @@ __global__ __launch_bounds__(RT) void k_ls_kspace(const float2* __restrict__ Ax,
   block_reduce_store<3>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PK_STRIDE);
 }
 
-// Pass 2 of the line search: Re<g, d>, |d|^2 and TV(x + t_j d) for all candidates t_j.
+// Pass 2 of the line search: Re<g, d>, |d|^2 (first pass) and TV(x + t_j d) for the
+// candidates j in [J0, J1).  The first pass covers t = 0 and the NT_A - 1 largest
+// steps, which is where Armijo almost always stops; the second pass covers the rest
+// and does work only for problems whose first pass found no step.  Each candidate's
+// sum is accumulated exactly as in a single pass over all candidates.
+template <int J0, int J1>
 __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, const float2* __restrict__ d,
                                                  const float2* __restrict__ g, TVDev tv, int64_t npix_p,
                                                  double ls_beta, const ProbState* __restrict__ st, double* part) {
+  constexpr bool FIRST = J0 == 0;
+  constexpr int NV = (FIRST ? 2 : 0) + (J1 - J0), V0 = FIRST ? 2 : 0;
   const int p = blockIdx.y;
-  double v[2 + NT];
+  double v[NV];
 #pragma unroll
-  for (int j = 0; j < 2 + NT; ++j) v[j] = 0.0;
+  for (int j = 0; j < NV; ++j) v[j] = 0.0;
   float tj[NT];
   tj[0] = 0.f;
   {
@@ -346,15 +355,17 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
 #pragma unroll
     for (int j = 1; j < NT; ++j) { tj[j] = (float)t; t *= ls_beta; }
   }
-  if (st[p].status == MRH_CG_ACTIVE) {
+  if (st[p].status == MRH_CG_ACTIVE && (FIRST || st[p].ls_pending)) {
     const int plane = tv.S0 * tv.S1;
     for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
       const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
       const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
       const int64_t n = (int64_t)p * npix_p + i;
-      const float2 dn = d[n], gn = g[n];
-      v[0] += (double)gn.x * dn.x + (double)gn.y * dn.y;
-      v[1] += (double)dn.x * dn.x + (double)dn.y * dn.y;
+      if (FIRST) {
+        const float2 dn = d[n], gn = g[n];
+        v[0] += (double)gn.x * dn.x + (double)gn.y * dn.y;
+        v[1] += (double)dn.x * dn.x + (double)dn.y * dn.y;
+      }
       for (int k = 0; k < tv.n; ++k) {
         const TVTermDev& tm = tv.t[k];
         if (tm.lam == 0.f || tm.axes == 0) continue;
@@ -369,7 +380,7 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
         }
         const float e2 = tm.eps * tm.eps;
 #pragma unroll
-        for (int j = 0; j < NT; ++j) {
+        for (int j = J0; j < J1; ++j) {
           const float t = tj[j];
           float e = 0.f;
           if (tm.iso) {
@@ -389,34 +400,43 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
               }
             }
           }
-          v[2 + j] += (double)(tm.lam * e);
+          v[V0 + j - J0] += (double)(tm.lam * e);
         }
       }
     }
   }
-  block_reduce_store<2 + NT>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
+  block_reduce_store<NV>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
 }
 
-// Finish the line-search reductions and select the step (one block per problem).
+__device__ __forceinline__ void ls_record(ProbState& s, mrh_cg_hist* hist, int max_hist, int P, int p) {
+  if (s.iter < max_hist) {
+    mrh_cg_hist& h = hist[(int64_t)s.iter * P + p];
+    h.f0 = s.f0; h.g0d = s.g0d; h.t = s.t; h.ft = s.ft;
+  }
+}
+
+// Finish the first pass's reductions and try its steps (one block per problem):
+// Armijo f(t) <= f(0) + c1 t <g, d> for t = 1, beta, ... in that order.
 __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* part_i, const double* part_k,
                                                   int nb_i, int nb_k, double c1, double ls_beta, int ls_max,
                                                   mrh_cg_hist* hist, int max_hist, int P, int* active) {
   const int p = blockIdx.x;
-  __shared__ double vals[3 + 2 + NT];
+  __shared__ double vals[3 + 2 + NT_A];
   for (int v = 0; v < 3; ++v) {
     double s = block_sum_partials(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, v);
     if (threadIdx.x == 0) vals[v] = s;
   }
-  for (int v = 0; v < 2 + NT; ++v) {
+  for (int v = 0; v < 2 + NT_A; ++v) {
     double s = block_sum_partials(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, v);
     if (threadIdx.x == 0) vals[3 + v] = s;
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
   ProbState& s = st[p];
+  s.ls_pending = 0;
   if (s.status != MRH_CG_ACTIVE) { s.t = 0.0; return; }
   s.a = vals[0]; s.b = vals[1]; s.c = vals[2]; s.g0d = vals[3]; s.dd = vals[4];
-  for (int j = 0; j < NT; ++j) s.tv[j] = vals[5 + j];
+  for (int j = 0; j < NT_A; ++j) s.tv[j] = vals[5 + j];
   const double f0 = 0.5 * s.a + s.tv[0];
   s.f0 = f0;
   s.t = 0.0;
@@ -427,17 +447,47 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
     return;
   }
   double t = 1.0;
-  bool ok = false;
-  for (int j = 1; j <= ls_max && j < NT; ++j) {
+  for (int j = 1; j <= ls_max && j < NT_A; ++j) {
     const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j];
-    if (ft <= f0 + c1 * t * s.g0d) { s.t = t; s.ft = ft; ok = true; break; }
+    if (ft <= f0 + c1 * t * s.g0d) {
+      s.t = t; s.ft = ft;
+      ls_record(s, hist, max_hist, P, p);
+      return;
+    }
     t *= ls_beta;
   }
-  if (!ok) { s.status = MRH_CG_LS_FAILED; s.t = 0.0; active[p] = 0; return; }
-  if (s.iter < max_hist) {
-    mrh_cg_hist& h = hist[(int64_t)s.iter * P + p];
-    h.f0 = f0; h.g0d = s.g0d; h.t = s.t; h.ft = s.ft;
+  if (ls_max >= NT_A) { s.ls_pending = 1; return; }
+  s.status = MRH_CG_LS_FAILED; s.t = 0.0; active[p] = 0;
+}
+
+// Second pass: the remaining steps t = beta^(NT_A - 1), ..., for problems still pending.
+__global__ __launch_bounds__(RT) void k_ls_select_rest(ProbState* st, const double* part_i, int nb_i, double c1,
+                                                       double ls_beta, int ls_max, mrh_cg_hist* hist,
+                                                       int max_hist, int P, int* active) {
+  const int p = blockIdx.x;
+  ProbState& s = st[p];
+  if (!(s.status == MRH_CG_ACTIVE && s.ls_pending)) return;   // uniform over the block
+  __shared__ double vals[NT - NT_A];
+  for (int v = 0; v < NT - NT_A; ++v) {
+    double sum = block_sum_partials(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, v);
+    if (threadIdx.x == 0) vals[v] = sum;
   }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  s.ls_pending = 0;
+  for (int j = NT_A; j < NT; ++j) s.tv[j] = vals[j - NT_A];
+  double t = 1.0;
+  for (int j = 1; j < NT_A; ++j) t *= ls_beta;
+  for (int j = NT_A; j <= ls_max && j < NT; ++j) {
+    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j];
+    if (ft <= s.f0 + c1 * t * s.g0d) {
+      s.t = t; s.ft = ft;
+      ls_record(s, hist, max_hist, P, p);
+      return;
+    }
+    t *= ls_beta;
+  }
+  s.status = MRH_CG_LS_FAILED; s.t = 0.0; active[p] = 0;
 }
 
 // x += t d (image) for every problem.
@@ -661,11 +711,17 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
   hipLaunchKernelGGL(k_ls_kspace, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
                      (const float2*)y, e->d_kprob, e->d_st, e->d_part_k);
   MRH_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ls_image, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (const float2*)d,
-                     (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
+  hipLaunchKernelGGL((k_ls_image<0, NT_A>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
                      e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL((k_ls_image<NT_A, NT>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_ls_select_rest, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, e->c1,
+                     e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

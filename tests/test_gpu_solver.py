@@ -142,6 +142,22 @@ def test_nlcg_dynamic_temporal_spatial_tv(direction):
     assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
 
 
+def test_small_steps_take_the_second_line_search_pass():
+    """Coil maps scaled by 8 (|A|^2 x 64): Armijo stops near t = 2^-6, past the five steps of the
+    first image pass, so the second pass selects; steps and objectives as in the oracle run."""
+    wl = small_dynamic(seed=61)
+    wl.maps = wl.maps * 8.0
+    eng, hist, ref, f_gpu = run_pair(wl, 8)
+    steps = [h["step"] for h in hist]
+    print(f"[small steps] {steps}")
+    assert [h["step"] for h in ref.history] == steps
+    assert min(steps) <= 0.5 ** 5
+    fh = np.array([h["f"] for h in hist])
+    fr = np.array([h["f"] for h in ref.history])
+    assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
+    assert abs(f_gpu - ref.f_final) <= 1e-4 * abs(ref.f_final)
+
+
 def test_graph_replay_equals_eager():
     wl = small_dynamic(seed=31)
     _, h_graph, _, f_graph = run_pair(wl, 8, use_graph=True)
