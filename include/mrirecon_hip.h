@@ -298,6 +298,15 @@ int mrh_unet_up2_concat(const void* src, const void* skip, void* dst, int nf, in
                         int C2, void* stream);
 int mrh_unet_up2_concat_bwd(const void* gdst, void* gsrc, void* gskip, int nf, int H, int W, int C1,
                             int C2, void* stream);
+/* 3x3 convolution of the concatenation above without storing it: out = conv_fwd of
+ * concat(up2(src), skip) with Cin = C1 + C2 (same weights layout, bias and ReLU as
+ * mrh_unet_conv_fwd, identical result).  For inference; MRH_EUNSUP unless
+ * mrh_unet_upcat_supported(H, W, C1, C2) returns 1 (C1, C2 multiples of 32 and an
+ * image that the 2-D tiles cover). */
+int mrh_unet_conv3x3_upcat(const void* src, const void* skip, const void* w, const float* bias, void* out,
+                           int nvol, int F, int H, int W, int C1, int C2, int Cout, int Cout_pad, int relu,
+                           void* stream);
+int mrh_unet_upcat_supported(int H, int W, int C1, int C2);
 /* complex64 image (npix) * scale -> BF16 (npix, C) channels (re, im, 0, ...). */
 int mrh_unet_pack_input(const void* img, void* out, int64_t npix, int C, float scale, void* stream);
 /* xout = xin + (net[:, 0] + i net[:, 1]) * inv_scale (complex64; net is BF16 (npix, Cn)). */

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -98,6 +98,8 @@ _SIGS = {
     "mrh_unet_pool2": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
     "mrh_unet_pool2_bwd": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_void_p]),
     "mrh_unet_up2_concat": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 5 + [c_void_p]),
+    "mrh_unet_conv3x3_upcat": (c_int, [c_void_p] * 5 + [c_int] * 9 + [c_void_p]),
+    "mrh_unet_upcat_supported": (c_int, [c_int] * 4),
     "mrh_unet_up2_concat_bwd": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 5 + [c_void_p]),
     "mrh_unet_pack_input": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),
     "mrh_unet_unpack_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),

@@ -305,10 +305,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const u16* __restrict__ x,
 // [64 w, 64 w + 64) as four 16-pixel MFMA columns.
 // Needs Cin % 32 == 0 and, for 3x3, W % TW == 0 and H % TR == 0 (temporal:
 // H W % 256 == 0).
-template <int MT, int TW, bool TEMP>
+// UPC (3x3 only): the input is the decoder's concatenation [up2(x), x2] read in
+// place -- channels [0, C1) from the half-resolution x (nearest neighbour),
+// channels [C1, Cin) from the full-resolution skip x2 -- so it is never stored
+// (C1 % 32 == 0: a chunk never straddles the two).
+template <int MT, int TW, bool TEMP, bool UPC = false>
 __global__ __launch_bounds__(256, 2) void k_conv_tile(const u16* __restrict__ x, const u16* __restrict__ w,
                                                      const float* __restrict__ bias, u16* __restrict__ out, int F,
-                                                     int H, int W, int Cin, int Cout, int Cout_pad, int relu) {
+                                                     int H, int W, int Cin, int Cout, int Cout_pad, int relu,
+                                                     const u16* __restrict__ x2 = nullptr, int C1 = 0) {
   constexpr int TR = TEMP ? 1 : 256 / TW, HC = TEMP ? 256 : TW + 2, NHP = (TR + 2) * HC;
   constexpr int NTAP = TEMP ? 3 : 9, KC = 32, PS = KC + 8, BC = 16 * MT;
   constexpr int NIN = NHP * 4, IPT = (NIN + 255) / 256;        // 16-byte pieces of the halo, per thread
@@ -348,9 +353,16 @@ __global__ __launch_bounds__(256, 2) void k_conv_tile(const u16* __restrict__ x,
           if ((unsigned)(fv + hy - 1) < (unsigned)F) src = ibase + (long long)(hy - 1) * HW + p0 + hx;
         } else {
           const int gy = y0 + hy - 1, gx = x0 + hx - 1;
-          if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) src = ibase + (long long)gy * W + gx;
+          if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) {
+            src = ibase + (long long)gy * W + gx;
+            if (UPC && ci0 < C1) src = ((long long)img * (H / 2) + gy / 2) * (W / 2) + gx / 2;
+          }
         }
-        if (src >= 0) pin[i] = *reinterpret_cast<const u32x4*>(x + src * Cin + ci0 + 8 * q);
+        if (src >= 0) {
+          const u16* base = x + src * Cin + ci0;
+          if (UPC) base = ci0 < C1 ? x + src * C1 + ci0 : x2 + src * (Cin - C1) + (ci0 - C1);
+          pin[i] = *reinterpret_cast<const u32x4*>(base + 8 * q);
+        }
       }
     }
   };
@@ -813,6 +825,43 @@ extern "C" int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias
     MRH_LDS_CONV(4, 2, 16);
   }
 #undef MRH_LDS_CONV
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+static bool upcat_tiles(int H, int W, int C1, int C2) {
+  const int tw = W >= 64 && W % 64 == 0 ? 64 : (W == 32 ? 32 : 0);
+  return tw && H % (256 / tw) == 0 && H % 2 == 0 && W % 2 == 0 && C1 % 32 == 0 && C2 % 32 == 0 && C1 > 0 &&
+         C2 > 0;
+}
+
+extern "C" int mrh_unet_upcat_supported(int H, int W, int C1, int C2) { return upcat_tiles(H, W, C1, C2) ? 1 : 0; }
+
+extern "C" int mrh_unet_conv3x3_upcat(const void* s, const void* skip, const void* w, const float* bias, void* out,
+                                      int nvol, int F, int H, int W, int C1, int C2, int Cout, int Cout_pad,
+                                      int relu, void* stream) {
+  MRH_REQUIRE(s && skip && w && out, MRH_EINVAL, "unet_conv3x3_upcat: null buffer");
+  TapGeom g;
+  MRH_TRY(make_geom(MRH_UNET_TAP_3X3, F, H, W, &g));
+  MRH_REQUIRE(upcat_tiles(H, W, C1, C2), MRH_EUNSUP,
+              "unet_conv3x3_upcat: %dx%d image with %d + %d channels has no tile (mrh_unet_upcat_supported)", H, W,
+              C1, C2);
+  MRH_REQUIRE(Cout >= 1 && Cout <= Cout_pad && Cout_pad % 32 == 0, MRH_EINVAL,
+              "unet_conv3x3_upcat: need 1 <= Cout=%d <= Cout_pad=%d, Cout_pad %% 32 == 0", Cout, Cout_pad);
+  const int64_t npix = (int64_t)nvol * F * H * W;
+  MRH_REQUIRE(nvol >= 1 && npix < (1ll << 31) - 256, MRH_EINVAL, "unet_conv3x3_upcat: %lld pixels out of range",
+              (long long)npix);
+  hipStream_t st = (hipStream_t)stream;
+  const u16 *xs = (const u16*)s, *x2 = (const u16*)skip, *ws = (const u16*)w;
+  u16* os = (u16*)out;
+  const int tw = W >= 64 ? 64 : 32, mt = Cout_pad % 64 == 0 ? 4 : 2, Cin = C1 + C2;
+  dim3 grid((unsigned)(npix / 256), Cout_pad / (16 * mt));
+#define MRH_UPC_CONV(MT, TW)                                                                                   \
+  hipLaunchKernelGGL((k_conv_tile<MT, TW, false, true>), grid, dim3(256), 0, st, xs, ws, bias, os, F, H, W, Cin, \
+                     Cout, Cout_pad, relu, x2, C1)
+  if (mt == 4) { if (tw == 64) MRH_UPC_CONV(4, 64); else MRH_UPC_CONV(4, 32); }
+  else { if (tw == 64) MRH_UPC_CONV(2, 64); else MRH_UPC_CONV(2, 32); }
+#undef MRH_UPC_CONV
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

@@ -242,7 +242,8 @@ class UNet2DT:
                 shapes[f"i{l}"] = (npx(l), self.ch[l - 1])
         for l in range(Lv - 1):
             shapes[f"s{l}"] = (npx(l + 1), self.ch[l])
-            shapes[f"u{l}"] = (npx(l), 2 * self.ch[l])
+            if train or not self._fused_upcat(l, H, W):
+                shapes[f"u{l}"] = (npx(l), 2 * self.ch[l])
             shapes[f"b{l}"] = (npx(l), self.ch[l])
             shapes[f"h{l}"] = (npx(l), self.ch[l])
         ws = {k: torch.empty(s, **bf) for k, s in shapes.items()}
@@ -275,7 +276,11 @@ class UNet2DT:
         return [1.0 / math.sqrt(max(s / per, 1e-30)) for s in ss]
 
     # ------------------------------------------------------------------ passes
-    def _forward(self, ws, nvol: int, F: int, H: int, W: int) -> None:
+    def _fused_upcat(self, l: int, H: int, W: int) -> bool:
+        """Inference reads the decoder concatenation of level l in place (no stored copy)."""
+        return bool(self.lib.mrh_unet_upcat_supported(H >> l, W >> l, self.ch[l], self.ch[l]))
+
+    def _forward(self, ws, nvol: int, F: int, H: int, W: int, train: bool = True) -> None:
         Lv, L = self.levels, self.layer
         geo = lambda l: (nvol, F, H >> l, W >> l)
         nf = nvol * F
@@ -289,10 +294,17 @@ class UNet2DT:
                 inp = ws[f"i{l + 1}"]
         for l in range(Lv - 2, -1, -1):
             self._conv(L[f"dec{l}_up"], ws[f"h{l + 1}"], ws[f"s{l}"], geo(l + 1))
-            _lib.check(self.lib.mrh_unet_up2_concat(ws[f"s{l}"].data_ptr(), ws[f"e{l}"].data_ptr(),
-                                                    ws[f"u{l}"].data_ptr(), nf, H >> l, W >> l, self.ch[l],
-                                                    self.ch[l], self._st()))
-            self._conv(L[f"dec{l}_s"], ws[f"u{l}"], ws[f"b{l}"], geo(l))
+            if not train and self._fused_upcat(l, H, W):
+                Ls = L[f"dec{l}_s"]
+                _lib.check(self.lib.mrh_unet_conv3x3_upcat(
+                    ws[f"s{l}"].data_ptr(), ws[f"e{l}"].data_ptr(), Ls.bufs["wf"].data_ptr(),
+                    Ls.bufs["b"].data_ptr(), ws[f"b{l}"].data_ptr(), nvol, F, H >> l, W >> l, self.ch[l],
+                    self.ch[l], Ls.cout, Ls.cout_pad, int(Ls.relu), self._st()))
+            else:
+                _lib.check(self.lib.mrh_unet_up2_concat(ws[f"s{l}"].data_ptr(), ws[f"e{l}"].data_ptr(),
+                                                        ws[f"u{l}"].data_ptr(), nf, H >> l, W >> l, self.ch[l],
+                                                        self.ch[l], self._st()))
+                self._conv(L[f"dec{l}_s"], ws[f"u{l}"], ws[f"b{l}"], geo(l))
             self._conv(L[f"dec{l}_t"], ws[f"b{l}"], ws[f"h{l}"], geo(l))
         self._conv(L["out"], ws["h0"], ws["out"], geo(0))
 
@@ -364,7 +376,7 @@ class UNet2DT:
             scales = self.volume_scales(x_adj, nvol, ws)
         out = ws["x_warm"] if out is None else out
         self._pack(ws, x_adj, scales, F)
-        self._forward(ws, nvol, F, H, W)
+        self._forward(ws, nvol, F, H, W, train=False)
         self._unpack(ws, x_adj, out, scales, F)
         return out
 

@@ -270,6 +270,55 @@ def test_training_reduces_loss_and_inference_is_deterministic(tmp_path):
     assert net2.step == net.step == 60
 
 
+@pytest.mark.parametrize("geom,c", [((1, 2, 8, 64), 32), ((2, 2, 16, 32), 64), ((1, 1, 8, 128), 128)])
+@pytest.mark.parametrize("cout", [32, 64])
+def test_fused_upsample_concat_conv_equals_the_stored_concatenation(geom, c, cout):
+    """The decoder's 3x3 convolution reading concat(up2(src), skip) in place gives the same bits as
+    storing the concatenation first; unsupported shapes are refused."""
+    lib = _lib.load()
+    nvol, F, H, W = geom
+    nf = nvol * F
+    assert lib.mrh_unet_upcat_supported(H, W, c, c) == 1
+    rng = np.random.default_rng(c + cout + W)
+    src = to_dev_bf16(rng.standard_normal((nf, H // 2, W // 2, c)))
+    skip = to_dev_bf16(rng.standard_normal((nf, H, W, c)))
+    cpad = (cout + 31) // 32 * 32
+    wd = to_dev_bf16(rng.standard_normal((9, cpad, 2 * c)) * np.sqrt(1.0 / (9 * c)))
+    bd = torch.from_numpy((rng.standard_normal(cpad) * 0.1).astype(np.float32)).to(DEV)
+    u = torch.empty((nf * H * W, 2 * c), dtype=torch.bfloat16, device=DEV)
+    _lib.check(lib.mrh_unet_up2_concat(src.data_ptr(), skip.data_ptr(), u.data_ptr(), nf, H, W, c, c, st()))
+    ref = torch.empty((nf * H * W, cout), dtype=torch.bfloat16, device=DEV)
+    _lib.check(lib.mrh_unet_conv_fwd(u.data_ptr(), wd.data_ptr(), bd.data_ptr(), ref.data_ptr(), ou.TAP_3X3,
+                                     nvol, F, H, W, 2 * c, cout, cpad, 1, st()))
+    out = torch.full_like(ref, 7.0)
+    _lib.check(lib.mrh_unet_conv3x3_upcat(src.data_ptr(), skip.data_ptr(), wd.data_ptr(), bd.data_ptr(),
+                                          out.data_ptr(), nvol, F, H, W, c, c, cout, cpad, 1, st()))
+    assert torch.equal(out, ref)
+    assert lib.mrh_unet_upcat_supported(18, 18, c, c) == 0 and lib.mrh_unet_upcat_supported(H, W, 48, c) == 0
+    rc = lib.mrh_unet_conv3x3_upcat(src.data_ptr(), skip.data_ptr(), wd.data_ptr(), bd.data_ptr(),
+                                    out.data_ptr(), nvol, F, H, W, 48, c, cout, cpad, 1, st())
+    with pytest.raises(NotImplementedError):
+        _lib.check(rc)
+
+
+def test_inference_reading_the_concatenation_in_place_equals_the_training_forward():
+    levels, nvol, F, H, W = 3, 1, 3, 64, 64       # levels 0 and 1 fused, level 2 (16 x 16) stored
+    net = UNet2DT(levels=levels, base=32, device=DEV, seed=5)
+    rng = np.random.default_rng(6)
+    for L in net.layer.values():          # random BF16 weights, a non-zero output layer included
+        L.bufs["wf"].copy_((torch.randn_like(L.bufs["wf"].float()) * 0.05).to(torch.bfloat16))
+    x_adj = (rng.standard_normal((nvol * F, H, W)) + 1j * rng.standard_normal((nvol * F, H, W))).astype(np.complex64)
+    xa = torch.from_numpy(x_adj).to(DEV)
+    a = net.infer(xa, scales=[1.0]).clone()
+    ws = net._workspace(nvol, F, H, W, train=True)
+    net._pack(ws, xa, [1.0], F)
+    net._forward(ws, nvol, F, H, W, train=True)
+    b = torch.empty_like(xa)
+    net._unpack(ws, xa, b, [1.0], F)
+    assert not torch.equal(a, xa)
+    assert torch.equal(a, b)
+
+
 def test_shape_errors():
     net = UNet2DT(levels=3, base=32, device=DEV)
     with pytest.raises(ValueError):
