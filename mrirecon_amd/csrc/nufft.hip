@@ -576,7 +576,44 @@ __device__ __forceinline__ float from_fixed(unsigned long long u, int sc) {
   return (float)ldexp((double)q, -sc);
 }
 
-// vs: the weighted sample already scaled by 2^sc.
+// Inverse length-N FFT of one column held as fixed-point (re, im) pairs at words
+// 2 PIDX(row), 2 PIDX(row) + 1 of acc, by one wavefront: the first Stockham pass
+// converts while it loads, writes its floats to buf[PIDX(.)] (buf aliases the
+// first half of acc) and the remaining passes run in place.  The result equals
+// converting every row first and calling fft_lds_wave.
+template <int N>
+__device__ __forceinline__ void fft_wave_from_fixed(float2* buf, const unsigned long long* acc, const float2* tw,
+                                                    int lane, int sc) {
+  constexpr int R = fft_next_radix(N), NBF = N / R, NB = (NBF + 63) / 64;
+  fft_sync<true>();
+  float2 v[NB][R];
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int j = lane + q * 64;
+    if (NBF % 64 == 0 || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int w = 2 * PIDX(j + r * NBF);
+        v[q][r] = make_float2(from_fixed(acc[w], sc), from_fixed(acc[w + 1], sc));
+      }
+      DFT<R, true>::run(v[q]);
+    }
+  }
+  fft_sync<true>();
+#pragma unroll
+  for (int q = 0; q < NB; ++q) {
+    const int j = lane + q * 64;
+    if (NBF % 64 == 0 || j < NBF) {
+#pragma unroll
+      for (int r = 0; r < R; ++r) buf[PIDX(j * R + r)] = v[q][r];
+    }
+  }
+  fft_sync<true>();
+  FFTStages<N, R, true, 64, true>::run(buf, tw, lane);
+}
+
+// vs: the weighted sample already scaled by 2^sc.  Column j of the strip owns
+// words [2 j CS, 2 j CS + 2 CS) of acc, row rr's (re, im) at 2 PIDX(rr) + {0, 1}.
 template <int WM, int CS>
 __device__ __forceinline__ void spread_one(const Foot<WM>& F, float2 vs, unsigned long long* acc, int W, int c0,
                                            int G0, int G1) {
@@ -683,30 +720,18 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
       }
     }
     __syncthreads();                       // every contribution is in
-    // fixed point -> float by the wavefront that owns the column: column col's
-    // float view lies inside its own int64 region, so the conversion is local
+    // Inverse column FFT by the wavefront that owns the column; its first pass
+    // reads the fixed-point accumulators and writes floats into the column's own
+    // int64 region (float slot p is the region's word p).  The wavefront then
+    // zeroes every word that the read-out below does not visit.
     {
       unsigned long long* ac = acc + 2 * col * CS;
       float2* fc = fbuf + 2 * col * CS;
-      constexpr int NPC = (CS + 63) / 64;
-      float2 cv[NPC];
-#pragma unroll
-      for (int t = 0; t < NPC; ++t) {
-        const int p = lane + 64 * t;
-        cv[t] = p < CS ? make_float2(from_fixed(ac[2 * p], sc), from_fixed(ac[2 * p + 1], sc))
-                       : make_float2(0.f, 0.f);
-      }
-      fft_sync<true>();
-#pragma unroll
-      for (int t = 0; t < NPC; ++t) {
-        const int p = lane + 64 * t;
-        if (p < CS) {
-          fc[p] = cv[t];
-          ac[CS + p] = 0ull;
-        }
-      }
-      fft_lds_wave<G0, true>(fc, stw, lane);
+      fft_wave_from_fixed<G0>(fc, ac, stw, lane, sc);
+      for (int q = lane; q < CS; q += 64) ac[CS + q] = 0ull;
       for (int r = S0 - h0 + lane; r < G0 - h0; r += 64) fc[PIDX(r)] = make_float2(0.f, 0.f);
+      for (int q = lane; q < CS; q += 64)   // slots that hold no row: the padding holes and the tail
+        if (q % 33 == 32 || q > PIDX(G0 - 1)) fc[q] = make_float2(0.f, 0.f);
     }
     __syncthreads();                       // all columns transformed
     float2* Tb = Tbb + c * cstride;
