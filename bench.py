@@ -218,8 +218,12 @@ def nufft_rate(wl, dev, reps=5):
     ncf = wl.B * wl.n_coils
     return be, {"coil_frames": ncf, "ms_per_whole_pair": round(ms, 4),
                 "coil_frame_pairs_per_s": round(ncf / (ms * 1e-3), 1),
+                "strip_cols": int(plan.info()["strip_cols"]),
                 "coil_chunk": plan.tuning["coil_chunk"] if plan.tuning else None,
-                "autotune_ms_per_pair": ({str(k): round(v, 4) for k, v in plan.tuning["ms_per_pair"].items()}
+                "autotune_ms_per_pair": ({str(w): {str(k): round(v, 4) for k, v in t.items()}
+                                          for w, t in plan.tuning["ms_per_pair_by_width"].items()}
+                                         if plan.tuning and "ms_per_pair_by_width" in plan.tuning else
+                                         {str(k): round(v, 4) for k, v in plan.tuning["ms_per_pair"].items()}
                                          if plan.tuning else None)}
 
 

@@ -56,8 +56,7 @@ __device__ __forceinline__ float kb_weight(const KBPoly& kb, float d) {
 }
 
 #define MRH_WMAX 8
-#define STRIP_TC 8                     // grid columns per strip workgroup
-#define STRIP_THREADS (64 * STRIP_TC)  // one wavefront per column
+#define STRIP_TC 8                     // default grid columns per strip workgroup (4 and 16 also built)
 #define STRIP_EPT 1                    // sample footprints cached in registers per thread
 
 // ---------------------------------------------------------------- plan
@@ -149,7 +148,8 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   MRH_REQUIRE(d->k_offsets && d->traj, MRH_EINVAL, "k_offsets and traj are required (host pointers)");
   MRH_REQUIRE(d->maps, MRH_EINVAL, "maps (device pointer) is required");
   int Tc = d->strip_cols > 0 ? d->strip_cols : STRIP_TC;
-  MRH_REQUIRE(Tc == STRIP_TC, MRH_EUNSUP, "strip width must be %d (got %d)", STRIP_TC, Tc);
+  MRH_REQUIRE(Tc == 4 || Tc == 8 || Tc == 16, MRH_EUNSUP, "strip width must be 4, 8 or 16 (got %d)", Tc);
+  MRH_REQUIRE(Tc != 16 || d->G0 <= 512, MRH_EUNSUP, "16-column strips need G0 <= 512 (LDS), got %d", d->G0);
   MRH_REQUIRE(d->G1 % Tc == 0, MRH_EINVAL, "G1=%d not a multiple of the strip width %d", d->G1, Tc);
   MRH_REQUIRE(Tc >= d->W, MRH_EINVAL, "strip width %d must be >= KB width %d", Tc, d->W);
   MRH_REQUIRE(d->G0 < 65536 && d->G1 < 65536, MRH_EINVAL, "grid too large");
@@ -179,6 +179,11 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   while (np < MRH_KB_MAXNP && coef[np] * std::pow(q, np) > 1e-10) ++np;
   MRH_REQUIRE(coef[np] * std::pow(q, np) < 1e-9, MRH_EUNSUP, "KB beta %.3f too large for the series", beta);
   p->np = np;
+  if (Tc == 4 && !(d->W == 4 && np <= 20)) {
+    free_plan(p);
+    mrh_set_error("4-column strips need W = 4 with at most 20 KB series terms (W=%d, %d terms)", d->W, np);
+    return MRH_EUNSUP;
+  }
   p->kb.qscale = (float)q;
   p->kb.two_over_w = (float)(2.0 / d->W);
   for (int n = 0; n < MRH_KB_MAXNP; ++n) p->kb.a[n] = n < np ? (float)coef[n] : 0.f;
@@ -400,7 +405,7 @@ __global__ __launch_bounds__(256) void k_fwd_rows(
 }
 
 // ---------------------------------------------------------------- strip kernels
-// One workgroup per (batch frame, strip of STRIP_TC grid columns); it loops
+// One workgroup per (batch frame, strip of TC grid columns); it loops
 // over the coils of the frame.  Each thread evaluates the KB footprint of up to
 // STRIP_EPT of the strip's samples once and keeps it in registers, so the
 // weights are shared by all coils; samples beyond that spill to an
@@ -417,7 +422,7 @@ struct Foot {
 // the remaining coefficients are zero, so the value equals the full series.
 __host__ __device__ constexpr int kb_terms(int wm) { return wm == 4 ? 20 : wm == 6 ? 24 : MRH_KB_MAXNP; }
 
-template <int WM>
+template <int WM, int TC>
 __device__ __forceinline__ void load_foot(Foot<WM>& F, int2 m, float4 f, int W, int c0, int G1,
                                           const KBPoly& kb) {
   F.r0 = m.x & 0xffff;
@@ -431,7 +436,7 @@ __device__ __forceinline__ void load_foot(Foot<WM>& F, int2 m, float4 f, int W, 
   for (int j = 0; j < WM; ++j) {
     int cc = F.my + j;
     if (cc >= G1) cc -= G1;
-    F.wy[j] = (j < W && (unsigned)(cc - c0) < (unsigned)STRIP_TC) ? kb_weight<kb_terms(WM)>(kb, f.y - (float)j) : 0.f;
+    F.wy[j] = (j < W && (unsigned)(cc - c0) < (unsigned)TC) ? kb_weight<kb_terms(WM)>(kb, f.y - (float)j) : 0.f;
   }
 }
 
@@ -474,14 +479,14 @@ __device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* sm
   return acc;
 }
 
-// Forward strip pass.  Grid (nstrips, nbatch), STRIP_THREADS threads (one wavefront per column).
-template <int G0, int WT>
-__global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
+// Forward strip pass.  Grid (nstrips, nbatch, coil chunks), 64 TC threads (one wavefront per column).
+template <int G0, int WT, int TC>
+__global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
     const float2* __restrict__ T, float2* __restrict__ y, const int2* __restrict__ em,
     const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
     const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk,
     FrameMask fm) {
-  constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
+  constexpr int NTH = 64 * TC, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
   if (frame_skipped(fm, blockIdx.y)) return;
@@ -500,7 +505,7 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
 #pragma unroll
   for (int q = 0; q < STRIP_EPT; ++q) {
     const int e = e0 + tid + q * NTH;
-    if (e < e1) load_foot<WM>(F[q], em[e], ef[e], W, c0, G1, kb);
+    if (e < e1) load_foot<WM, TC>(F[q], em[e], ef[e], W, c0, G1, kb);
     else F[q].k = -1;
   }
   const int h0 = S0 / 2;
@@ -551,7 +556,7 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_fwd_stri
     }
     for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
       Foot<WM> Fo;
-      load_foot<WM>(Fo, em[e], ef[e], W, c0, G1, kb);
+      load_foot<WM, TC>(Fo, em[e], ef[e], W, c0, G1, kb);
       const float2 a = interp_one<WM, CS>(Fo, smem, W, c0, G0, G1);
       unsafeAtomicAdd(&yc[Fo.k].x, a.x * Fo.wA);
       unsafeAtomicAdd(&yc[Fo.k].y, a.y * Fo.wA);
@@ -639,14 +644,14 @@ __device__ __forceinline__ void spread_one(const Foot<WM>& F, float2 vs, unsigne
 }
 
 // Adjoint strip pass: fixed-point spreading into the LDS strip, inverse column
-// FFT, crop to the S0 image rows.  Grid (nstrips, nbatch), STRIP_THREADS threads.
-template <int G0, int WT>
-__global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
+// FFT, crop to the S0 image rows.  Grid (nstrips, nbatch, coil chunks), 64 TC threads.
+template <int G0, int WT, int TC>
+__global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
     const float2* __restrict__ y, float2* __restrict__ T, const int2* __restrict__ em,
     const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
     const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk,
     FrameMask fm) {
-  constexpr int TC = STRIP_TC, NTH = STRIP_THREADS, CS = fft_padded(G0) + 1;
+  constexpr int NTH = 64 * TC, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
   if (frame_skipped(fm, blockIdx.y)) return;
@@ -667,7 +672,7 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
 #pragma unroll
   for (int q = 0; q < STRIP_EPT; ++q) {
     const int e = e0 + tid + q * NTH;
-    if (e < e1) load_foot<WM>(F[q], em[e], ef[e], W, c0, G1, kb);
+    if (e < e1) load_foot<WM, TC>(F[q], em[e], ef[e], W, c0, G1, kb);
     else F[q].k = -1;
   }
   const int h0 = S0 / 2;
@@ -714,7 +719,7 @@ __global__ __launch_bounds__(STRIP_THREADS, (G0 <= 512 ? 4 : 2)) void k_adj_stri
           spread_one<WM, CS>(F[q], make_float2(ldexpf(v[q].x, sc), ldexpf(v[q].y, sc)), acc, W, c0, G0, G1);
       for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
         Foot<WM> Fo;
-        load_foot<WM>(Fo, em[e], ef[e], W, c0, G1, kb);
+        load_foot<WM, TC>(Fo, em[e], ef[e], W, c0, G1, kb);
         const float2 vo = cscale(yc[Fo.k], Fo.wAH);
         spread_one<WM, CS>(Fo, make_float2(ldexpf(vo.x, sc), ldexpf(vo.y, sc)), acc, W, c0, G0, G1);
       }
@@ -836,18 +841,35 @@ static int launch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, FrameMask
   return MRH_OK;
 }
 
-template <int G, int WT>
-static int launch_strips(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
-                         int nb, int like, FrameMask fm, hipStream_t st) {
-  dim3 grid(p->nstrips, nb, (p->C + p->cchunk - 1) / p->cchunk), block(STRIP_THREADS);
+template <int G, int WT, int TC>
+static int launch_strips_tc(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
+                            int nb, int like, FrameMask fm, hipStream_t st) {
+  dim3 grid(p->nstrips, nb, (p->C + p->cchunk - 1) / p->cchunk), block(64 * TC);
   if (adjoint)
-    hipLaunchKernelGGL((k_adj_strips<G, WT>), grid, block, 0, st, yin, p->d_T, p->d_em, p->d_ef,
+    hipLaunchKernelGGL((k_adj_strips<G, WT, TC>), grid, block, 0, st, yin, p->d_T, p->d_em, p->d_ef,
                        p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk, fm);
   else
-    hipLaunchKernelGGL((k_fwd_strips<G, WT>), grid, block, 0, st, p->d_T, yout, p->d_em, p->d_ef,
+    hipLaunchKernelGGL((k_fwd_strips<G, WT, TC>), grid, block, 0, st, p->d_T, yout, p->d_em, p->d_ef,
                        p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk, fm);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
+}
+
+// Strip widths built: 8 for every grid and width, 16 for G0 <= 512 (two 16-column
+// strips need more LDS than a CU has beyond that), 4 for the W = 4 kernels (a
+// footprint must span at most two strips).  plan_create admits only these.
+template <int G, int WT>
+static int launch_strips(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
+                         int nb, int like, FrameMask fm, hipStream_t st) {
+  if constexpr (G <= 512)
+    if (p->Tc == 16) return launch_strips_tc<G, WT, 16>(p, adjoint, yin, yout, nb, like, fm, st);
+  if constexpr (WT == 4)
+    if (p->Tc == 4) return launch_strips_tc<G, WT, 4>(p, adjoint, yin, yout, nb, like, fm, st);
+  if (p->Tc != 8) {
+    mrh_set_error("no %d-column strip kernels for G0=%d, W=%d", p->Tc, G, p->W);
+    return MRH_EUNSUP;
+  }
+  return launch_strips_tc<G, WT, 8>(p, adjoint, yin, yout, nb, like, fm, st);
 }
 
 #define MRH_FOR_EACH_G(X) X(32) X(64) X(96) X(128) X(192) X(256) X(384) X(512) X(768) X(1024)

@@ -182,8 +182,8 @@ class HipNUFFTAdapter:
     apply_dcf_in_adj: bool = True
     kb_width: int = 6          # torchkb numpoints=6 (torchkb.py:139)
     osf: float = 2.0           # torchkb default 2x grid
-    strip_cols: int = 0        # 0 = chosen by the plan
-    autotune: bool = True      # time the strip kernels' work split per plan (NufftPlan.autotune)
+    strip_cols: int = 0        # 0 = chosen by the autotuner (or 8 with autotune=False)
+    autotune: bool = True      # time strip widths and coil chunks per plan (NufftPlan.autotune)
 
     _im_shape: Optional[Tuple[int, ...]] = None
     _maps: Optional[torch.Tensor] = None
@@ -272,18 +272,48 @@ class HipNUFFTAdapter:
         self._alpha_scalar = None
         self._plan(1)
 
+    def _new_plan(self, like: int, strip_cols: int) -> NufftPlan:
+        C, S0, S1 = self._im_shape
+        return NufftPlan(S=(S0, S1), G=(_grid_size(S0, self.osf), _grid_size(S1, self.osf)),
+                         W=self.kb_width, C=C, k_offsets=self._k_offsets, traj_flat=self._traj_flat,
+                         dcf_flat=self._dcf_flat, dcf_mode=self.dcf_mode,
+                         dcf_in_fwd=self.apply_dcf_in_fwd, dcf_in_adj=self.apply_dcf_in_adj,
+                         maps=self._maps, like=like, strip_cols=strip_cols)
+
     def _plan(self, like: int) -> NufftPlan:
+        """The plan for ``like`` images per frame.  With ``autotune`` and ``strip_cols = 0`` every
+        strip width the library builds for this grid and KB width (4, 8, 16 grid columns) gets a
+        plan whose coil chunk is tuned; the fastest plan is kept (``plan.tuning`` lists all times)."""
         p = self._plans.get(like)
-        if p is None:
-            C, S0, S1 = self._im_shape
-            p = NufftPlan(S=(S0, S1), G=(_grid_size(S0, self.osf), _grid_size(S1, self.osf)),
-                          W=self.kb_width, C=C, k_offsets=self._k_offsets, traj_flat=self._traj_flat,
-                          dcf_flat=self._dcf_flat, dcf_mode=self.dcf_mode,
-                          dcf_in_fwd=self.apply_dcf_in_fwd, dcf_in_adj=self.apply_dcf_in_adj,
-                          maps=self._maps, like=like, strip_cols=self.strip_cols)
-            if self.autotune:
-                p.autotune(like=like)
-            self._plans[like] = p
+        if p is not None:
+            return p
+        if not self.autotune:
+            p = self._new_plan(like, self.strip_cols)
+        elif self.strip_cols:
+            p = self._new_plan(like, self.strip_cols)
+            p.autotune(like=like)
+        else:
+            G0 = _grid_size(self._im_shape[1], self.osf)
+            widths = [8] + ([16] if G0 <= 512 else []) + ([4] if self.kb_width == 4 else [])
+            best, times = None, {}
+            for tc in widths:
+                try:
+                    q = self._new_plan(like, tc)
+                except NotImplementedError:      # a width this plan cannot use (e.g. too many KB terms)
+                    continue
+                t = q.autotune(like=like)
+                times[tc] = t
+                if best is None or min(t.values()) < min(best.tuning["ms_per_pair"].values()):
+                    if best is not None:
+                        best.close()
+                    best = q
+                else:
+                    q.close()
+            p = best
+            p.tuning = {"strip_cols": int(p.info()["strip_cols"]), "coil_chunk": p.tuning["coil_chunk"],
+                        "ms_per_pair": p.tuning["ms_per_pair"],
+                        "ms_per_pair_by_width": {tc: dict(t) for tc, t in times.items()}}
+        self._plans[like] = p
         return p
 
     @property

@@ -77,7 +77,7 @@ def test_reference_precision_problem():
     assert relA_o <= TOL and relAH_o <= TOL
 
 
-@pytest.mark.parametrize("strip_cols", [0, 8])
+@pytest.mark.parametrize("strip_cols", [0, 4, 8, 16])
 def test_config1_forward_adjoint_parity(strip_cols):
     wl = synth.config_c1()
     be = adapter_for(wl, strip_cols)
@@ -142,15 +142,18 @@ def test_config2_full_size_frames_and_adjointness():
         assert e_f <= TOL and e_a <= TOL
 
 
-def test_ragged_frames_parity():
-    """Uneven spokes per frame (the batched workload's layout) at a small size."""
+@pytest.mark.parametrize("strip_cols", [8, 16])
+def test_ragged_frames_parity(strip_cols):
+    """Uneven spokes per frame (the batched workload's layout) at a small size, both strip widths
+    of W = 6."""
     S, C, R = 64, 3, 128
     spokes = [13, 55, 21, 34, 17]
     traj, dcf, off = synth.radial_frames(spokes, R)
     maps = synth.gaussian_maps(C, S)
     wl = synth.Workload("ragged", S, 1, len(spokes), C, R, 6, 2.0, spokes, maps, traj, dcf, off,
                         crand((len(spokes), S, S), 9))
-    be = adapter_for(wl)
+    be = adapter_for(wl, strip_cols)
+    assert be.plan.info()["strip_cols"] == strip_cols
     ora = oracle_for(wl)
     y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
     be.plan.forward(c64(wl.x_true), y)
@@ -160,6 +163,22 @@ def test_ragged_frames_parity():
     torch.cuda.synchronize()
     assert rel_l2(y.cpu().numpy(), ora.forward(wl.x_true)) <= TOL
     assert rel_l2(xa.cpu().numpy(), ora.adjoint(yr)) <= TOL
+
+
+def test_strip_width_autotune_and_unbuilt_widths():
+    """The adapter times every strip width built for the plan (W = 6, G = 512: 8 and 16) and keeps
+    the fastest; widths the kernels do not cover are refused at plan creation."""
+    wl = synth.config_c2()
+    be = adapter_for(wl)
+    t = be.plan.tuning
+    assert set(t["ms_per_pair_by_width"]) == {8, 16} and t["strip_cols"] in (8, 16)
+    assert be.plan.info()["strip_cols"] == t["strip_cols"]
+    best = min(min(v.values()) for v in t["ms_per_pair_by_width"].values())
+    assert min(t["ms_per_pair"].values()) == best
+    with pytest.raises(ValueError):
+        adapter_for(wl, strip_cols=4)            # W = 6 footprints would span three 4-column strips
+    with pytest.raises(NotImplementedError):
+        adapter_for(synth.config_c3(), strip_cols=16)   # G0 = 768: two 16-column strips exceed the LDS
 
 
 def test_edge_of_kspace_and_empty_frame():
