@@ -54,7 +54,8 @@ typedef struct {
   int dcf_in_fwd;           /* 'standard' mode: multiply by dcf in A                   */
   int dcf_in_adj;           /* 'standard' mode: multiply by dcf in A^H                 */
   const void* maps;         /* device complex64 (C, S0, S1); copied into the plan      */
-  int strip_cols;           /* grid columns per strip workgroup: 8, 16 or 0 (auto)     */
+  int strip_cols;           /* grid columns per strip workgroup: 0 (= 8), 8, 16 (G0 <= 512)
+                               or 4 (W = 4); the Python adapter autotunes it            */
 } mrh_nufft_desc;
 
 typedef struct {
