@@ -60,6 +60,39 @@ __device__ double block_sum_partials(const double* part, int n, int stride, int 
   return r;
 }
 
+// Sums of values [v0, v0 + NV) over n block partials (stride doubles apart) in one
+// sweep: each thread adds its partials for every value, then one block reduction.
+// Thread 0 receives the sums in out.
+template <int NV>
+__device__ void block_sum_partials_n(const double* part, int n, int stride, int v0, double* out) {
+  __shared__ double sh[NV][RT / 64];
+  double s[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  for (int i = threadIdx.x; i < n; i += RT) {
+    const double* q = part + (int64_t)i * stride + v0;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) s[k] += q[k];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    const double w = warp_sum_d(s[k]);
+    if (lane == 0) sh[k][wid] = w;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      double t = 0.0;
+#pragma unroll
+      for (int w = 0; w < RT / 64; ++w) t += sh[k][w];
+      out[k] = t;
+    }
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(RT) void k_finish_sum(const double* part, int n, double* out) {
   double s = block_sum_partials(part, n, 1, 0);
   if (threadIdx.x == 0) out[0] = s;
@@ -345,6 +378,7 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
   constexpr bool FIRST = J0 == 0;
   constexpr int NV = (FIRST ? 2 : 0) + (J1 - J0), V0 = FIRST ? 2 : 0;
   const int p = blockIdx.y;
+  if (!FIRST && !(st[p].status == MRH_CG_ACTIVE && st[p].ls_pending)) return;   // partials unread: skip
   double v[NV];
 #pragma unroll
   for (int j = 0; j < NV; ++j) v[j] = 0.0;
@@ -421,16 +455,9 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
                                                   int nb_i, int nb_k, double c1, double ls_beta, int ls_max,
                                                   mrh_cg_hist* hist, int max_hist, int P, int* active) {
   const int p = blockIdx.x;
-  __shared__ double vals[3 + 2 + NT_A];
-  for (int v = 0; v < 3; ++v) {
-    double s = block_sum_partials(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, v);
-    if (threadIdx.x == 0) vals[v] = s;
-  }
-  for (int v = 0; v < 2 + NT_A; ++v) {
-    double s = block_sum_partials(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, v);
-    if (threadIdx.x == 0) vals[3 + v] = s;
-  }
-  __syncthreads();
+  double vals[3 + 2 + NT_A];
+  block_sum_partials_n<3>(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, 0, vals);
+  block_sum_partials_n<2 + NT_A>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 3);
   if (threadIdx.x != 0) return;
   ProbState& s = st[p];
   s.ls_pending = 0;
@@ -467,12 +494,8 @@ __global__ __launch_bounds__(RT) void k_ls_select_rest(ProbState* st, const doub
   const int p = blockIdx.x;
   ProbState& s = st[p];
   if (!(s.status == MRH_CG_ACTIVE && s.ls_pending)) return;   // uniform over the block
-  __shared__ double vals[NT - NT_A];
-  for (int v = 0; v < NT - NT_A; ++v) {
-    double sum = block_sum_partials(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, v);
-    if (threadIdx.x == 0) vals[v] = sum;
-  }
-  __syncthreads();
+  double vals[NT - NT_A];
+  block_sum_partials_n<NT - NT_A>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals);
   if (threadIdx.x != 0) return;
   s.ls_pending = 0;
   for (int j = NT_A; j < NT; ++j) s.tv[j] = vals[j - NT_A];
@@ -560,12 +583,8 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
                                                 const double* __restrict__ target, mrh_cg_hist* hist,
                                                 int max_hist, int P, int* active) {
   const int p = blockIdx.x;
-  __shared__ double vals[5];
-  for (int v = 0; v < 5; ++v) {
-    double s = block_sum_partials(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, v);
-    if (threadIdx.x == 0) vals[v] = s;
-  }
-  __syncthreads();
+  double vals[5];
+  block_sum_partials_n<5>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals);
   if (threadIdx.x != 0) return;
   ProbState& s = st[p];
   if (!prime && s.status != MRH_CG_ACTIVE) return;
@@ -623,9 +642,12 @@ __global__ __launch_bounds__(RT) void k_cg_direction(float2* __restrict__ d, con
 }
 
 // ---------------------------------------------------------------- engine C ABI
+// Blocks per problem for the grid-stride image / k-space kernels: about two elements per
+// thread, at most 2048 blocks (8 per CU on 256 CUs, so a problem's sweep keeps enough
+// wavefronts in flight to hide its loads); each block leaves one FP64 partial per value.
 static int blocks_for(int64_t n) {
-  int64_t b = (n + RT * 8 - 1) / (RT * 8);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(512, b));
+  int64_t b = (n + RT * 2 - 1) / (RT * 2);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(2048, b));
 }
 
 extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
