@@ -19,7 +19,7 @@
 #include <vector>
 
 #define RT 256            // threads per reduction block
-#define RED_BLOCKS 1024   // blocks of the standalone reductions
+#define RED_BLOCKS 2048   // blocks of the standalone reductions (8 per CU)
 #define NT 21             // line-search candidates: t = 0, 1, beta, ..., beta^19
 #define NT_A 6            // candidates of the first image pass: t = 0, 1, ..., beta^4
 #define PI_STRIDE 32      // doubles per block partial (image pass)
@@ -101,7 +101,7 @@ __global__ __launch_bounds__(RT) void k_finish_sum(const double* part, int n, do
 extern "C" int64_t mrh_reduce_workspace_len(void) { return (int64_t)RED_BLOCKS * 64; }
 
 static int red_blocks(int64_t n) {
-  int64_t b = (n + RT * 4 - 1) / (RT * 4);
+  int64_t b = (n + RT * 2 - 1) / (RT * 2);
   return (int)std::max<int64_t>(1, std::min<int64_t>(RED_BLOCKS, b));
 }
 
