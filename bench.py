@@ -48,8 +48,9 @@ def kernel_bytes(wl, plan_info):
     }
 
 
-def survey_pair_bytes(wl, G):
-    """Per-pair algorithmic bytes of the SURVEY's pipeline model (includes a G^2 grid round trip)."""
+def grid_pipeline_pair_bytes(wl, G):
+    """Per-pair algorithmic bytes of a gridding pipeline that stores the oversampled G x G grid:
+    8 S^2 (1 + 1/C) + 16 S G + 16 G^2 + 8 K + 12 K / C per coil-frame and direction."""
     S, C = wl.S, wl.n_coils
     tot = 0
     for b in range(wl.B):
@@ -398,10 +399,10 @@ def main():
     per_kernel = {n: {"avg_ms": round(stage_ms[n], 4), "algorithmic_GBps": round(kb[n] / (stage_ms[n] * 1e-3) / 1e9, 1)}
                   for n in names}
     pair_ms = sum(stage_ms.values())
-    survey = survey_pair_bytes(wl, info["G1"])
-    pipeline = {"survey_model_bytes_per_pair": int(survey), "kernel_sum_ms": round(pair_ms, 4),
-                "survey_model_GBps": round(survey / (pair_ms * 1e-3) / 1e9, 1),
-                "survey_model_frac": round(survey / (pair_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+    model = grid_pipeline_pair_bytes(wl, info["G1"])
+    pipeline = {"grid_model_bytes_per_pair": int(model), "kernel_sum_ms": round(pair_ms, 4),
+                "grid_model_GBps": round(model / (pair_ms * 1e-3) / 1e9, 1),
+                "grid_model_frac": round(model / (pair_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
                 "whole_problem_pairs_per_s": round(1e3 / ms_per_step, 2)}
 
     ttt, unet = None, None
