@@ -169,6 +169,8 @@ def time_to_tolerance(args, dev):
            "network_and_fallback_check_ms_median": round(float(np.median([r["warm_start_ms"] for r in rows])), 3),
            "weights": "mrirecon_amd/weights/unet_grasp.npz", "per_volume": [
                {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows]}
+    if not args.no_batched_tol:
+        ttt["batched"] = batched_grasp(args, dev, net, seeds, phantoms)
     # U-Net forward alone: dense MFMA FLOPs over HIP-event time on the launch stream
     x = pb.x_adj
     for _ in range(3):
@@ -190,6 +192,85 @@ def time_to_tolerance(args, dev):
                      f"{sum(L.n_weight for L in net.layers)} weights (BF16, FP32 accumulation)",
             "train_steps": net.step}
     return ttt, unet
+
+
+def batched_grasp(args, dev, net, seeds, phantoms):
+    """The time-to-tolerance volumes as one batched problem set: P GRASP volumes (P x 34 frames,
+    each with the single-volume trajectory, maps and noise) in the same launches, each with its own
+    line search and stop; stopped volumes skip their NUFFT work.  Wall time from the adjoint to the
+    last volume's 1e-4 relative-change stop, plain and warm-started (one U-Net pass over all volumes,
+    per-volume fallback check)."""
+    import math as _m
+
+    import numpy as np
+    import torch
+
+    from mrirecon_amd import NLCGEngine, RegManager, TVND, TVParams, synth
+    from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+    from mrirecon_amd.train_warmstart import GRASP, spokes_for
+    from mrirecon_amd.warmstart import ObjectiveEval, warm_start
+
+    g = GRASP
+    P, F, S, C = len(seeds), g["F"], g["S"], g["C"]
+    print(f"bench: {P} GRASP volumes batched (plain vs warm start)", file=sys.stderr, flush=True)
+    traj, dcf, off = synth.radial_frames(spokes_for(g, 0), g["R"])
+    K = int(off[-1])
+    off_p = np.concatenate([np.asarray(off[:-1]) + v * K for v in range(P)] + [[P * K]]).astype(np.int64)
+    maps = synth.gaussian_maps(C, S).astype(np.complex64)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=g["W"], osf=2.0)
+    be.prepare((C, S, S), torch.from_numpy(np.tile(traj, (1, P))), torch.from_numpy(np.tile(dcf, P)),
+               torch.from_numpy(maps).to(dev), device=dev, k_offsets=off_p)
+    plan = be.plan
+    x = torch.from_numpy(np.concatenate(phantoms).astype(np.complex64)).to(dev)
+    y = torch.empty(plan.n_k, dtype=torch.complex64, device=dev)
+    plan.forward(x, y)
+    nk = C * K
+    for v, seed in enumerate(seeds):                       # the single-volume noise of each volume
+        yv = y[v * nk:(v + 1) * nk]
+        sigma = g["noise"] * float(yv.abs().max())
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(seed) + 7)
+        nz = torch.randn((2, nk), generator=gen, device=dev, dtype=torch.float32)
+        yv += torch.complex(nz[0], nz[1]) * (sigma / _m.sqrt(2.0))
+    regm = RegManager([TVND("tv_t", TVParams(weight=g["lam_t"], axes="temporal")),
+                       TVND("tv_s", TVParams(weight=g["lam_s"], axes="spatial"))])
+    eng = NLCGEngine(plan, y, problems=P, frames=F, regm=regm, direction="prplus", rel_tol=1e-4, restart_c=0.0,
+                     max_hist=args.tol_iters + 1)
+    xa = torch.empty_like(x)
+    plan.adjoint(y, xa)
+    eng.prime(xa)
+    eng.capture()
+
+    def solve(x0, Ax0=None):
+        eng.prime(x0, Ax0)
+        return eng.run(args.tol_iters, use_graph=True, check_every=1)
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.adjoint(y, xa)
+    solve(xa)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    _, it_p = eng.status()
+    obj = ObjectiveEval(plan, y, regm, problems=P, frames=F)
+    obj_adj = ObjectiveEval(plan, y, regm, problems=P, frames=F)
+    x0 = torch.empty_like(x)
+    net.infer(xa, nvol=P)                                  # workspace allocation outside the timing
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.adjoint(y, xa)
+    x0, acc, _, _ = warm_start(net, obj, obj_adj, xa, nvol=P, out=x0)
+    solve(x0, obj.Ax if acc.all() else None)
+    torch.cuda.synchronize()
+    t_warm = time.perf_counter() - t0
+    _, it_w = eng.status()
+    return {"volumes": P, "frames": P * F, "plain_seconds": round(t_plain, 4), "warm_seconds": round(t_warm, 4),
+            "speedup": round(t_plain / t_warm, 3), "plain_volumes_per_second": round(P / t_plain, 2),
+            "warm_volumes_per_second": round(P / t_warm, 2),
+            "plain_iterations_median": float(np.median(it_p)), "plain_iterations_max": int(it_p.max()),
+            "warm_iterations_median": float(np.median(it_w)), "warm_iterations_max": int(it_w.max()),
+            "fallback_rate": float(np.mean(~acc)), "strip_cols": int(plan.info()["strip_cols"]),
+            "coil_chunk": int(plan.info()["coil_chunk"])}
 
 
 def nufft_rate(wl, dev, reps=5):
@@ -318,6 +399,8 @@ def main():
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance NLCG run")
     ap.add_argument("--tol-iters", type=int, default=200)
     ap.add_argument("--tol-volumes", type=int, default=16, help="held-out volumes of the time-to-tolerance runs")
+    ap.add_argument("--no-batched-tol", action="store_true",
+                    help="skip the run of the same volumes as one batched problem set")
     ap.add_argument("--c4-slices", type=int, default=48, help="slices of the batched uneven workload")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched uneven workload")
     ap.add_argument("--no-configs", action="store_true", help="skip the c1 / c3 NUFFT rates")

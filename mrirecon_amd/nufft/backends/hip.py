@@ -299,8 +299,10 @@ class HipNUFFTAdapter:
             for tc in widths:
                 try:
                     q = self._new_plan(like, tc)
-                except NotImplementedError:      # a width this plan cannot use (e.g. too many KB terms)
-                    continue
+                except (NotImplementedError, ValueError):
+                    if tc == widths[0]:           # the default width: the plan itself is invalid
+                        raise
+                    continue                      # a width this grid cannot use (G1 % tc, KB terms)
                 t = q.autotune(like=like)
                 times[tc] = t
                 if best is None or min(t.values()) < min(best.tuning["ms_per_pair"].values()):
