@@ -171,6 +171,7 @@ typedef struct mrh_cg mrh_cg;
 
 #define MRH_CG_DIR_FR 0
 #define MRH_CG_DIR_PRPLUS 1
+#define MRH_CG_DIR_DY 2       /* Dai-Yuan: beta = |g|^2 / Re<d, g - g_prev> */
 
 #define MRH_CG_ACTIVE 0
 #define MRH_CG_LS_FAILED 1

@@ -19,7 +19,7 @@ ABI_VERSION = 5
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
-CG_DIR_FR, CG_DIR_PRPLUS = 0, 1
+CG_DIR_FR, CG_DIR_PRPLUS, CG_DIR_DY = 0, 1, 2
 CG_ACTIVE, CG_LS_FAILED, CG_NOT_DESCENT, CG_REL_TOL, CG_TARGET = 0, 1, 2, 3, 4
 
 

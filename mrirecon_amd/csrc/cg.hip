@@ -601,6 +601,9 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
   if (direction == MRH_CG_DIR_PRPLUS) {
     beta = (s.rho - s.gpg) / fmax(s.rho_prev, 1e-20);
     beta = fmax(beta, 0.0);
+  } else if (direction == MRH_CG_DIR_DY) {
+    // directions.py:111-139: Re<d, g> is this pass's gdot_step, Re<d, g_prev> the line search's g0d
+    beta = s.rho / fmax(s.gdot_step - s.g0d, 1e-20);
   } else {
     if (!s.have_rho_prev) { s.rho_prev = s.rho; s.have_rho_prev = 1; }  // first beta = 1 (directions.py:145-149)
     beta = s.rho / fmax(s.rho_prev, 1e-20);
@@ -655,8 +658,8 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   *out = nullptr;
   MRH_REQUIRE(d->ls_max_iter >= 1 && d->ls_max_iter <= NT - 1, MRH_EUNSUP,
               "ls_max_iter must be in [1, %d]", NT - 1);
-  MRH_REQUIRE(d->direction == MRH_CG_DIR_FR || d->direction == MRH_CG_DIR_PRPLUS, MRH_EINVAL,
-              "direction must be FR or PR+");
+  MRH_REQUIRE(d->direction == MRH_CG_DIR_FR || d->direction == MRH_CG_DIR_PRPLUS || d->direction == MRH_CG_DIR_DY,
+              MRH_EINVAL, "direction must be FR, PR+ or DY");
   MRH_REQUIRE(d->tv.P >= 1 && d->tv.P <= 65535, MRH_EINVAL, "P outside [1, 65535]");
   mrh_cg* e = new mrh_cg();
   std::memset(e, 0, sizeof(*e));

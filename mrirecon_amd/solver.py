@@ -37,6 +37,11 @@ from .regularization import RegManager
 _STATUS = {0: "active", 1: "ls_failed", 2: "not_descent", 3: "rel_tol", 4: "target"}
 
 
+# direction rules of the reference (numerics/directions.py): FR (:141-162, first beta = 1),
+# PR+ (:75-109) and Dai-Yuan (:111-139)
+_DIRECTIONS = {"fr": _lib.CG_DIR_FR, "prplus": _lib.CG_DIR_PRPLUS, "dy": _lib.CG_DIR_DY}
+
+
 class NLCGEngine:
     """Device NLCG for P problems x F frames (image (P*F, S0, S1), flat ragged k-space)."""
 
@@ -44,8 +49,8 @@ class NLCGEngine:
                  direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5, ls_max_iter: int = 20,
                  rel_tol: float = 0.0, max_hist: int = 256, restart_c: Optional[float] = None,
                  skip_stopped: bool = True):
-        if direction not in ("fr", "prplus"):
-            raise ValueError("direction must be 'fr' or 'prplus'")
+        if direction not in _DIRECTIONS:
+            raise ValueError(f"direction must be one of {sorted(_DIRECTIONS)}")
         self.plan = plan
         self.P, self.F = int(problems), int(frames)
         if self.P * self.F != plan.B:
@@ -71,7 +76,7 @@ class NLCGEngine:
         desc = _lib.CGDesc()
         desc.tv = (regm or RegManager([])).tv_desc(self.P, self.F, self.S0, self.S1)
         desc.k_prob = self.k_prob.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
-        desc.direction = _lib.CG_DIR_FR if direction == "fr" else _lib.CG_DIR_PRPLUS
+        desc.direction = _DIRECTIONS[direction]
         desc.c1, desc.ls_beta, desc.ls_max_iter = float(c1), float(ls_beta), int(ls_max_iter)
         desc.rel_tol, desc.max_hist = float(rel_tol or 0.0), int(max_hist)
         desc.restart_c = -1.0 if restart_c is None else float(restart_c)

@@ -19,15 +19,17 @@ included, on any operator exposing ``forward`` / ``adjoint`` over
 * CG loop: prime d = -g (graspcg/solvers/cg.py:178-190), accept x += t d
   (:205-207), stats norms (:213-220), the reference convergence test
   (graspcg/solvers/utils.py:40-43), direction update FR with the first beta = 1
-  quirk (graspcg/numerics/directions.py:141-162) or PR+ (:75-109), D = 1
-  (cg.py:95).
+  quirk (graspcg/numerics/directions.py:141-162), PR+ (:75-109) or Dai-Yuan
+  (:111-139), D = 1 (cg.py:95).
 
 Pinning (tests/test_oracle_nlcg.py): closed-form quadratic with a unitary
 operator (graspcg/tests/test_numerics_ns.py:110-194), rejection of an ascent
 direction (:229-248), the Armijo inequality and monotone history
 (graspcg/tests/test_cg_descend_compile_ns.py:75-123), the Richardson
 finite-difference slope (:180-229), TV of a constant, the fwd/bwd adjoint
-identity and the hand-computable step-edge energy.
+identity and the hand-computable step-edge energy; Dai-Yuan by its identity
+Re<g_k, d_k> = beta_k Re<g_(k-1), d_(k-1)>, which no other beta rule satisfies
+in general.
 """
 from __future__ import annotations
 
@@ -240,12 +242,19 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
             den = rdt.type(np.vdot(g_prev.reshape(-1), g_prev.reshape(-1)).real)
             beta = max(num / max(den, rdt.type(1e-20)), rdt.type(0.0))
             g_prev = g.copy()
+        elif direction == "dy":
+            # beta_DY = <g, g> / Re<d, g - g_prev> (directions.py:120-130)
+            den = (rdt.type(np.vdot(d.reshape(-1), g.reshape(-1)).real)
+                   - rdt.type(np.vdot(d.reshape(-1), g_prev.reshape(-1)).real))
+            beta = rho / max(den, rdt.type(1e-20))
+            g_prev = g.copy()
         else:
             raise ValueError(direction)
         if restart_c is not None:
             gd_old = rdt.type(np.vdot(g.reshape(-1), d.reshape(-1)).real)
             if -rho + beta * gd_old > -restart_c * rho:
                 beta = rdt.type(0.0)
+        res.history[-1]["beta"] = float(beta)
         d = d * rdt.type(beta) - g
     if not res.stop:
         res.stop = "max_iter"

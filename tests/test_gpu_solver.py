@@ -130,16 +130,20 @@ def test_nlcg_config1_matches_reference_semantics():
         assert h["f_step"] <= h["f"] + 1e-4 * h["step"] * h["gdot"] + 1e-9 * abs(h["f"])
 
 
-@pytest.mark.parametrize("direction", ["fr", "prplus"])
+@pytest.mark.parametrize("direction", ["fr", "prplus", "dy"])
 def test_nlcg_dynamic_temporal_spatial_tv(direction):
     wl = small_dynamic()
     eng, hist, ref, f_gpu = run_pair(wl, 15, direction=direction)
     rel = abs(f_gpu - ref.f_final) / ref.f_final
-    print(f"[dyn {direction}] f_gpu={f_gpu:.8e} f_ref={ref.f_final:.8e} rel={rel:.2e}")
+    print(f"[dyn {direction}] f_gpu={f_gpu:.8e} f_ref={ref.f_final:.8e} rel={rel:.2e} iters={len(hist)}")
     assert rel <= 1e-4
     fh = np.array([h["f"] for h in hist])
     fr = np.array([h["f"] for h in ref.history])
+    assert len(fh) == len(fr)
     assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
+    if direction == "dy":      # Re<g_k, d_k> = beta_k Re<g_(k-1), d_(k-1)> on the device history too
+        for a, b in zip(hist, hist[1:]):
+            assert abs(b["gdot"] - a["beta"] * a["gdot"]) <= 1e-6 * abs(b["gdot"])
 
 
 def test_small_steps_take_the_second_line_search_pass():

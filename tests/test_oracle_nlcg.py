@@ -194,3 +194,22 @@ def test_restart_safeguard_keeps_every_direction_descending():
     # identical until the first direction the reference rejects
     for a, b in zip(ref.history, rst.history):
         assert a["step"] == b["step"] and a["f"] == b["f"]
+
+
+@pytest.mark.parametrize("direction,holds", [("dy", True), ("fr", False), ("prplus", False)])
+def test_dai_yuan_identity(direction, holds):
+    """Dai-Yuan's beta = |g_k|^2 / Re<d_(k-1), g_k - g_(k-1)> makes Re<g_k, d_k> =
+    beta_k Re<g_(k-1), d_(k-1)> exactly (d_k = beta_k d_(k-1) - g_k); FR and PR+ do not satisfy it,
+    so the check tells the rules apart (graspcg/numerics/directions.py:111-139)."""
+    A, AH, y, x0, regs = dense_problem(21)
+    # the descent restart never fires for DY here (beta_k > 0 and Re<g_k, d_k> < 0 keep d descending);
+    # it keeps FR and PR+ running past their first non-descent direction
+    res = nlcg(A, AH, y, x0, regs, max_iter=8, direction=direction, restart_c=0.0)
+    assert res.n_iter >= 6
+    h = res.history
+    rel = [abs(h[k + 1]["gdot"] - h[k]["beta"] * h[k]["gdot"]) / abs(h[k + 1]["gdot"]) for k in range(res.n_iter - 1)]
+    if holds:
+        assert max(rel) <= 1e-9, rel
+        assert all(x["gdot"] < 0 for x in h)
+    else:
+        assert max(rel) > 1e-3, rel
