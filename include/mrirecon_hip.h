@@ -173,6 +173,9 @@ typedef struct mrh_cg mrh_cg;
 #define MRH_CG_DIR_PRPLUS 1
 #define MRH_CG_DIR_DY 2       /* Dai-Yuan: beta = |g|^2 / Re<d, g - g_prev> */
 
+#define MRH_CG_LS_ARMIJO 0    /* backtracking Armijo (graspcg/numerics/line_search.py:24-79)        */
+#define MRH_CG_LS_WOLFE 1     /* strong Wolfe with bisection zoom (line_search.py:89-204)          */
+
 #define MRH_CG_ACTIVE 0
 #define MRH_CG_LS_FAILED 1
 #define MRH_CG_NOT_DESCENT 2
@@ -193,6 +196,8 @@ typedef struct {
                                whenever the new direction -g + beta d would have
                                Re<g, d_new> > -restart_c |g|^2 (not a reference
                                rule; the descent safeguard of PR+ with restarts) */
+  int ls_kind;              /* MRH_CG_LS_* (CGConfig.ls_name, graspcg/solvers/cg.py:37) */
+  double c2;                /* strong-Wolfe curvature constant (0.9, cg.py:39)   */
 } mrh_cg_desc;
 
 /* Per-problem, per-iteration record copied out by mrh_cg_read_history. */
@@ -225,7 +230,12 @@ int mrh_cg_residual(mrh_cg* eng, const void* Ax, const void* y, void* r, void* s
  * pass over x and d, first t with f(t) <= f(0) + c1 t Re<g, d> accepted.
  * Accepts the same t as sequential backtracking
  * (graspcg/numerics/line_search.py:24-79 with Objective.f_g_tensor,
- * graspcg/ops/objective.py:184-291) without evaluating gradients at trials. */
+ * graspcg/ops/objective.py:184-291) without evaluating gradients at trials.
+ * With ls_kind = MRH_CG_LS_WOLFE the strong-Wolfe search of line_search.py:89-204
+ * (doubling from t = 1, bisection zoom, lo end accepted after ls_max_iter bisections):
+ * f'(t) = b + c t + Re<g_TV(x + t d), d>, with TV values and slopes at seven candidate
+ * steps per image pass; the sequential search is replayed over the evaluated steps, so it
+ * accepts the same t as the reference's one-trial-at-a-time search. */
 int mrh_cg_linesearch(mrh_cg* eng, const void* x, const void* d, const void* g, const void* Ax,
                       const void* Ad, const void* y, void* stream);
 

@@ -15,11 +15,12 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
 CG_DIR_FR, CG_DIR_PRPLUS, CG_DIR_DY = 0, 1, 2
+CG_LS_ARMIJO, CG_LS_WOLFE = 0, 1
 CG_ACTIVE, CG_LS_FAILED, CG_NOT_DESCENT, CG_REL_TOL, CG_TARGET = 0, 1, 2, 3, 4
 
 
@@ -54,7 +55,7 @@ class TVDesc(Structure):
 class CGDesc(Structure):
     _fields_ = [("tv", TVDesc), ("k_prob", POINTER(c_int64)), ("direction", c_int), ("c1", c_double),
                 ("ls_beta", c_double), ("ls_max_iter", c_int), ("rel_tol", c_double), ("max_hist", c_int),
-                ("restart_c", c_double)]
+                ("restart_c", c_double), ("ls_kind", c_int), ("c2", c_double)]
 
 
 class CGHist(Structure):

@@ -24,6 +24,7 @@
 #define NT_A 6            // candidates of the first image pass: t = 0, 1, ..., beta^4
 #define PI_STRIDE 32      // doubles per block partial (image pass)
 #define PK_STRIDE 4       // doubles per block partial (k-space pass)
+#define NW 7              // strong-Wolfe candidates per image pass
 
 // ---------------------------------------------------------------- block reduction
 template <int NV>
@@ -307,13 +308,18 @@ struct ProbState {
   double a, b, c, g0d, dd;
   double tv[NT];
   double f0, t, ft, rho, rho_prev, beta, gpg, freg, gdot_step, f_prev;
+  // strong-Wolfe search: candidates of the next image pass, the expansion's next step and
+  // previous trial (t, f, f'), the zoom bracket (lo with f, f'; hi)
+  double wt[NW];
+  double w_t, w_tp, w_fp, w_gp, w_lo, w_flo, w_glo, w_hi;
   int status, iter, have_rho_prev, have_f_prev;
-  int ls_pending;   // the first line-search pass found no step: the second pass evaluates t <= beta^5
+  int ls_pending;   // Armijo: the first pass found no step; Wolfe: the next pass has candidates
+  int wn, wzoom, wi; // Wolfe: candidates in wt, zooming, trials (expansion) or bisections (zoom) done
 };
 
 struct mrh_cg {
-  int P, F, S0, S1, direction, ls_max_iter, max_hist;
-  double c1, ls_beta, rel_tol, restart_c;
+  int P, F, S0, S1, direction, ls_max_iter, max_hist, ls_kind;
+  double c1, ls_beta, rel_tol, restart_c, c2;
   int64_t npix_p, kmax_p;
   int nb_i, nb_k;
   TVDev tv;
@@ -513,6 +519,222 @@ __global__ __launch_bounds__(RT) void k_ls_select_rest(ProbState* st, const doub
   s.status = MRH_CG_LS_FAILED; s.t = 0.0; active[p] = 0;
 }
 
+// ---------------------------------------------------------------- strong-Wolfe search
+// graspcg/numerics/line_search.py:89-204 needs f(t) and f'(t) = Re<g(x + t d), d> at every
+// trial.  Along d the data term is 0.5 (a + 2 b t + c t^2) with slope b + c t; the
+// regulariser's value and slope come from image passes that each evaluate NW candidate
+// steps.  A step kernel then replays the sequential search over the evaluated candidates
+// (looked up by their exact t) until it needs a step that was not evaluated; that step
+// opens the next pass's candidates: the expansion's next NW doublings, or the three
+// bisection levels (7 midpoints) of the zoom bracket, which contain the next three
+// midpoints whichever end each bisection moves.
+
+// Image pass: TV(x + t d) and Re<g_TV(x + t d), d> for the candidates.  g_TV is the
+// regulariser gradient the reference evaluates (-lambda times the divergence of the flux,
+// without the axis weight, graspcg/regularization/tv_nd.py:54-88), so the slope is the one
+// the reference computes from its gradient at the trial.  The first pass also sums
+// Re<g, d> and |d|^2 and takes the candidates t = 0, 1, 2, 4, ..., 32; later passes take the
+// state's candidates and skip problems with nothing pending.
+template <bool FIRST>
+__global__ __launch_bounds__(RT) void k_wolfe_image(const float2* __restrict__ x, const float2* __restrict__ d,
+                                                    const float2* __restrict__ g, TVDev tv, int64_t npix_p,
+                                                    const ProbState* __restrict__ st, double* part) {
+  constexpr int V0 = FIRST ? 2 : 0, NV = V0 + 2 * NW;
+  const int p = blockIdx.y;
+  const bool live = st[p].status == MRH_CG_ACTIVE && (FIRST || st[p].ls_pending);
+  if (!FIRST && !live) return;   // partials unread: skip
+  double v[NV];
+#pragma unroll
+  for (int j = 0; j < NV; ++j) v[j] = 0.0;
+  float tj[NW];
+#pragma unroll
+  for (int j = 0; j < NW; ++j) tj[j] = FIRST ? (j == 0 ? 0.f : (float)(1 << (j - 1))) : (float)st[p].wt[j];
+  if (live) {
+    const int plane = tv.S0 * tv.S1;
+    for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
+      const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
+      const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
+      const int64_t n = (int64_t)p * npix_p + i;
+      if (FIRST) {
+        const float2 dn = d[n], gn = g[n];
+        v[0] += (double)gn.x * dn.x + (double)gn.y * dn.y;
+        v[1] += (double)dn.x * dn.x + (double)dn.y * dn.y;
+      }
+      for (int k = 0; k < tv.n; ++k) {
+        const TVTermDev& tm = tv.t[k];
+        if (tm.lam == 0.f || tm.axes == 0) continue;
+        float2 dx[3], du[3], dw[3];   // weighted differences of x and d, unweighted of d
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          dx[a] = make_float2(0.f, 0.f); du[a] = make_float2(0.f, 0.f);
+          if (tm.axes & (1 << a)) {
+            dx[a] = fdiff(x, n, f, ix, iy, a, tv, tm.w[a]);
+            du[a] = fdiff(d, n, f, ix, iy, a, tv, 1.f);
+          }
+          dw[a] = make_float2(tm.w[a] * du[a].x, tm.w[a] * du[a].y);
+        }
+        const float e2 = tm.eps * tm.eps;
+#pragma unroll
+        for (int j = 0; j < NW; ++j) {
+          const float t = tj[j];
+          float e = 0.f, sl = 0.f;
+          if (tm.iso) {
+            float ss = 0.f, zd = 0.f;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              const float zx = fmaf(t, dw[a].x, dx[a].x), zy = fmaf(t, dw[a].y, dx[a].y);
+              ss = fmaf(zx, zx, fmaf(zy, zy, ss));
+              zd = fmaf(zx, du[a].x, fmaf(zy, du[a].y, zd));
+            }
+            const float den = sqrtf(ss + e2);
+            e = den - tm.eps;
+            sl = zd / fmaxf(den, 1e-12f);
+          } else {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+              if (tm.axes & (1 << a)) {
+                const float zx = fmaf(t, dw[a].x, dx[a].x), zy = fmaf(t, dw[a].y, dx[a].y);
+                const float den = sqrtf(fmaf(zx, zx, zy * zy) + e2);
+                e += den - tm.eps;
+                sl += fmaf(zx, du[a].x, zy * du[a].y) / fmaxf(den, 1e-12f);
+              }
+            }
+          }
+          v[V0 + j] += (double)(tm.lam * e);
+          v[V0 + NW + j] += (double)(tm.lam * sl);
+        }
+      }
+    }
+  }
+  block_reduce_store<NV>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
+}
+
+__device__ __forceinline__ int wolfe_find(const double (&wt)[NW], int wn, double t) {
+  for (int j = 0; j < wn; ++j)
+    if (wt[j] == t) return j;
+  return -1;
+}
+
+// Step kernel (one block per problem): finish the pass's sums and continue the
+// sequential search of line_search.py:89-204 over the evaluated candidates.
+//   expansion (:121-152): trial i at t (t = 1, 2, 4, ...): Armijo break, or f >= f(previous
+//     trial) for i > 0, zooms on (previous, t); |f'| <= c2 |f'(0)| accepts; f' >= 0 zooms on
+//     (t, previous); else t = min(2 t, 1e6).  ls_max trials without a result fail the search.
+//   zoom (:161-204): mid = (lo + hi) / 2; Armijo break or f(mid) >= f(lo) sets hi = mid;
+//     otherwise the curvature test accepts, else hi = lo when f'(mid) (hi - lo) >= 0, and
+//     lo = mid.  After ls_max bisections lo is accepted (t = 0 included), as the reference does.
+// `last`: the final pass of the graph; a search still pending there fails (the pass count
+// covers the longest search, so this does not occur).
+template <bool FIRST>
+__global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* part_i, const double* part_k,
+                                                   int nb_i, int nb_k, double c1, double c2, int ls_max,
+                                                   int last, mrh_cg_hist* hist, int max_hist, int P,
+                                                   int* active) {
+  const int p = blockIdx.x;
+  ProbState& s = st[p];
+  if (!FIRST && !(s.status == MRH_CG_ACTIVE && s.ls_pending)) return;   // uniform over the block
+  double vals[5 + 2 * NW];
+  if (FIRST) {
+    block_sum_partials_n<3>(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, 0, vals);
+    block_sum_partials_n<2 + 2 * NW>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 3);
+  } else {
+    block_sum_partials_n<2 * NW>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 5);
+  }
+  if (threadIdx.x != 0) return;
+  const double* tvv = vals + 5;        // TV(x + t_j d)
+  const double* tvs = vals + 5 + NW;   // its slope
+  double wt[NW];
+  int wn;
+  s.ls_pending = 0;
+  if (FIRST) {
+    if (s.status != MRH_CG_ACTIVE) { s.t = 0.0; return; }
+    s.a = vals[0]; s.b = vals[1]; s.c = vals[2]; s.g0d = vals[3]; s.dd = vals[4];
+    s.f0 = 0.5 * s.a + tvv[0];
+    s.t = 0.0;
+    s.ft = s.f0;
+    if (!(s.g0d < 0.0) || !isfinite(s.g0d)) {
+      s.status = MRH_CG_NOT_DESCENT;
+      active[p] = 0;
+      return;
+    }
+    for (int j = 0; j < NW; ++j) wt[j] = j == 0 ? 0.0 : (double)(1 << (j - 1));
+    wn = NW;
+    s.wzoom = 0; s.wi = 0;
+    s.w_t = 1.0; s.w_tp = 0.0; s.w_fp = s.f0; s.w_gp = s.g0d;
+  } else {
+    for (int j = 0; j < NW; ++j) wt[j] = s.wt[j];
+    wn = s.wn;
+  }
+  const double f0 = s.f0, g0d = s.g0d, gcurv = c2 * fabs(g0d);
+  auto accept = [&](double t, double ft) {
+    s.t = t; s.ft = ft;
+    ls_record(s, hist, max_hist, P, p);
+  };
+  auto zoom = [&](double lo, double flo, double glo, double hi) {
+    s.wzoom = 1; s.wi = 0;
+    s.w_lo = lo; s.w_flo = flo; s.w_glo = glo; s.w_hi = hi;
+  };
+  for (;;) {
+    if (!s.wzoom) {
+      if (s.wi >= ls_max) { s.status = MRH_CG_LS_FAILED; s.t = 0.0; active[p] = 0; return; }
+      const double t = s.w_t;
+      const int j = wolfe_find(wt, wn, t);
+      if (j < 0) {   // next pass: this step and its doublings, as many as trials remain
+        const int n = min(NW, ls_max - s.wi);
+        double u = t;
+        for (int k = 0; k < NW; ++k) {
+          s.wt[k] = k < n ? u : s.wt[0];
+          u = fmin(2.0 * u, 1e6);
+        }
+        s.wn = n;
+        s.ls_pending = 1;
+        break;
+      }
+      const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + tvv[j];
+      const double gt = s.b + t * s.c + tvs[j];
+      if (ft > f0 + c1 * t * g0d || (s.wi > 0 && ft >= s.w_fp)) { zoom(s.w_tp, s.w_fp, s.w_gp, t); continue; }
+      if (fabs(gt) <= gcurv) { accept(t, ft); return; }
+      if (gt >= 0.0) { zoom(t, ft, gt, s.w_tp); continue; }
+      s.w_tp = t; s.w_fp = ft; s.w_gp = gt;
+      s.w_t = fmin(2.0 * t, 1e6);
+      s.wi += 1;
+    } else {
+      if (s.wi >= ls_max) { accept(s.w_lo, s.w_flo); return; }
+      const double lo = s.w_lo, hi = s.w_hi, m = 0.5 * (lo + hi);
+      const int j = wolfe_find(wt, wn, m);
+      if (j < 0) {   // next pass: the bracket's midpoints to depth three
+        const double mL = 0.5 * (lo + m), mR = 0.5 * (m + hi);
+        s.wt[0] = m; s.wt[1] = mL; s.wt[2] = mR;
+        s.wt[3] = 0.5 * (lo + mL); s.wt[4] = 0.5 * (mL + m);
+        s.wt[5] = 0.5 * (m + mR); s.wt[6] = 0.5 * (mR + hi);
+        s.wn = NW;
+        s.ls_pending = 1;
+        break;
+      }
+      const double fm = 0.5 * (s.a + 2.0 * m * s.b + m * m * s.c) + tvv[j];
+      const double gm = s.b + m * s.c + tvs[j];
+      if (fm > f0 + c1 * m * g0d || fm >= s.w_flo) {
+        s.w_hi = m;
+      } else {
+        if (fabs(gm) <= gcurv) { accept(m, fm); return; }
+        if (gm * (hi - lo) >= 0.0) s.w_hi = lo;
+        s.w_lo = m; s.w_flo = fm; s.w_glo = gm;
+      }
+      s.wi += 1;
+    }
+  }
+  if (last) { s.ls_pending = 0; s.status = MRH_CG_LS_FAILED; s.t = 0.0; active[p] = 0; }
+}
+
+// Image passes the strong-Wolfe search may need after the first: ls_max expansion trials
+// (NW - 1 in the first pass, NW per later pass) and ls_max bisections (three per pass), the
+// zoom starting on the pass after the one that opened it.
+static int wolfe_rest_passes(int ls_max) {
+  const int exp_passes = 1 + (std::max(0, ls_max - (NW - 1)) + NW - 1) / NW;
+  const int zoom_passes = (ls_max + 2) / 3;
+  return exp_passes + zoom_passes - 1;
+}
+
 // x += t d (image) for every problem.
 __global__ __launch_bounds__(RT) void k_accept_img(float2* __restrict__ x, const float2* __restrict__ d,
                                                    int64_t npix_p, const ProbState* __restrict__ st) {
@@ -661,6 +883,10 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   MRH_REQUIRE(d->direction == MRH_CG_DIR_FR || d->direction == MRH_CG_DIR_PRPLUS || d->direction == MRH_CG_DIR_DY,
               MRH_EINVAL, "direction must be FR, PR+ or DY");
   MRH_REQUIRE(d->tv.P >= 1 && d->tv.P <= 65535, MRH_EINVAL, "P outside [1, 65535]");
+  MRH_REQUIRE(d->ls_kind == MRH_CG_LS_ARMIJO || d->ls_kind == MRH_CG_LS_WOLFE, MRH_EINVAL,
+              "ls_kind must be Armijo or strong Wolfe");
+  MRH_REQUIRE(!(d->ls_kind == MRH_CG_LS_WOLFE) || (d->c2 >= 0.0 && std::isfinite(d->c2)), MRH_EINVAL,
+              "c2 must be a finite value >= 0");
   mrh_cg* e = new mrh_cg();
   std::memset(e, 0, sizeof(*e));
   int rc = make_tvdev(&d->tv, &e->tv);
@@ -668,6 +894,7 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   e->P = d->tv.P; e->F = d->tv.F; e->S0 = d->tv.S0; e->S1 = d->tv.S1;
   e->direction = d->direction; e->ls_max_iter = d->ls_max_iter;
   e->c1 = d->c1; e->ls_beta = d->ls_beta; e->rel_tol = d->rel_tol; e->restart_c = d->restart_c;
+  e->ls_kind = d->ls_kind; e->c2 = d->c2;
   e->max_hist = std::max(1, d->max_hist);
   e->npix_p = (int64_t)e->F * e->S0 * e->S1;
   int64_t kmax = 0;
@@ -736,6 +963,26 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
   hipLaunchKernelGGL(k_ls_kspace, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
                      (const float2*)y, e->d_kprob, e->d_st, e->d_part_k);
   MRH_CHECK_LAUNCH();
+  if (e->ls_kind == MRH_CG_LS_WOLFE) {
+    const int rest = wolfe_rest_passes(e->ls_max_iter);
+    hipLaunchKernelGGL(k_wolfe_image<true>, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+                       (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->d_st, e->d_part_i);
+    MRH_CHECK_LAUNCH();
+    hipLaunchKernelGGL(k_wolfe_step<true>, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
+                       e->nb_k, e->c1, e->c2, e->ls_max_iter, rest == 0 ? 1 : 0, e->d_hist, e->max_hist, e->P,
+                       e->d_active);
+    MRH_CHECK_LAUNCH();
+    for (int q = 0; q < rest; ++q) {
+      hipLaunchKernelGGL(k_wolfe_image<false>, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+                         (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->d_st, e->d_part_i);
+      MRH_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k_wolfe_step<false>, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k,
+                         e->nb_i, e->nb_k, e->c1, e->c2, e->ls_max_iter, q + 1 == rest ? 1 : 0, e->d_hist,
+                         e->max_hist, e->P, e->d_active);
+      MRH_CHECK_LAUNCH();
+    }
+    return MRH_OK;
+  }
   hipLaunchKernelGGL((k_ls_image<0, NT_A>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
                      (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
   MRH_CHECK_LAUNCH();

@@ -5,7 +5,10 @@ ragged per-frame k-space in one set of launches.  One iteration is
 
     line search  : a = |Ax-y|^2, b = Re<Ax-y, Ad>, c = |Ad|^2 and TV(x + t d) for
                    every candidate t in {1, 1/2, ..., 2^-19} in one pass, first t
-                   with f(t) <= f(0) + c1 t Re<g, d> selected on the device;
+                   with f(t) <= f(0) + c1 t Re<g, d> selected on the device
+                   (``ls_name="wolfe"``: the strong-Wolfe search, TV values and
+                   slopes at seven steps per pass, the sequential search replayed
+                   over them on the device);
     accept       : x += t d, Ax += t Ad, r = Ax - y;
     gradient     : g = A^H r + grad TV(x), |g|^2, beta (FR, first beta = 1; or PR+);
     direction    : d = beta d - g;  Ad = A d.
@@ -40,6 +43,8 @@ _STATUS = {0: "active", 1: "ls_failed", 2: "not_descent", 3: "rel_tol", 4: "targ
 # direction rules of the reference (numerics/directions.py): FR (:141-162, first beta = 1),
 # PR+ (:75-109) and Dai-Yuan (:111-139)
 _DIRECTIONS = {"fr": _lib.CG_DIR_FR, "prplus": _lib.CG_DIR_PRPLUS, "dy": _lib.CG_DIR_DY}
+# line searches (numerics/line_search.py): backtracking Armijo (:24-79), strong Wolfe (:89-204)
+_SEARCHES = {"armijo": _lib.CG_LS_ARMIJO, "wolfe": _lib.CG_LS_WOLFE}
 
 
 class NLCGEngine:
@@ -48,9 +53,11 @@ class NLCGEngine:
     def __init__(self, plan, y: torch.Tensor, *, problems: int, frames: int, regm: Optional[RegManager],
                  direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5, ls_max_iter: int = 20,
                  rel_tol: float = 0.0, max_hist: int = 256, restart_c: Optional[float] = None,
-                 skip_stopped: bool = True):
+                 skip_stopped: bool = True, ls_name: str = "armijo", c2: float = 0.9):
         if direction not in _DIRECTIONS:
             raise ValueError(f"direction must be one of {sorted(_DIRECTIONS)}")
+        if ls_name not in _SEARCHES:
+            raise ValueError(f"ls_name must be one of {sorted(_SEARCHES)}")
         self.plan = plan
         self.P, self.F = int(problems), int(frames)
         if self.P * self.F != plan.B:
@@ -80,6 +87,8 @@ class NLCGEngine:
         desc.c1, desc.ls_beta, desc.ls_max_iter = float(c1), float(ls_beta), int(ls_max_iter)
         desc.rel_tol, desc.max_hist = float(rel_tol or 0.0), int(max_hist)
         desc.restart_c = -1.0 if restart_c is None else float(restart_c)
+        desc.ls_kind, desc.c2 = _SEARCHES[ls_name], float(c2)
+        self.ls_name = ls_name
         if restart_c is not None and restart_c < 0:
             raise ValueError("restart_c must be >= 0 (None keeps the reference behaviour)")
         self.max_hist = int(max_hist)
@@ -281,8 +290,6 @@ class CGSolver:
 
     def __init__(self, y: torch.Tensor, nufft_op, regm: RegManager, cfg: Optional[CGConfig] = None):
         self.cfg = cfg or CGConfig()
-        if self.cfg.ls_name.lower() != "armijo":
-            raise NotImplementedError("only the Armijo line search runs on the device")
         self.y, self.nufft_op, self.regm = y, nufft_op, regm
         be = getattr(nufft_op, "backend", nufft_op)
         plan = be.plan
@@ -291,7 +298,8 @@ class CGSolver:
                                  direction=self.cfg.direction.lower(), c1=self.cfg.c1,
                                  ls_beta=self.cfg.ls_beta, ls_max_iter=self.cfg.ls_max_iter,
                                  rel_tol=self.cfg.rel_tol or 0.0, max_hist=max(64, self.cfg.max_iter),
-                                 restart_c=self.cfg.restart_c)
+                                 restart_c=self.cfg.restart_c, ls_name=self.cfg.ls_name.lower(),
+                                 c2=self.cfg.c2)
         self.ws = Workspace(self.engine)
         if self.cfg.init_mode == "zero":
             self.engine.x.zero_()

@@ -16,6 +16,8 @@ included, on any operator exposing ``forward`` / ``adjoint`` over
   (:256-266), slope Re<g, d> (:268-277);
 * Armijo backtracking from t = 1 with factor 0.5, c1 = 1e-4, at most 20 trials,
   rejecting a non-descent / non-finite slope (graspcg/numerics/line_search.py:24-79);
+  or the strong-Wolfe search (doubling from t = 1, bisection zoom, c2 = 0.9;
+  line_search.py:89-204) with ``ls_name="wolfe"``;
 * CG loop: prime d = -g (graspcg/solvers/cg.py:178-190), accept x += t d
   (:205-207), stats norms (:213-220), the reference convergence test
   (graspcg/solvers/utils.py:40-43), direction update FR with the first beta = 1
@@ -29,7 +31,11 @@ direction (:229-248), the Armijo inequality and monotone history
 finite-difference slope (:180-229), TV of a constant, the fwd/bwd adjoint
 identity and the hand-computable step-edge energy; Dai-Yuan by its identity
 Re<g_k, d_k> = beta_k Re<g_(k-1), d_(k-1)>, which no other beta rule satisfies
-in general.
+in general; strong Wolfe by the reference's zoom-to-1/2 case
+(graspcg/tests/test_numerics_ns.py:195-225), hand-traced steps of the expansion
+and of both zoom branches on scaled quadratics, its trial limits, and the two
+Wolfe conditions along TV-regularised runs of every direction rule
+(graspcg/tests/test_cg_descend_compile_ns.py:125-176).
 """
 from __future__ import annotations
 
@@ -166,6 +172,55 @@ def armijo(obj: "Objective", x: np.ndarray, d: np.ndarray, c1: float = 1e-4, ls_
     return False, t, f0, g0d, ft, gdt, g
 
 
+def wolfe(obj: "Objective", x: np.ndarray, d: np.ndarray, c1: float = 1e-4, c2: float = 0.9,
+          ls_max_iter: int = 20, t_max: float = 1e6):
+    """Strong-Wolfe search with doubling and bisection zoom (graspcg/numerics/line_search.py:89-204).
+
+    Expansion from t = 1 (:121-152): a step that breaks Armijo, or (after the first trial)
+    does not lower f below the previous trial, opens a zoom between the previous trial and it;
+    a step meeting |g'(t)| <= c2 |g'(0)| is accepted; a non-negative slope opens a zoom with the
+    ends swapped; otherwise t doubles (capped at t_max).  Out of trials: failure (ok False).
+    Zoom (:161-204): bisect; Armijo break or f(mid) >= f(lo) moves hi; otherwise accept on the
+    curvature test, else hi <- lo when g'(mid) (hi - lo) >= 0, then lo <- mid.  After
+    ``ls_max_iter`` bisections the lo end is accepted (its f and g evaluated again, :197-204),
+    even at t = 0.  Returns (ok, t, f0, g0d, f_t, gdot_t, g) like ``armijo``."""
+    rdt = x.real.dtype
+    obj.begin_linesearch(x, d)
+    f0, g0d, g = obj.f_g(x, d, 0.0)
+    if not (g0d < 0) or not np.isfinite(g0d):
+        return False, rdt.type(0.0), f0, g0d, f0, g0d, g
+    c1, c2 = rdt.type(c1), rdt.type(c2)
+
+    def zoom(t_lo, f_lo, t_hi):
+        for _ in range(ls_max_iter):
+            t_mid = rdt.type(0.5) * (t_lo + t_hi)
+            f_mid, g_mid, gm = obj.f_g(x, d, t_mid)
+            if f_mid > f0 + c1 * t_mid * g0d or f_mid >= f_lo:
+                t_hi = t_mid
+            else:
+                if abs(g_mid) <= c2 * abs(g0d):
+                    return True, t_mid, f0, g0d, f_mid, g_mid, gm
+                if g_mid * (t_hi - t_lo) >= 0:
+                    t_hi = t_lo
+                t_lo, f_lo = t_mid, f_mid
+        f_lo, g_lo, gl = obj.f_g(x, d, t_lo)
+        return True, t_lo, f0, g0d, f_lo, g_lo, gl
+
+    t = rdt.type(1.0)
+    t_prev, f_prev, g_prev = rdt.type(0.0), f0, g0d
+    for i in range(ls_max_iter):
+        ft, gdt, g = obj.f_g(x, d, t)
+        if ft > f0 + c1 * t * g0d or (i > 0 and ft >= f_prev):
+            return zoom(t_prev, f_prev, t)
+        if abs(gdt) <= c2 * abs(g0d):
+            return True, t, f0, g0d, ft, gdt, g
+        if gdt >= 0:
+            return zoom(t, ft, t_prev)
+        t_prev, f_prev, g_prev = t, ft, gdt
+        t = rdt.type(min(2.0 * t, t_max))
+    return False, t_prev, f0, g0d, f_prev, g_prev, g
+
+
 # --------------------------------------------------------------- solver
 
 @dataclass
@@ -187,8 +242,10 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
          max_iter: int = 50, direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5,
          ls_max_iter: int = 20, tol_g: float = 1e-4, tol_step: float = 1e-9,
          rel_tol: Optional[float] = None, mutate_x: bool = True,
-         restart_c: Optional[float] = None) -> CGResult:
-    """Reference-semantics NLCG (graspcg/solvers/cg.py:162-246) with Armijo search.
+         restart_c: Optional[float] = None, ls_name: str = "armijo", c2: float = 0.9) -> CGResult:
+    """Reference-semantics NLCG (graspcg/solvers/cg.py:162-246) with Armijo search, or the
+    strong-Wolfe search with ``ls_name="wolfe"`` (line_search.py:12-17, CGConfig.ls_name / c2
+    at cg.py:37-39).
 
     ``rel_tol`` adds the stop rule |f_k - f_{k-1}| / |f_{k-1}| < rel_tol on the
     accepted objective values (not a reference rule; used for time-to-tolerance).
@@ -209,7 +266,12 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
     g_prev = g.copy()
     f_prev = None
     for k in range(max_iter):
-        ok, t, f0, g0d, ft, gdt, g_new = armijo(obj, x, d, c1=c1, ls_beta=ls_beta, ls_max_iter=ls_max_iter)
+        if ls_name == "wolfe":
+            ok, t, f0, g0d, ft, gdt, g_new = wolfe(obj, x, d, c1=c1, c2=c2, ls_max_iter=ls_max_iter)
+        elif ls_name == "armijo":
+            ok, t, f0, g0d, ft, gdt, g_new = armijo(obj, x, d, c1=c1, ls_beta=ls_beta, ls_max_iter=ls_max_iter)
+        else:
+            raise ValueError(ls_name)
         if g_new is not None:
             g = g_new
         if not ok:

@@ -96,19 +96,19 @@ def regs_for(wl):
     return RegManager(g), o
 
 
-def run_pair(wl, n_iter, direction="fr", use_graph=True, rel_tol=None):
+def run_pair(wl, n_iter, direction="fr", use_graph=True, rel_tol=None, ls_name="armijo", c2=0.9):
     be, ora = prepare(wl)
     regm, oregs = regs_for(wl)
     x0 = ora.adjoint(wl.y)
     eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, direction=direction,
-                     rel_tol=rel_tol or 0.0)
+                     rel_tol=rel_tol or 0.0, ls_name=ls_name, c2=c2)
     eng.prime(c64(x0))
     eng.run(n_iter, use_graph=use_graph)
     torch.cuda.synchronize()
     hist = eng.history()[0]
     x_gpu = eng.x.cpu().numpy()
     ref = nlcg(ora.forward, ora.adjoint, wl.y.astype(np.complex128), x0.astype(np.complex128), oregs,
-               max_iter=n_iter, direction=direction, mutate_x=False, rel_tol=rel_tol)
+               max_iter=n_iter, direction=direction, mutate_x=False, rel_tol=rel_tol, ls_name=ls_name, c2=c2)
     f_gpu = objective_value(ora.forward, wl.y, x_gpu.astype(np.complex128), oregs)
     return eng, hist, ref, f_gpu
 
@@ -144,6 +144,55 @@ def test_nlcg_dynamic_temporal_spatial_tv(direction):
     if direction == "dy":      # Re<g_k, d_k> = beta_k Re<g_(k-1), d_(k-1)> on the device history too
         for a, b in zip(hist, hist[1:]):
             assert abs(b["gdot"] - a["beta"] * a["gdot"]) <= 1e-6 * abs(b["gdot"])
+
+
+def check_wolfe_run(hist, ref, f_gpu, c2, tag):
+    steps = [h["step"] for h in hist]
+    print(f"[wolfe {tag}] f_gpu={f_gpu:.8e} f_ref={ref.f_final:.8e} steps={steps} stop={ref.stop}")
+    assert [h["step"] for h in ref.history] == steps
+    fh = np.array([h["f"] for h in hist])
+    fr = np.array([h["f"] for h in ref.history])
+    assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
+    assert abs(f_gpu - ref.f_final) <= 1e-4 * abs(ref.f_final)
+    for h, r in zip(hist, ref.history):
+        # Armijo and the strong curvature condition at every accepted step (the slope at the step
+        # from the gradient pass, float rounding aside), as graspcg/tests/test_cg_descend_compile_ns.py:125-176
+        assert h["f_step"] <= h["f"] + 1e-4 * h["step"] * h["gdot"] + 1e-9 * abs(h["f"])
+        assert abs(h["gdot_at_step"]) <= c2 * abs(h["gdot"]) * (1 + 1e-4)
+        assert abs(h["gdot_at_step"] - r["gdot_at_step"]) <= 1e-3 * abs(r["gdot"])
+    return steps
+
+
+@pytest.mark.parametrize("direction", ["fr", "prplus", "dy"])
+def test_nlcg_wolfe_matches_oracle(direction):
+    """Strong-Wolfe search (graspcg/numerics/line_search.py:89-204, c2 = 0.1) with temporal + spatial TV:
+    the device replay of the search accepts the oracle's steps (zooms from (0, 1) two passes deep),
+    objectives within 1e-4."""
+    wl = small_dynamic(seed=71)
+    eng, hist, ref, f_gpu = run_pair(wl, 12, direction=direction, ls_name="wolfe", c2=0.1)
+    assert len(hist) == 12 and ref.n_iter == 12
+    steps = check_wolfe_run(hist, ref, f_gpu, 0.1, direction)
+    assert any(s * 32 != round(s * 32) for s in steps)   # a bisection below 1/32: a second zoom pass
+
+
+def test_wolfe_expansion_and_zoom_over_several_passes():
+    """Coil maps scaled by 1/16 (|A|^2 / 256), no regulariser: the search doubles to t = 256, past the
+    first pass's six trials, and zooms back (160, 64, 48, 72, ...); steps as in the oracle run."""
+    wl = small_dynamic(seed=61)
+    wl.maps = wl.maps / 16.0
+    wl.lam_t = wl.lam_s = 0.0
+    eng, hist, ref, f_gpu = run_pair(wl, 6, ls_name="wolfe", c2=0.1)
+    steps = check_wolfe_run(hist, ref, f_gpu, 0.1, "expansion")
+    assert max(steps) > 64
+
+
+def test_wolfe_small_steps_zoom_deep():
+    """Coil maps scaled by 8: accepted steps near 2^-8, about ten bisections of (0, 1) (four passes)."""
+    wl = small_dynamic(seed=61)
+    wl.maps = wl.maps * 8.0
+    eng, hist, ref, f_gpu = run_pair(wl, 6, ls_name="wolfe", c2=0.1)
+    steps = check_wolfe_run(hist, ref, f_gpu, 0.1, "deep zoom")
+    assert min(steps) < 2.0 ** -8
 
 
 def test_small_steps_take_the_second_line_search_pass():
@@ -184,10 +233,11 @@ def test_mixed_axes_tv_term_matches_oracle():
     assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
 
 
-def test_graph_replay_equals_eager():
+@pytest.mark.parametrize("ls_name", ["armijo", "wolfe"])
+def test_graph_replay_equals_eager(ls_name):
     wl = small_dynamic(seed=31)
-    _, h_graph, _, f_graph = run_pair(wl, 8, use_graph=True)
-    _, h_eager, _, f_eager = run_pair(wl, 8, use_graph=False)
+    _, h_graph, _, f_graph = run_pair(wl, 8, use_graph=True, ls_name=ls_name, c2=0.1)
+    _, h_eager, _, f_eager = run_pair(wl, 8, use_graph=False, ls_name=ls_name, c2=0.1)
     for a, b in zip(h_graph, h_eager):
         assert a["step"] == b["step"]
         assert abs(a["f"] - b["f"]) <= 1e-6 * abs(a["f"])
@@ -202,8 +252,10 @@ def test_rel_tol_stop_rule():
     assert abs(len(hist) - ref.n_iter) <= 1
 
 
-def test_batched_problems_equal_separate_runs():
-    """Two slices with uneven spokes in one engine == each slice alone."""
+@pytest.mark.parametrize("ls_name", ["armijo", "wolfe"])
+def test_batched_problems_equal_separate_runs(ls_name):
+    """Two slices with uneven spokes in one engine == each slice alone (with either search; the
+    strong-Wolfe passes skip the slice whose search has ended)."""
     S, C, R, F = 48, 3, 96, 4
     spokes = [[13, 21, 17, 30], [25, 14, 40, 19]]
     ws = []
@@ -216,7 +268,7 @@ def test_batched_problems_equal_separate_runs():
     for wl in ws:
         be, ora = prepare(wl)
         regm, _ = regs_for(wl)
-        eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=F, regm=regm)
+        eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=F, regm=regm, ls_name=ls_name, c2=0.1)
         eng.prime(c64(ora.adjoint(wl.y)))
         eng.run(6)
         solo.append((eng.history()[0], eng.x.cpu().numpy()))
@@ -228,7 +280,7 @@ def test_batched_problems_equal_separate_runs():
                k_offsets=off)
     y = np.concatenate([w.y for w in ws])
     x0 = np.concatenate([KBNufft(w.maps, w.traj, w.k_offsets, w.dcf, 6).adjoint(w.y) for w in ws])
-    eng = NLCGEngine(be.plan, c64(y), problems=2, frames=F, regm=regs_for(ws[0])[0])
+    eng = NLCGEngine(be.plan, c64(y), problems=2, frames=F, regm=regs_for(ws[0])[0], ls_name=ls_name, c2=0.1)
     eng.prime(c64(x0))
     eng.run(6)
     hb = eng.history()
@@ -254,8 +306,12 @@ def test_cgsolver_reference_surface():
         assert k in h[0]
     assert tuple(solver.result().shape) == (1, 1, wl.S, wl.S)
     assert solver.ws.get("x").data_ptr() == solver.ws.get("var").data_ptr()
-    with pytest.raises(NotImplementedError):
-        CGSolver(y, be, regm, CGConfig(ls_name="wolfe"))
+    with pytest.raises(ValueError):
+        CGSolver(y, be, regm, CGConfig(ls_name="more-thuente"))
+    wsolver = CGSolver(y, be, regm, CGConfig(max_iter=4, init_mode="backproj", ls_name="wolfe", c2=0.5))
+    wsolver.run()
+    for h in wsolver.history():
+        assert abs(h["gdot_at_step"]) <= 0.5 * abs(h["gdot"]) * (1 + 1e-4)
 
 
 def test_restart_safeguard_matches_oracle():

@@ -12,7 +12,7 @@ import math
 import numpy as np
 import pytest
 
-from oracle.nlcg import TV, Objective, armijo, bwd_diff, fwd_diff, nlcg, objective_value, tv_energy_grad
+from oracle.nlcg import TV, Objective, armijo, bwd_diff, fwd_diff, nlcg, objective_value, tv_energy_grad, wolfe
 
 
 def fft_op():
@@ -213,3 +213,59 @@ def test_dai_yuan_identity(direction, holds):
         assert all(x["gdot"] < 0 for x in h)
     else:
         assert max(rel) > 1e-3, rel
+
+
+# ------------------------------------------------------------------ strong Wolfe
+
+def quad_search(seed, scale, c1=1e-4, c2=0.9, ls_max_iter=20):
+    """A unitary, y = 0, d = -x / scale: f(t) = 0.5 (1 - t / scale)^2 |x|^2, minimum at t = scale."""
+    A, AH = fft_op()
+    x = rand_c((2, 16, 16), seed)
+    obj = Objective(A, AH, np.zeros(2 * 256, complex), [], mutate_x=False)
+    return wolfe(obj, x, -x / scale, c1=c1, c2=c2, ls_max_iter=ls_max_iter), float(np.vdot(x, x).real)
+
+
+def test_wolfe_zoom_lands_at_half():
+    """graspcg/tests/test_numerics_ns.py:195-225: c1 large enough that t = 1 breaks Armijo, the zoom
+    on (0, 1) accepts its first midpoint t = 1/2, where |f'| = |f'(0)| / 2.  (c1 = 0.6: the reference's
+    0.75 puts f(1/2) exactly on the Armijo line, where rounding decides.)"""
+    (ok, t, f0, g0d, ft, gdt, _), n2 = quad_search(30, 1.0, c1=0.6)
+    assert ok and t == 0.5
+    assert abs(ft - 0.125 * n2) <= 1e-12 * n2 and abs(gdt + 0.5 * n2) <= 1e-12 * n2
+    assert abs(gdt) <= 0.9 * abs(g0d)
+
+
+@pytest.mark.parametrize("scale,c2,t_expect,why", [
+    (4.0, 0.1, 4.0, "doubling 1, 2 (|f'| too large), 4 (f' = 0)"),
+    (2.2, 0.05, 2.25, "f(4) >= f(2) zooms (2, 4): 3 and 2.5 move hi, 2.25 meets both conditions"),
+    (2.4, 0.01, 2.40625, "zoom (2, 4) takes the hi <- lo branch at 2.5 and 2.375, accepts 77/32"),
+])
+def test_wolfe_known_steps(scale, c2, t_expect, why):
+    """Hand-traced runs of the expansion and zoom (line_search.py:121-204) on the quadratic."""
+    (ok, t, f0, g0d, ft, gdt, _), n2 = quad_search(31, scale, c2=c2)
+    assert ok and t == t_expect, (why, t)
+    assert abs(ft - 0.5 * (1 - t / scale) ** 2 * n2) <= 1e-12 * n2
+    assert abs(gdt) <= c2 * abs(g0d) and ft <= f0 + 1e-4 * t * g0d
+
+
+def test_wolfe_trial_limits():
+    """Out of expansion trials: failure (:151-155).  Out of bisections: the lo end is accepted
+    (:197-204): the 2.4 case above, cut to 4 trials, stops the zoom at lo = 2.375."""
+    (ok, t, *_), _ = quad_search(32, 64.0, c2=0.1, ls_max_iter=2)
+    assert not ok
+    (ok, t, f0, g0d, ft, gdt, _), n2 = quad_search(31, 2.4, c2=0.01, ls_max_iter=4)
+    assert ok and t == 2.375 and abs(gdt) > 0.01 * abs(g0d)
+    (ok, t, f0, g0d, *_), _ = quad_search(33, -1.0)   # d = +x: ascent, rejected with t = 0
+    assert not ok and t == 0.0 and g0d >= 0
+
+
+@pytest.mark.parametrize("direction", ["fr", "prplus", "dy"])
+def test_wolfe_runs_satisfy_strong_conditions(direction):
+    """graspcg/tests/test_cg_descend_compile_ns.py:125-176: Armijo and |f'(t_k)| <= c2 |f'(0)| at every
+    accepted step of a TV-regularised run, for every direction rule."""
+    A, AH, y, x0, regs = dense_problem(41)
+    res = nlcg(A, AH, y, x0, regs, max_iter=10, direction=direction, ls_name="wolfe", restart_c=0.0)
+    assert res.n_iter >= 5, res.stop
+    for h in res.history:
+        assert h["f_step"] <= h["f"] + 1e-4 * h["step"] * h["gdot"] + 1e-12
+        assert abs(h["gdot_at_step"]) <= 0.9 * abs(h["gdot"]) + 1e-12
