@@ -198,6 +198,9 @@ typedef struct {
                                rule; the descent safeguard of PR+ with restarts) */
   int ls_kind;              /* MRH_CG_LS_* (CGConfig.ls_name, graspcg/solvers/cg.py:37) */
   double c2;                /* strong-Wolfe curvature constant (0.9, cg.py:39)   */
+  float* diag;              /* device, P*F*S0*S1 floats: the diagonal preconditioner D of the
+                               direction rules (the workspace's "diag", cg.py:95), read by
+                               mrh_cg_gradient / mrh_cg_direction; NULL: D = 1 */
 } mrh_cg_desc;
 
 /* Per-problem, per-iteration record copied out by mrh_cg_read_history. */
@@ -262,6 +265,13 @@ int mrh_cg_direction(mrh_cg* eng, void* d, const void* g, void* g_prev, int prim
  * Used to time how long a run takes to reach a given objective (e.g. another
  * run's final value); not a reference stop rule.  Synchronises `stream`. */
 int mrh_cg_set_targets(mrh_cg* eng, const double* targets, void* stream);
+
+/* Fill desc.diag with the regularisers' diagonal majoriser at x: 1 plus, per TV term, its
+ * frame-axis profile 2 lambda w_0^2 x degree (a term over frames) or the lagged-diffusivity
+ * weights lambda w_a^2 / sqrt(|D_a x|^2 + eps^2) x degree per axis (any other term).
+ * Replaces RegManager.add_diag (graspcg/regularization/manager.py:145-218, TVND
+ * majorizer_profile / add_diag at graspcg/regularization/tv_nd.py:113-215). */
+int mrh_cg_build_diag(mrh_cg* eng, const void* x, void* stream);
 
 /* Device int[P]: 1 while problem p iterates, 0 once it stopped (updated by the
  * engine's kernels; reset to 1 by mrh_cg_reset).  Owned by the engine. */

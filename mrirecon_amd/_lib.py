@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -55,7 +55,8 @@ class TVDesc(Structure):
 class CGDesc(Structure):
     _fields_ = [("tv", TVDesc), ("k_prob", POINTER(c_int64)), ("direction", c_int), ("c1", c_double),
                 ("ls_beta", c_double), ("ls_max_iter", c_int), ("rel_tol", c_double), ("max_hist", c_int),
-                ("restart_c", c_double), ("ls_kind", c_int), ("c2", c_double)]
+                ("restart_c", c_double), ("ls_kind", c_int), ("c2", c_double),
+                ("diag", c_void_p)]
 
 
 class CGHist(Structure):
@@ -91,6 +92,7 @@ _SIGS = {
     "mrh_cg_direction": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_void_p]),
     "mrh_cg_set_targets": (c_int, [c_void_p, POINTER(c_double), c_void_p]),
     "mrh_cg_active_flags": (c_int, [c_void_p, POINTER(c_void_p)]),
+    "mrh_cg_build_diag": (c_int, [c_void_p, c_void_p, c_void_p]),
     "mrh_cg_read_status": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), c_void_p]),
     "mrh_cg_read_history": (c_int, [c_void_p, POINTER(CGHist), c_int, c_void_p]),
     "mrh_unet_conv_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 9 + [c_void_p]),

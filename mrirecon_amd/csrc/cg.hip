@@ -330,6 +330,7 @@ struct mrh_cg {
   mrh_cg_hist* d_hist;  // [max_hist][P]
   double* d_target;     // [P] objective targets (-inf: none)
   int* d_active;        // [P] 1 while the problem iterates (read by the masked NUFFT entry points)
+  float* d_diag;        // caller's diagonal preconditioner (P F S0 S1 floats) or null
 };
 
 __global__ void k_cg_reset(ProbState* st, int* active, int P) {
@@ -771,11 +772,13 @@ __global__ __launch_bounds__(RT) void k_accept_k(float2* __restrict__ Ax, const 
   }
 }
 
-// g += TV gradient at x; partials: TV energy, |g|^2, Re<g_prev, g>, Re<g, d>.
+// g += TV gradient at x; partials: TV energy, |g|^2, Re<g_prev, g>, Re<g, d>.  With a diagonal
+// preconditioner D (null: identity) the first two are weighted by 1 / D (graspcg/ops/dot.py:5-71
+// with diag, as the direction rules use them, graspcg/numerics/directions.py:75-162).
 __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, float2* __restrict__ g,
                                                 const float2* __restrict__ d, const float2* __restrict__ gp,
-                                                TVDev tv, int64_t npix_p, const ProbState* __restrict__ st,
-                                                int prime, double* part) {
+                                                const float* __restrict__ D, TVDev tv, int64_t npix_p,
+                                                const ProbState* __restrict__ st, int prime, double* part) {
   const int p = blockIdx.y;
   double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   if (prime || st[p].status == MRH_CG_ACTIVE) {
@@ -791,8 +794,9 @@ __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, fl
       const float2 xv = x[n];
       v[4] += (double)xv.x * xv.x + (double)xv.y * xv.y;
       v[0] += (double)e;
-      v[1] += (double)gv.x * gv.x + (double)gv.y * gv.y;
-      if (gp) { const float2 q = gp[n]; v[2] += (double)q.x * gv.x + (double)q.y * gv.y; }
+      const double wd = D ? 1.0 / (double)D[n] : 1.0;
+      v[1] += ((double)gv.x * gv.x + (double)gv.y * gv.y) * wd;
+      if (gp) { const float2 q = gp[n]; v[2] += ((double)q.x * gv.x + (double)q.y * gv.y) * wd; }
       if (d) { const float2 q = d[n]; v[3] += (double)q.x * gv.x + (double)q.y * gv.y; }
     }
   }
@@ -848,10 +852,11 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
   s.have_f_prev = 1;
 }
 
-// d = beta d - g (prime: d = -g); PR+ keeps g in g_prev.
+// d = beta d - g / D (prime: d = -g / D; D null: identity); PR+ keeps g in g_prev.
 __global__ __launch_bounds__(RT) void k_cg_direction(float2* __restrict__ d, const float2* __restrict__ g,
-                                                     float2* __restrict__ gp, int64_t npix_p,
-                                                     const ProbState* __restrict__ st, int prime) {
+                                                     float2* __restrict__ gp, const float* __restrict__ D,
+                                                     int64_t npix_p, const ProbState* __restrict__ st,
+                                                     int prime) {
   const int p = blockIdx.y;
   if (!prime && st[p].status != MRH_CG_ACTIVE) return;
   const float beta = prime ? 0.f : (float)st[p].beta;
@@ -859,10 +864,53 @@ __global__ __launch_bounds__(RT) void k_cg_direction(float2* __restrict__ d, con
     const int64_t n = (int64_t)p * npix_p + i;
     const float2 gv = g[n];
     float2 dv = prime ? make_float2(0.f, 0.f) : d[n];
-    dv.x = beta * dv.x - gv.x;
-    dv.y = beta * dv.y - gv.y;
+    if (D) {
+      const float dn = D[n];
+      dv.x = beta * dv.x - gv.x / dn;
+      dv.y = beta * dv.y - gv.y / dn;
+    } else {
+      dv.x = beta * dv.x - gv.x;
+      dv.y = beta * dv.y - gv.y;
+    }
     d[n] = dv;
     if (gp) gp[n] = gv;
+  }
+}
+
+// Diagonal majoriser of the regularisers at x (graspcg/regularization/manager.py:145-218), the
+// preconditioner D of the direction rules.  D starts at 1 (the workspace's ones,
+// graspcg/solvers/cg.py:95); per TV term, in order:
+//   a term over the frame axis adds its axis profile only (tv_nd.py:181-215): 2 lambda w_0^2,
+//     twice that on interior frames (the 1-D Laplacian degree);
+//   any other term adds the lagged-diffusivity weights (tv_nd.py:113-162): per axis a,
+//     lambda w_a^2 / sqrt(|D_a x|^2 + eps^2) at the pixel (D_a the unweighted forward
+//     difference, zero on the last slice), twice that away from the axis' two ends.
+__device__ __forceinline__ float axis_degree(int pos, int n) { return (n > 2 && pos > 0 && pos < n - 1) ? 2.f : 1.f; }
+
+__global__ __launch_bounds__(RT) void k_build_diag(const float2* __restrict__ x, float* __restrict__ D, TVDev tv,
+                                                   int64_t npix_p) {
+  const int p = blockIdx.y;
+  const int plane = tv.S0 * tv.S1;
+  for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
+    const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
+    const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
+    const int64_t n = (int64_t)p * npix_p + i;
+    float dv = 1.f;
+    for (int k = 0; k < tv.n; ++k) {
+      const TVTermDev& tm = tv.t[k];
+      if (tm.lam == 0.f || tm.axes == 0) continue;
+      if (tm.axes & 1) {
+        dv += 2.f * tm.lam * tm.w[0] * tm.w[0] * axis_degree(f, tv.F);
+        continue;
+      }
+      for (int a = 1; a < 3; ++a) {
+        if (!(tm.axes & (1 << a))) continue;
+        const float2 gi = fdiff(x, n, f, ix, iy, a, tv, 1.f);
+        const float wi = 1.f / sqrtf(gi.x * gi.x + gi.y * gi.y + tm.eps * tm.eps);
+        dv += tm.lam * (tm.w[a] * tm.w[a]) * wi * axis_degree(a == 1 ? ix : iy, a == 1 ? tv.S0 : tv.S1);
+      }
+    }
+    D[n] = dv;
   }
 }
 
@@ -895,6 +943,7 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   e->direction = d->direction; e->ls_max_iter = d->ls_max_iter;
   e->c1 = d->c1; e->ls_beta = d->ls_beta; e->rel_tol = d->rel_tol; e->restart_c = d->restart_c;
   e->ls_kind = d->ls_kind; e->c2 = d->c2;
+  e->d_diag = d->diag;
   e->max_hist = std::max(1, d->max_hist);
   e->npix_p = (int64_t)e->F * e->S0 * e->S1;
   int64_t kmax = 0;
@@ -1016,7 +1065,8 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
   MRH_REQUIRE(e && x && g, MRH_EINVAL, "cg_gradient: null argument");
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_cg_grad, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
-                     (const float2*)d, (const float2*)g_prev, e->tv, e->npix_p, e->d_st, prime, e->d_part_i);
+                     (const float2*)d, (const float2*)g_prev, (const float*)e->d_diag, e->tv, e->npix_p, e->d_st,
+                     prime, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
                      e->direction, e->rel_tol, e->restart_c, e->d_target, e->d_hist, e->max_hist, e->P,
@@ -1028,7 +1078,16 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
 extern "C" int mrh_cg_direction(mrh_cg* e, void* d, const void* g, void* g_prev, int prime, void* stream) {
   MRH_REQUIRE(e && d && g, MRH_EINVAL, "cg_direction: null argument");
   hipLaunchKernelGGL(k_cg_direction, dim3(e->nb_i, e->P), dim3(RT), 0, (hipStream_t)stream, (float2*)d,
-                     (const float2*)g, (float2*)g_prev, e->npix_p, e->d_st, prime);
+                     (const float2*)g, (float2*)g_prev, (const float*)e->d_diag, e->npix_p, e->d_st, prime);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_cg_build_diag(mrh_cg* e, const void* x, void* stream) {
+  MRH_REQUIRE(e && x, MRH_EINVAL, "cg_build_diag: null argument");
+  MRH_REQUIRE(e->d_diag, MRH_EINVAL, "cg_build_diag: the engine was created without a diag buffer");
+  hipLaunchKernelGGL(k_build_diag, dim3(e->nb_i, e->P), dim3(RT), 0, (hipStream_t)stream, (const float2*)x,
+                     e->d_diag, e->tv, e->npix_p);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

@@ -53,7 +53,8 @@ class NLCGEngine:
     def __init__(self, plan, y: torch.Tensor, *, problems: int, frames: int, regm: Optional[RegManager],
                  direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5, ls_max_iter: int = 20,
                  rel_tol: float = 0.0, max_hist: int = 256, restart_c: Optional[float] = None,
-                 skip_stopped: bool = True, ls_name: str = "armijo", c2: float = 0.9):
+                 skip_stopped: bool = True, ls_name: str = "armijo", c2: float = 0.9,
+                 precond: bool = False):
         if direction not in _DIRECTIONS:
             raise ValueError(f"direction must be one of {sorted(_DIRECTIONS)}")
         if ls_name not in _SEARCHES:
@@ -89,6 +90,10 @@ class NLCGEngine:
         desc.restart_c = -1.0 if restart_c is None else float(restart_c)
         desc.ls_kind, desc.c2 = _SEARCHES[ls_name], float(c2)
         self.ls_name = ls_name
+        # diagonal preconditioner D (the reference workspace's "diag"): the regularisers'
+        # majoriser at x0, built by prime() (RegManager.add_diag); None: D = 1
+        self.diag = torch.ones(shape, dtype=torch.float32, device=dev) if precond else None
+        desc.diag = self.diag.data_ptr() if precond else None
         if restart_c is not None and restart_c < 0:
             raise ValueError("restart_c must be >= 0 (None keeps the reference behaviour)")
         self.max_hist = int(max_hist)
@@ -110,12 +115,15 @@ class NLCGEngine:
         return _lib.stream_of(self.x)
 
     def prime(self, x0: Optional[torch.Tensor] = None, Ax0: Optional[torch.Tensor] = None) -> None:
-        """Start a run at x0 (or the current x): g = grad f(x0), d = -g, Ad = A d.
-        ``Ax0`` (= A x0, already computed by the caller) saves the forward NUFFT."""
+        """Start a run at x0 (or the current x): g = grad f(x0), d = -g / D, Ad = A d.
+        ``Ax0`` (= A x0, already computed by the caller) saves the forward NUFFT.  With a
+        preconditioner, D is rebuilt here from x0 and kept for the run."""
         lib, e = self._lib, self.handle
         if x0 is not None:
             self.x.copy_(x0.reshape(self.x.shape))
         _lib.check(lib.mrh_cg_reset(e, self._st()))
+        if self.diag is not None:
+            _lib.check(lib.mrh_cg_build_diag(e, self.x.data_ptr(), self._st()))
         if Ax0 is not None:
             self.Ax.copy_(Ax0.reshape(self.Ax.shape))
         else:
@@ -232,6 +240,8 @@ class NLCGEngine:
 class CGConfig:
     """graspcg/solvers/cg.py:30-52, plus ``rel_tol`` (relative objective-change stop rule),
     ``restart_c`` (beta = 0 restart when the next direction would not descend; None = reference),
+    ``precond`` (the regularisers' diagonal majoriser at x0 as the preconditioner D of the direction
+    rules, built before the prime step; False = the reference's D = 1),
     ``use_graph`` (HIP-graph replay of the iteration) and ``problems`` (independent slices)."""
     devices: Optional[object] = None
     direction: str = "fr"
@@ -250,6 +260,7 @@ class CGConfig:
     verbose: bool = False
     rel_tol: Optional[float] = None
     restart_c: Optional[float] = None
+    precond: bool = False
     use_graph: bool = True
     problems: int = 1
 
@@ -266,6 +277,8 @@ class Workspace:
                    "fwd_dir_sh": eng.Ad, "resid_sh": eng.r}
         if eng.g_prev is not None:
             self._b["g_prev"] = eng.g_prev
+        if eng.diag is not None:
+            self._b["diag"] = eng.diag
 
     def has(self, name: str) -> bool:
         return self._ALIAS.get(name, name) in self._b
@@ -299,7 +312,7 @@ class CGSolver:
                                  ls_beta=self.cfg.ls_beta, ls_max_iter=self.cfg.ls_max_iter,
                                  rel_tol=self.cfg.rel_tol or 0.0, max_hist=max(64, self.cfg.max_iter),
                                  restart_c=self.cfg.restart_c, ls_name=self.cfg.ls_name.lower(),
-                                 c2=self.cfg.c2)
+                                 c2=self.cfg.c2, precond=self.cfg.precond)
         self.ws = Workspace(self.engine)
         if self.cfg.init_mode == "zero":
             self.engine.x.zero_()

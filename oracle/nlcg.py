@@ -22,7 +22,8 @@ included, on any operator exposing ``forward`` / ``adjoint`` over
   (:205-207), stats norms (:213-220), the reference convergence test
   (graspcg/solvers/utils.py:40-43), direction update FR with the first beta = 1
   quirk (graspcg/numerics/directions.py:141-162), PR+ (:75-109) or Dai-Yuan
-  (:111-139), D = 1 (cg.py:95).
+  (:111-139), D = 1 (cg.py:95) or, with ``precond``, the regularisers' diagonal
+  majoriser (graspcg/regularization/manager.py:145-218, tv_nd.py:113-215).
 
 Pinning (tests/test_oracle_nlcg.py): closed-form quadratic with a unitary
 operator (graspcg/tests/test_numerics_ns.py:110-194), rejection of an ascent
@@ -35,7 +36,10 @@ in general; strong Wolfe by the reference's zoom-to-1/2 case
 (graspcg/tests/test_numerics_ns.py:195-225), hand-traced steps of the expansion
 and of both zoom branches on scaled quadratics, its trial limits, and the two
 Wolfe conditions along TV-regularised runs of every direction rule
-(graspcg/tests/test_cg_descend_compile_ns.py:125-176).
+(graspcg/tests/test_cg_descend_compile_ns.py:125-176); the diagonal majoriser by
+hand-evaluated entries (constant image, step edge, axis weight, single-frame
+profile) and the preconditioned rules by two properties: a constant D = 2 only
+doubles the accepted steps, and DY's identity holds with D from the majoriser.
 """
 from __future__ import annotations
 
@@ -107,6 +111,42 @@ def tv_energy_grad(x: np.ndarray, reg: TV, g: Optional[np.ndarray] = None):
         div = sum(bwd_diff(pi, ax) for pi, ax in zip(flux, reg.axes))
         g += (-div) * lam
     return (e * lam).sum(dtype=rdt)
+
+
+def _degree(n: int) -> np.ndarray:
+    """Ends of an axis count once, interior positions twice (tv_nd.py:150-159, :207-209)."""
+    v = np.ones(n)
+    if n > 2:
+        v[1:-1] = 2.0
+    return v
+
+
+def majoriser_diag(x: np.ndarray, regs: Sequence[TV]) -> np.ndarray:
+    """Diagonal preconditioner of graspcg/regularization/manager.py:145-218 for one problem (F, S0, S1).
+
+    Starts from the workspace's ones (graspcg/solvers/cg.py:95).  A term whose axes include the
+    frame axis contributes its axis profile only (TVND.majorizer_profile, tv_nd.py:181-215):
+    2 lambda w_0^2 times the frame degree.  Any other term adds the lagged-diffusivity weights
+    (TVND.add_diag, tv_nd.py:113-162): per axis, lambda w_a^2 / sqrt(|D_a x|^2 + eps^2) with D_a
+    the unweighted forward difference (zero on the last slice), times the position's degree."""
+    rdt = x.real.dtype
+    D = np.ones(x.shape, dtype=rdt)
+    for reg in regs:
+        if reg.weight == 0.0 or len(reg.axes) == 0:
+            continue
+        ws = list(reg.axis_weights) if reg.axis_weights is not None else [1.0] * len(reg.axes)
+        lam, eps = rdt.type(reg.weight), rdt.type(reg.eps)
+        if 0 in reg.axes:
+            w0 = ws[list(reg.axes).index(0)]
+            D += (rdt.type(2.0 * reg.weight * w0 * w0) * _degree(x.shape[0]).astype(rdt)).reshape(-1, 1, 1)
+            continue
+        for ax, w in zip(reg.axes, ws):
+            gi = fwd_diff(x, ax)
+            wi = rdt.type(1.0) / np.sqrt(_abs2(gi) + eps * eps)
+            shape = [1] * x.ndim
+            shape[ax] = -1
+            D += lam * rdt.type(w * w) * wi * _degree(x.shape[ax]).astype(rdt).reshape(shape)
+    return D
 
 
 # --------------------------------------------------------------- objective
@@ -242,7 +282,8 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
          max_iter: int = 50, direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5,
          ls_max_iter: int = 20, tol_g: float = 1e-4, tol_step: float = 1e-9,
          rel_tol: Optional[float] = None, mutate_x: bool = True,
-         restart_c: Optional[float] = None, ls_name: str = "armijo", c2: float = 0.9) -> CGResult:
+         restart_c: Optional[float] = None, ls_name: str = "armijo", c2: float = 0.9,
+         precond: bool = False) -> CGResult:
     """Reference-semantics NLCG (graspcg/solvers/cg.py:162-246) with Armijo search, or the
     strong-Wolfe search with ``ls_name="wolfe"`` (line_search.py:12-17, CGConfig.ls_name / c2
     at cg.py:37-39).
@@ -252,16 +293,23 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
     ``restart_c`` (not a reference rule) sets beta = 0 whenever the new direction
     -g + beta d would have Re<g, d_new> > -restart_c |g|^2, so every searched
     direction descends; None keeps the reference behaviour (such a direction
-    ends the run with "not_descent")."""
+    ends the run with "not_descent").  ``precond`` builds the diagonal majoriser D of the
+    regularisers at x0 (``majoriser_diag``, RegManager.add_diag) before the prime step and runs
+    the Jacobi-preconditioned rules of graspcg/numerics/directions.py: d = beta d - g / D, the
+    beta numerators / FR and PR+ denominators and the reported gradient norm weighted by 1 / D
+    (ops/dot.py:5-71 with diag, solvers/utils.py:29-37); D stays fixed for the run."""
     x = np.array(x0, copy=True)
     rdt = x.real.dtype
     obj = Objective(A, AH, y, regs, mutate_x=mutate_x)
     d = np.zeros_like(x)
     res = CGResult(x=x)
-    # prime (cg.py:178-190)
+    Dinv = (rdt.type(1.0) / majoriser_diag(x, regs)) if precond else None
+    pdot = ((lambda a, b: rdt.type(np.sum((a.conj() * (b * Dinv)).real))) if precond else
+            (lambda a, b: rdt.type(np.vdot(a.reshape(-1), b.reshape(-1)).real)))
+    # prime (cg.py:178-190): d = -g / D
     obj.begin_linesearch(x, d)
     _, _, g = obj.f_g(x, d, 0.0)
-    d = -g
+    d = -g * Dinv if precond else -g
     rho_prev = None
     g_prev = g.copy()
     f_prev = None
@@ -279,7 +327,7 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
             break
         x += d * t
         xn, sn = _norm(x, rdt), _norm(d, rdt)
-        gn = _norm(g, rdt)
+        gn = rdt.type(math.sqrt(max(float(pdot(g, g)), 0.0)))
         res.history.append(dict(iter=k, f=float(f0), gdot=float(g0d), gdot_at_step=float(gdt),
                                 step=float(t), f_step=float(ft), xnorm=float(xn),
                                 gnorm=float(gn), stepnorm=float(sn)))
@@ -293,15 +341,15 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
             break
         f_prev = ft
         # direction update (directions.py)
-        rho = rdt.type(np.vdot(g.reshape(-1), g.reshape(-1)).real)
+        rho = pdot(g, g)
         if direction == "fr":
             if rho_prev is None:
                 rho_prev = rho
             beta = rho / max(rho_prev, rdt.type(1e-20))
             rho_prev = rho
         elif direction == "prplus":
-            num = rho - rdt.type(np.vdot(g_prev.reshape(-1), g.reshape(-1)).real)
-            den = rdt.type(np.vdot(g_prev.reshape(-1), g_prev.reshape(-1)).real)
+            num = rho - pdot(g_prev, g)
+            den = pdot(g_prev, g_prev)
             beta = max(num / max(den, rdt.type(1e-20)), rdt.type(0.0))
             g_prev = g.copy()
         elif direction == "dy":
@@ -317,7 +365,7 @@ def nlcg(A: Callable, AH: Callable, y: np.ndarray, x0: np.ndarray, regs: Sequenc
             if -rho + beta * gd_old > -restart_c * rho:
                 beta = rdt.type(0.0)
         res.history[-1]["beta"] = float(beta)
-        d = d * rdt.type(beta) - g
+        d = d * rdt.type(beta) - (g * Dinv if precond else g)
     if not res.stop:
         res.stop = "max_iter"
     res.x = x

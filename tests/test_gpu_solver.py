@@ -12,7 +12,7 @@ import torch
 from mrirecon_amd import CGConfig, CGSolver, NLCGEngine, RegManager, TVND, TVParams, _lib, synth
 from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
 from oracle.kbnufft import KBNufft, rel_l2
-from oracle.nlcg import TV, nlcg, objective_value, tv_energy_grad
+from oracle.nlcg import TV, majoriser_diag, nlcg, objective_value, tv_energy_grad
 
 pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
@@ -96,19 +96,20 @@ def regs_for(wl):
     return RegManager(g), o
 
 
-def run_pair(wl, n_iter, direction="fr", use_graph=True, rel_tol=None, ls_name="armijo", c2=0.9):
+def run_pair(wl, n_iter, direction="fr", use_graph=True, rel_tol=None, ls_name="armijo", c2=0.9, precond=False):
     be, ora = prepare(wl)
     regm, oregs = regs_for(wl)
     x0 = ora.adjoint(wl.y)
     eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, direction=direction,
-                     rel_tol=rel_tol or 0.0, ls_name=ls_name, c2=c2)
+                     rel_tol=rel_tol or 0.0, ls_name=ls_name, c2=c2, precond=precond)
     eng.prime(c64(x0))
     eng.run(n_iter, use_graph=use_graph)
     torch.cuda.synchronize()
     hist = eng.history()[0]
     x_gpu = eng.x.cpu().numpy()
     ref = nlcg(ora.forward, ora.adjoint, wl.y.astype(np.complex128), x0.astype(np.complex128), oregs,
-               max_iter=n_iter, direction=direction, mutate_x=False, rel_tol=rel_tol, ls_name=ls_name, c2=c2)
+               max_iter=n_iter, direction=direction, mutate_x=False, rel_tol=rel_tol, ls_name=ls_name, c2=c2,
+               precond=precond)
     f_gpu = objective_value(ora.forward, wl.y, x_gpu.astype(np.complex128), oregs)
     return eng, hist, ref, f_gpu
 
@@ -193,6 +194,51 @@ def test_wolfe_small_steps_zoom_deep():
     eng, hist, ref, f_gpu = run_pair(wl, 6, ls_name="wolfe", c2=0.1)
     steps = check_wolfe_run(hist, ref, f_gpu, 0.1, "deep zoom")
     assert min(steps) < 2.0 ** -8
+
+
+def test_majoriser_diag_matches_oracle():
+    """mrh_cg_build_diag (via prime) == the oracle's RegManager.add_diag restatement: temporal profile,
+    an anisotropic lagged-diffusivity term with axis weights, and a term over frames and image axes
+    together (its frame profile only).  (The isotropic spatial term is covered by the preconditioned
+    NLCG runs below.)"""
+    wl = small_dynamic(seed=81, F=4)
+    be, ora = prepare(wl)
+    regs = [TVND("t", TVParams(weight=0.02, axes="temporal")),
+            TVND("a", TVParams(weight=0.001, axes="spatial", isotropic=False, eps=2e-3, axis_weights=[0.5, 2.0])),
+            TVND("m", TVParams(weight=0.004, axes=(0, 2, 3), axis_weights=[1.5, 1.0, 1.0]))]
+    oregs = [TV(0.02, (0,)), TV(0.001, (1, 2), isotropic=False, eps=2e-3, axis_weights=[0.5, 2.0]),
+             TV(0.004, (0, 1, 2), axis_weights=[1.5, 1.0, 1.0])]
+    x0 = crand((wl.B, wl.S, wl.S), 82) * 0.01
+    x0[:, :10, :] = 0.25        # flat rows: weights near 1 / eps
+    eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=RegManager(regs), precond=True)
+    eng.prime(c64(x0))
+    torch.cuda.synchronize()
+    D = eng.diag.cpu().numpy().astype(np.float64)
+    ref = majoriser_diag(x0.astype(np.complex128), oregs)
+    err = np.max(np.abs(D - ref) / ref)
+    print(f"[majoriser] max rel err {err:.2e}, D in [{ref.min():.3f}, {ref.max():.3f}]")
+    assert err <= 2e-6
+    assert ref.max() > 5.0 and ref.min() < 1.2     # flat rows and edges both present
+
+
+@pytest.mark.parametrize("direction", ["fr", "prplus", "dy"])
+def test_preconditioned_nlcg_matches_oracle(direction):
+    """Directions preconditioned by the regularisers' majoriser at x0 (d = beta d - g / D, 1/D-weighted
+    beta inner products): accepted steps equal the oracle's, objectives within 1e-4."""
+    wl = small_dynamic(seed=71)
+    eng, hist, ref, f_gpu = run_pair(wl, 12, direction=direction, precond=True)
+    steps = [h["step"] for h in hist]
+    rel = abs(f_gpu - ref.f_final) / ref.f_final
+    print(f"[precond {direction}] f_gpu={f_gpu:.8e} f_ref={ref.f_final:.8e} rel={rel:.2e} steps={steps} "
+          f"stop={ref.stop}")
+    assert [h["step"] for h in ref.history] == steps
+    fh = np.array([h["f"] for h in hist])
+    fr = np.array([h["f"] for h in ref.history])
+    assert np.max(np.abs(fh - fr) / np.abs(fr)) <= 1e-4
+    assert rel <= 1e-4
+    gn = np.array([h["gnorm"] for h in hist])
+    gr = np.array([h["gnorm"] for h in ref.history])
+    assert np.max(np.abs(gn - gr) / gr) <= 1e-3      # the 1/D-weighted gradient norm
 
 
 def test_small_steps_take_the_second_line_search_pass():
@@ -308,6 +354,9 @@ def test_cgsolver_reference_surface():
     assert solver.ws.get("x").data_ptr() == solver.ws.get("var").data_ptr()
     with pytest.raises(ValueError):
         CGSolver(y, be, regm, CGConfig(ls_name="more-thuente"))
+    psolver = CGSolver(y, be, regm, CGConfig(max_iter=3, init_mode="backproj", precond=True, restart_c=0.0))
+    psolver.run()
+    assert len(psolver.history()) == 3 and float(psolver.ws.get("diag").min()) >= 1.0
     wsolver = CGSolver(y, be, regm, CGConfig(max_iter=4, init_mode="backproj", ls_name="wolfe", c2=0.5))
     wsolver.run()
     for h in wsolver.history():

@@ -12,7 +12,8 @@ import math
 import numpy as np
 import pytest
 
-from oracle.nlcg import TV, Objective, armijo, bwd_diff, fwd_diff, nlcg, objective_value, tv_energy_grad, wolfe
+from oracle.nlcg import (TV, Objective, armijo, bwd_diff, fwd_diff, majoriser_diag, nlcg, objective_value,
+                         tv_energy_grad, wolfe)
 
 
 def fft_op():
@@ -269,3 +270,69 @@ def test_wolfe_runs_satisfy_strong_conditions(direction):
     for h in res.history:
         assert h["f_step"] <= h["f"] + 1e-4 * h["step"] * h["gdot"] + 1e-12
         assert abs(h["gdot_at_step"]) <= 0.9 * abs(h["gdot"]) + 1e-12
+
+
+# ------------------------------------------------------------------ diagonal majoriser preconditioner
+
+def test_majoriser_of_a_constant_image():
+    """x constant: every difference is 0, so the lagged-diffusivity weight is 1 / eps.  Temporal term
+    (lambda 0.02, profile 2 lambda = 0.04 per frame, doubled inside) + isotropic spatial term
+    (lambda 0.002, eps 1e-3: 2 per axis end, 4 inside); hand-evaluated entries."""
+    x = np.full((4, 3, 5), 1.5 - 0.5j)
+    D = majoriser_diag(x, [TV(0.02, (0,)), TV(0.002, (1, 2))])
+    assert D.shape == x.shape
+    assert abs(D[0, 0, 0] - 5.04) <= 1e-12       # 1 + 0.04 + 2 + 2
+    assert abs(D[1, 1, 2] - 9.08) <= 1e-12       # 1 + 0.08 + 4 + 4
+    assert abs(D[3, 1, 0] - 7.04) <= 1e-12       # 1 + 0.04 + 4 + 2
+    assert abs(D[2, 2, 4] - 5.08) <= 1e-12       # 1 + 0.08 + 2 + 2
+
+
+def test_majoriser_of_a_step_edge():
+    """One column x = (0, 0, 3, 3) down axis 1, lambda 1, eps 4: the differences (0, 3, 0, 0) give weights
+    1/4, 1/5, 1/4, 1/4, times the degrees (1, 2, 2, 1); axis 2 has one sample (difference 0, degree 1,
+    weight 1/4).  D = 1 + (0.25, 0.4, 0.5, 0.25) + 0.25.  An axis weight of 2 multiplies a term by 4;
+    a temporal term over the single frame adds 2 lambda."""
+    x = np.array([0.0, 0.0, 3.0, 3.0]).reshape(1, 4, 1).astype(complex)
+    D = majoriser_diag(x, [TV(1.0, (1, 2), eps=4.0)])
+    assert np.allclose(D.reshape(-1), [1.5, 1.65, 1.75, 1.5], rtol=0, atol=1e-12)
+    D2 = majoriser_diag(x, [TV(1.0, (1,), eps=4.0, axis_weights=[2.0]), TV(0.5, (0,))])
+    assert np.allclose(D2.reshape(-1), [3.0, 3.6, 4.0, 3.0], rtol=0, atol=1e-12)
+
+
+def test_constant_preconditioner_only_rescales_the_step():
+    """A temporal term over a single frame has no energy or gradient but a majoriser of 2 lambda, so
+    lambda = 0.5 gives D = 2 everywhere: every preconditioned direction is half the plain one, and
+    backtracking from t = 1 then accepts twice the plain step whenever the plain step is at most 1/2
+    (FR and PR+ betas are unchanged by a constant D)."""
+    rng = np.random.default_rng(5)
+    n, m = 6, 40
+    M = 2.0 * (rng.standard_normal((m, n * n)) + 1j * rng.standard_normal((m, n * n))) / math.sqrt(m)
+    A = lambda x: M @ x.reshape(-1)
+    AH = lambda y: (M.conj().T @ y).reshape(1, n, n)
+    y = A(rand_c((1, n, n), 6))
+    regs = [TV(0.5, (0,))]
+    for direction in ("fr", "prplus"):
+        plain = nlcg(A, AH, y, np.zeros((1, n, n), complex), regs, max_iter=8, direction=direction)
+        pre = nlcg(A, AH, y, np.zeros((1, n, n), complex), regs, max_iter=8, direction=direction, precond=True)
+        assert max(h["step"] for h in plain.history) <= 0.5
+        assert [2 * h["step"] for h in plain.history] == [h["step"] for h in pre.history]
+        for a, b in zip(plain.history, pre.history):
+            assert abs(a["f"] - b["f"]) <= 1e-12 * abs(a["f"])
+        assert abs(plain.f_final - pre.f_final) <= 1e-12 * abs(plain.f_final)
+
+
+def test_preconditioned_dai_yuan_identity_and_descent():
+    """With D from the majoriser (not constant), DY's beta = (g^T D^-1 g) / Re<d, g - g_prev> with
+    d_k = beta d_(k-1) - D^-1 g_k still gives Re<g_k, d_k> = beta_k Re<g_(k-1), d_(k-1)>; an unweighted
+    numerator, or an unpreconditioned update, would not.  The run descends and keeps Armijo."""
+    A, AH, y, x0, regs = dense_problem(23)
+    x0 = rand_c(x0.shape, 24) * 0.1
+    res = nlcg(A, AH, y, x0, regs, max_iter=8, direction="dy", restart_c=0.0, precond=True)
+    plain = nlcg(A, AH, y, x0, regs, max_iter=8, direction="dy", restart_c=0.0)
+    assert res.n_iter >= 6
+    h = res.history
+    rel = [abs(h[k + 1]["gdot"] - h[k]["beta"] * h[k]["gdot"]) / abs(h[k + 1]["gdot"]) for k in range(res.n_iter - 1)]
+    assert max(rel) <= 1e-9, rel
+    assert all(r["gdot"] < 0 for r in h)
+    assert [r["step"] for r in h] != [r["step"] for r in plain.history] or \
+        any(abs(a["f"] - b["f"]) > 1e-9 * abs(b["f"]) for a, b in zip(h, plain.history))
