@@ -201,6 +201,13 @@ typedef struct {
   float* diag;              /* device, P*F*S0*S1 floats: the diagonal preconditioner D of the
                                direction rules (the workspace's "diag", cg.py:95), read by
                                mrh_cg_gradient / mrh_cg_direction; NULL: D = 1 */
+  double gl_weight;         /* graph-Laplacian term along frames (graspcg/regularization/
+                               graph_laplacian.py:22-321): lambda; 0 = no term          */
+  int gl_F;                 /* frames of its graph (= tv.F, at most 48)               */
+  const float* gl_L;        /* host F*F row-major Laplacian (plain / rw / sym, from the
+                               symmetrised graph)                                      */
+  const float* gl_prof;     /* host F: its diagonal majoriser per frame (lambda deg or
+                               lambda [deg > 0], graph_laplacian.py:283-321)            */
 } mrh_cg_desc;
 
 /* Per-problem, per-iteration record copied out by mrh_cg_read_history. */

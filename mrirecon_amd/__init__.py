@@ -9,8 +9,8 @@ in hand-written gfx950 kernels behind the C ABI of include/mrirecon_hip.h.
 from .roles import Roles, IMAGE_ROLES
 from .nufft.layout import AxisSpec
 from .nufft.api import NUFFT, NUFFTConfig
-from .regularization import TVND, TVParams, RegManager
+from .regularization import TVND, TVParams, RegManager, GraphLaplacian, GraphLapParams
 from .solver import CGSolver, CGConfig, NLCGEngine
 
-__all__ = ["NUFFT", "NUFFTConfig", "AxisSpec", "TVND", "TVParams", "RegManager", "CGSolver",
+__all__ = ["NUFFT", "NUFFTConfig", "AxisSpec", "TVND", "TVParams", "RegManager", "GraphLaplacian", "GraphLapParams", "CGSolver",
            "CGConfig", "NLCGEngine", "Roles", "IMAGE_ROLES"]

@@ -25,6 +25,9 @@
 #define PI_STRIDE 32      // doubles per block partial (image pass)
 #define PK_STRIDE 4       // doubles per block partial (k-space pass)
 #define NW 7              // strong-Wolfe candidates per image pass
+#define PQ_STRIDE 4       // doubles per block partial (graph-Laplacian passes)
+#define GL_MAXF 48        // frames per problem of a graph-Laplacian term (LDS of k_gl_ls <= 64 KiB)
+#define GL_TILE 64        // pixels per workgroup tile of the graph-Laplacian passes
 
 // ---------------------------------------------------------------- block reduction
 template <int NV>
@@ -303,9 +306,26 @@ extern "C" int mrh_tv_energy_grad(const mrh_tv_desc* desc, const void* x, void* 
   return MRH_OK;
 }
 
+// ---------------------------------------------------------------- graph Laplacian
+// Graph-Laplacian term along the frame axis (graspcg/regularization/graph_laplacian.py:22-321):
+// E = (lambda / 2) sum_p Re<x[:, p], (L x)[:, p]>, gradient lambda L x, L (F x F) the plain,
+// random-walk or symmetric Laplacian of the term's symmetrised graph (built on the host).
+// A workgroup stages L and a tile of GL_TILE pixels x F frames in LDS (dynamic shared memory);
+// wavefront w computes rows w, w + 4, ... of L x for its lane's pixel.  Along d the energy is
+// the quadratic (lambda / 2)(a + t (b1 + b2) + t^2 c), so a line search needs one pass.
+struct GLDev {
+  int F;
+  float lam;          // 0: no term
+  const float* L;     // F x F, row-major
+  const float* prof;  // F: the term's diagonal majoriser per frame
+};
+
+__host__ __device__ constexpr int gl_lds_floats(int F, int tiles) { return ((F * F + 1) & ~1) + tiles * 2 * F * GL_TILE; }
+
 // ---------------------------------------------------------------- NLCG engine
 struct ProbState {
   double a, b, c, g0d, dd;
+  double gq_a, gq_b1, gq_b2, gq_c;   // graph term along d: Re<x,Lx>, Re<d,Lx>, Re<x,Ld>, Re<d,Ld>
   double tv[NT];
   double f0, t, ft, rho, rho_prev, beta, gpg, freg, gdot_step, f_prev;
   // strong-Wolfe search: candidates of the next image pass, the expansion's next step and
@@ -331,7 +351,118 @@ struct mrh_cg {
   double* d_target;     // [P] objective targets (-inf: none)
   int* d_active;        // [P] 1 while the problem iterates (read by the masked NUFFT entry points)
   float* d_diag;        // caller's diagonal preconditioner (P F S0 S1 floats) or null
+  GLDev gl;             // graph-Laplacian term (gl.lam == 0: none); L and its profile in d_gl
+  float* d_gl;
+  double* d_part_q;     // [P][nb_q][PQ_STRIDE] graph-term partials
+  int nb_q;
+  size_t gl_lds_grad, gl_lds_ls;   // dynamic LDS bytes of k_gl_grad / k_gl_ls
 };
+
+// g += lambda L x; block partial: E.
+__global__ __launch_bounds__(256) void k_gl_grad(const float2* __restrict__ x, float2* __restrict__ g, GLDev gl,
+                                                 int plane, int64_t npix_p, const ProbState* __restrict__ st,
+                                                 int prime, double* part) {
+  extern __shared__ float gl_lds[];
+  const int F = gl.F, p = blockIdx.y;
+  float* Ls = gl_lds;
+  float2* Xs = reinterpret_cast<float2*>(gl_lds + ((F * F + 1) & ~1));
+  double v[1] = {0.0};
+  if (prime || st[p].status == MRH_CG_ACTIVE) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < F * F; i += 256) Ls[i] = gl.L[i];
+    const float2* xp = x + (int64_t)p * npix_p;
+    float2* gp = g + (int64_t)p * npix_p;
+    for (int t0 = blockIdx.x * GL_TILE; t0 < plane; t0 += gridDim.x * GL_TILE) {
+      const int pix = t0 + lane;
+      const bool ok = pix < plane;
+      __syncthreads();                       // L staged / the previous tile consumed
+      for (int f = wv; f < F; f += 4)
+        Xs[f * GL_TILE + lane] = ok ? xp[(int64_t)f * plane + pix] : make_float2(0.f, 0.f);
+      __syncthreads();
+      if (ok) {
+        for (int i = wv; i < F; i += 4) {
+          const float* Li = Ls + i * F;
+          float yr = 0.f, yi = 0.f;
+          for (int j = 0; j < F; ++j) {
+            const float l = Li[j];
+            const float2 xj = Xs[j * GL_TILE + lane];
+            yr = fmaf(l, xj.x, yr);
+            yi = fmaf(l, xj.y, yi);
+          }
+          const float2 xi = Xs[i * GL_TILE + lane];
+          v[0] += 0.5 * (double)gl.lam * ((double)xi.x * yr + (double)xi.y * yi);
+          const int64_t n = (int64_t)i * plane + pix;
+          float2 gv = gp[n];
+          gv.x += gl.lam * yr;
+          gv.y += gl.lam * yi;
+          gp[n] = gv;
+        }
+      }
+    }
+  }
+  block_reduce_store<1>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PQ_STRIDE);
+}
+
+// Line-search pass: block partials of Re<x, Lx>, Re<d, Lx>, Re<x, Ld>, Re<d, Ld>.
+__global__ __launch_bounds__(256) void k_gl_ls(const float2* __restrict__ x, const float2* __restrict__ d,
+                                               GLDev gl, int plane, int64_t npix_p,
+                                               const ProbState* __restrict__ st, double* part) {
+  extern __shared__ float gl_lds[];
+  const int F = gl.F, p = blockIdx.y;
+  float* Ls = gl_lds;
+  float2* Xs = reinterpret_cast<float2*>(gl_lds + ((F * F + 1) & ~1));
+  float2* Ds = Xs + F * GL_TILE;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (st[p].status == MRH_CG_ACTIVE) {
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < F * F; i += 256) Ls[i] = gl.L[i];
+    const float2* xp = x + (int64_t)p * npix_p;
+    const float2* dp = d + (int64_t)p * npix_p;
+    for (int t0 = blockIdx.x * GL_TILE; t0 < plane; t0 += gridDim.x * GL_TILE) {
+      const int pix = t0 + lane;
+      const bool ok = pix < plane;
+      __syncthreads();
+      for (int f = wv; f < F; f += 4) {
+        Xs[f * GL_TILE + lane] = ok ? xp[(int64_t)f * plane + pix] : make_float2(0.f, 0.f);
+        Ds[f * GL_TILE + lane] = ok ? dp[(int64_t)f * plane + pix] : make_float2(0.f, 0.f);
+      }
+      __syncthreads();
+      if (ok) {
+        for (int i = wv; i < F; i += 4) {
+          const float* Li = Ls + i * F;
+          float xr = 0.f, xim = 0.f, dr = 0.f, dim = 0.f;
+          for (int j = 0; j < F; ++j) {
+            const float l = Li[j];
+            const float2 xj = Xs[j * GL_TILE + lane], dj = Ds[j * GL_TILE + lane];
+            xr = fmaf(l, xj.x, xr); xim = fmaf(l, xj.y, xim);
+            dr = fmaf(l, dj.x, dr); dim = fmaf(l, dj.y, dim);
+          }
+          const float2 xi = Xs[i * GL_TILE + lane], di = Ds[i * GL_TILE + lane];
+          v[0] += (double)xi.x * xr + (double)xi.y * xim;
+          v[1] += (double)di.x * xr + (double)di.y * xim;
+          v[2] += (double)xi.x * dr + (double)xi.y * dim;
+          v[3] += (double)di.x * dr + (double)di.y * dim;
+        }
+      }
+    }
+  }
+  block_reduce_store<4>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PQ_STRIDE);
+}
+
+// The graph term's value and slope along d at step t (zero without a term).
+__device__ __forceinline__ double gl_value(const ProbState& s, double lam, double t) {
+  return 0.5 * lam * (s.gq_a + t * (s.gq_b1 + s.gq_b2) + t * t * s.gq_c);
+}
+__device__ __forceinline__ double gl_slope(const ProbState& s, double lam, double t) {
+  return lam * (s.gq_b1 + t * s.gq_c);
+}
+
+// Sum the line-search pass's graph partials into the state (zeros without a term).
+__device__ __forceinline__ void gl_collect(ProbState& s, const double* part_q, int nb_q, int p) {
+  double q[4] = {0.0, 0.0, 0.0, 0.0};
+  if (part_q) block_sum_partials_n<4>(part_q + (int64_t)p * nb_q * PQ_STRIDE, nb_q, PQ_STRIDE, 0, q);
+  if (threadIdx.x == 0) { s.gq_a = q[0]; s.gq_b1 = q[1]; s.gq_b2 = q[2]; s.gq_c = q[3]; }
+}
 
 __global__ void k_cg_reset(ProbState* st, int* active, int P) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -460,18 +591,20 @@ __device__ __forceinline__ void ls_record(ProbState& s, mrh_cg_hist* hist, int m
 // Armijo f(t) <= f(0) + c1 t <g, d> for t = 1, beta, ... in that order.
 __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* part_i, const double* part_k,
                                                   int nb_i, int nb_k, double c1, double ls_beta, int ls_max,
-                                                  mrh_cg_hist* hist, int max_hist, int P, int* active) {
+                                                  mrh_cg_hist* hist, int max_hist, int P, int* active,
+                                                  const double* part_q, int nb_q, double gl_lam) {
   const int p = blockIdx.x;
   double vals[3 + 2 + NT_A];
   block_sum_partials_n<3>(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, 0, vals);
   block_sum_partials_n<2 + NT_A>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 3);
+  gl_collect(st[p], part_q, nb_q, p);
   if (threadIdx.x != 0) return;
   ProbState& s = st[p];
   s.ls_pending = 0;
   if (s.status != MRH_CG_ACTIVE) { s.t = 0.0; return; }
   s.a = vals[0]; s.b = vals[1]; s.c = vals[2]; s.g0d = vals[3]; s.dd = vals[4];
   for (int j = 0; j < NT_A; ++j) s.tv[j] = vals[5 + j];
-  const double f0 = 0.5 * s.a + s.tv[0];
+  const double f0 = 0.5 * s.a + s.tv[0] + gl_value(s, gl_lam, 0.0);
   s.f0 = f0;
   s.t = 0.0;
   s.ft = f0;
@@ -482,7 +615,7 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
   }
   double t = 1.0;
   for (int j = 1; j <= ls_max && j < NT_A; ++j) {
-    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j];
+    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j] + gl_value(s, gl_lam, t);
     if (ft <= f0 + c1 * t * s.g0d) {
       s.t = t; s.ft = ft;
       ls_record(s, hist, max_hist, P, p);
@@ -497,7 +630,7 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
 // Second pass: the remaining steps t = beta^(NT_A - 1), ..., for problems still pending.
 __global__ __launch_bounds__(RT) void k_ls_select_rest(ProbState* st, const double* part_i, int nb_i, double c1,
                                                        double ls_beta, int ls_max, mrh_cg_hist* hist,
-                                                       int max_hist, int P, int* active) {
+                                                       int max_hist, int P, int* active, double gl_lam) {
   const int p = blockIdx.x;
   ProbState& s = st[p];
   if (!(s.status == MRH_CG_ACTIVE && s.ls_pending)) return;   // uniform over the block
@@ -509,7 +642,7 @@ __global__ __launch_bounds__(RT) void k_ls_select_rest(ProbState* st, const doub
   double t = 1.0;
   for (int j = 1; j < NT_A; ++j) t *= ls_beta;
   for (int j = NT_A; j <= ls_max && j < NT; ++j) {
-    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j];
+    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j] + gl_value(s, gl_lam, t);
     if (ft <= s.f0 + c1 * t * s.g0d) {
       s.t = t; s.ft = ft;
       ls_record(s, hist, max_hist, P, p);
@@ -630,7 +763,7 @@ template <bool FIRST>
 __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* part_i, const double* part_k,
                                                    int nb_i, int nb_k, double c1, double c2, int ls_max,
                                                    int last, mrh_cg_hist* hist, int max_hist, int P,
-                                                   int* active) {
+                                                   int* active, const double* part_q, int nb_q, double gl_lam) {
   const int p = blockIdx.x;
   ProbState& s = st[p];
   if (!FIRST && !(s.status == MRH_CG_ACTIVE && s.ls_pending)) return;   // uniform over the block
@@ -638,6 +771,7 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
   if (FIRST) {
     block_sum_partials_n<3>(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, 0, vals);
     block_sum_partials_n<2 + 2 * NW>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 3);
+    gl_collect(s, part_q, nb_q, p);
   } else {
     block_sum_partials_n<2 * NW>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 5);
   }
@@ -650,7 +784,7 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
   if (FIRST) {
     if (s.status != MRH_CG_ACTIVE) { s.t = 0.0; return; }
     s.a = vals[0]; s.b = vals[1]; s.c = vals[2]; s.g0d = vals[3]; s.dd = vals[4];
-    s.f0 = 0.5 * s.a + tvv[0];
+    s.f0 = 0.5 * s.a + tvv[0] + gl_value(s, gl_lam, 0.0);
     s.t = 0.0;
     s.ft = s.f0;
     if (!(s.g0d < 0.0) || !isfinite(s.g0d)) {
@@ -691,8 +825,8 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
         s.ls_pending = 1;
         break;
       }
-      const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + tvv[j];
-      const double gt = s.b + t * s.c + tvs[j];
+      const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + tvv[j] + gl_value(s, gl_lam, t);
+      const double gt = s.b + t * s.c + tvs[j] + gl_slope(s, gl_lam, t);
       if (ft > f0 + c1 * t * g0d || (s.wi > 0 && ft >= s.w_fp)) { zoom(s.w_tp, s.w_fp, s.w_gp, t); continue; }
       if (fabs(gt) <= gcurv) { accept(t, ft); return; }
       if (gt >= 0.0) { zoom(t, ft, gt, s.w_tp); continue; }
@@ -712,8 +846,8 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
         s.ls_pending = 1;
         break;
       }
-      const double fm = 0.5 * (s.a + 2.0 * m * s.b + m * m * s.c) + tvv[j];
-      const double gm = s.b + m * s.c + tvs[j];
+      const double fm = 0.5 * (s.a + 2.0 * m * s.b + m * m * s.c) + tvv[j] + gl_value(s, gl_lam, m);
+      const double gm = s.b + m * s.c + tvs[j] + gl_slope(s, gl_lam, m);
       if (fm > f0 + c1 * m * g0d || fm >= s.w_flo) {
         s.w_hi = m;
       } else {
@@ -807,14 +941,16 @@ __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, fl
 __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* part_i, int nb_i, int prime,
                                                 int direction, double rel_tol, double restart_c,
                                                 const double* __restrict__ target, mrh_cg_hist* hist,
-                                                int max_hist, int P, int* active) {
+                                                int max_hist, int P, int* active, const double* part_q,
+                                                int nb_q) {
   const int p = blockIdx.x;
-  double vals[5];
+  double vals[5], eq[1] = {0.0};
   block_sum_partials_n<5>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals);
+  if (part_q) block_sum_partials_n<1>(part_q + (int64_t)p * nb_q * PQ_STRIDE, nb_q, PQ_STRIDE, 0, eq);
   if (threadIdx.x != 0) return;
   ProbState& s = st[p];
   if (!prime && s.status != MRH_CG_ACTIVE) return;
-  s.freg = vals[0];
+  s.freg = vals[0] + eq[0];
   s.rho = vals[1];
   s.gpg = vals[2];
   s.gdot_step = vals[3];
@@ -888,7 +1024,7 @@ __global__ __launch_bounds__(RT) void k_cg_direction(float2* __restrict__ d, con
 __device__ __forceinline__ float axis_degree(int pos, int n) { return (n > 2 && pos > 0 && pos < n - 1) ? 2.f : 1.f; }
 
 __global__ __launch_bounds__(RT) void k_build_diag(const float2* __restrict__ x, float* __restrict__ D, TVDev tv,
-                                                   int64_t npix_p) {
+                                                   GLDev gl, int64_t npix_p) {
   const int p = blockIdx.y;
   const int plane = tv.S0 * tv.S1;
   for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
@@ -910,6 +1046,7 @@ __global__ __launch_bounds__(RT) void k_build_diag(const float2* __restrict__ x,
         dv += tm.lam * (tm.w[a] * tm.w[a]) * wi * axis_degree(a == 1 ? ix : iy, a == 1 ? tv.S0 : tv.S1);
       }
     }
+    if (gl.lam != 0.f) dv += gl.prof[f];   // graph term: its frame profile (graph_laplacian.py:283-321)
     D[n] = dv;
   }
 }
@@ -944,6 +1081,16 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   e->c1 = d->c1; e->ls_beta = d->ls_beta; e->rel_tol = d->rel_tol; e->restart_c = d->restart_c;
   e->ls_kind = d->ls_kind; e->c2 = d->c2;
   e->d_diag = d->diag;
+  if (d->gl_weight != 0.0) {
+    if (d->gl_F != e->F || d->gl_F < 1 || d->gl_F > GL_MAXF || !d->gl_L || !d->gl_prof) {
+      delete e;
+      mrh_set_error("graph term: F = %d must equal the frames per problem (%d), be <= %d, with L and profile",
+                    d->gl_F, d->tv.F, GL_MAXF);
+      return MRH_EINVAL;
+    }
+    e->gl.F = d->gl_F;
+    e->gl.lam = (float)d->gl_weight;
+  }
   e->max_hist = std::max(1, d->max_hist);
   e->npix_p = (int64_t)e->F * e->S0 * e->S1;
   int64_t kmax = 0;
@@ -966,6 +1113,20 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
   ALLOC(e->d_target, sizeof(double) * e->P);
   ALLOC(e->d_active, sizeof(int) * e->P);
+  if (e->gl.lam != 0.f) {
+    const int plane = e->S0 * e->S1;
+    e->nb_q = std::max(1, std::min(2048, (plane + GL_TILE - 1) / GL_TILE));
+    e->gl_lds_grad = sizeof(float) * gl_lds_floats(e->gl.F, 1);
+    e->gl_lds_ls = sizeof(float) * gl_lds_floats(e->gl.F, 2);
+    ALLOC(e->d_gl, sizeof(float) * (size_t)e->gl.F * (e->gl.F + 1));
+    ALLOC(e->d_part_q, sizeof(double) * (size_t)e->P * e->nb_q * PQ_STRIDE);
+    err = hipMemcpy(e->d_gl, d->gl_L, sizeof(float) * e->gl.F * e->gl.F, hipMemcpyHostToDevice);
+    if (err == hipSuccess)
+      err = hipMemcpy(e->d_gl + e->gl.F * e->gl.F, d->gl_prof, sizeof(float) * e->gl.F, hipMemcpyHostToDevice);
+    if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: copy failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
+    e->gl.L = e->d_gl;
+    e->gl.prof = e->d_gl + e->gl.F * e->gl.F;
+  }
 #undef ALLOC
   {
     std::vector<double> none(e->P, -HUGE_VAL);
@@ -983,7 +1144,8 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
 
 extern "C" int mrh_cg_destroy(mrh_cg* e) {
   if (!e) return MRH_OK;
-  void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist, e->d_target, e->d_active};
+  void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist, e->d_target, e->d_active, e->d_gl,
+                  e->d_part_q};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete e;
   return MRH_OK;
@@ -1012,6 +1174,13 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
   hipLaunchKernelGGL(k_ls_kspace, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
                      (const float2*)y, e->d_kprob, e->d_st, e->d_part_k);
   MRH_CHECK_LAUNCH();
+  const double gl_lam = e->gl.lam;
+  const double* part_q = e->gl.lam != 0.f ? e->d_part_q : nullptr;
+  if (part_q) {
+    hipLaunchKernelGGL(k_gl_ls, dim3(e->nb_q, e->P), dim3(256), e->gl_lds_ls, st, (const float2*)x, (const float2*)d,
+                       e->gl, e->S0 * e->S1, e->npix_p, e->d_st, e->d_part_q);
+    MRH_CHECK_LAUNCH();
+  }
   if (e->ls_kind == MRH_CG_LS_WOLFE) {
     const int rest = wolfe_rest_passes(e->ls_max_iter);
     hipLaunchKernelGGL(k_wolfe_image<true>, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
@@ -1019,7 +1188,7 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
     MRH_CHECK_LAUNCH();
     hipLaunchKernelGGL(k_wolfe_step<true>, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
                        e->nb_k, e->c1, e->c2, e->ls_max_iter, rest == 0 ? 1 : 0, e->d_hist, e->max_hist, e->P,
-                       e->d_active);
+                       e->d_active, part_q, e->nb_q, gl_lam);
     MRH_CHECK_LAUNCH();
     for (int q = 0; q < rest; ++q) {
       hipLaunchKernelGGL(k_wolfe_image<false>, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
@@ -1027,7 +1196,7 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
       MRH_CHECK_LAUNCH();
       hipLaunchKernelGGL(k_wolfe_step<false>, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k,
                          e->nb_i, e->nb_k, e->c1, e->c2, e->ls_max_iter, q + 1 == rest ? 1 : 0, e->d_hist,
-                         e->max_hist, e->P, e->d_active);
+                         e->max_hist, e->P, e->d_active, part_q, e->nb_q, gl_lam);
       MRH_CHECK_LAUNCH();
     }
     return MRH_OK;
@@ -1036,13 +1205,14 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
                      (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
-                     e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active);
+                     e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active,
+                     part_q, e->nb_q, gl_lam);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL((k_ls_image<NT_A, NT>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
                      (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_ls_select_rest, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, e->c1,
-                     e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active);
+                     e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active, gl_lam);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -1064,13 +1234,19 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
                                int prime, void* stream) {
   MRH_REQUIRE(e && x && g, MRH_EINVAL, "cg_gradient: null argument");
   hipStream_t st = (hipStream_t)stream;
+  const double* part_q = e->gl.lam != 0.f ? e->d_part_q : nullptr;
+  if (part_q) {
+    hipLaunchKernelGGL(k_gl_grad, dim3(e->nb_q, e->P), dim3(256), e->gl_lds_grad, st, (const float2*)x, (float2*)g,
+                       e->gl, e->S0 * e->S1, e->npix_p, e->d_st, prime, e->d_part_q);
+    MRH_CHECK_LAUNCH();
+  }
   hipLaunchKernelGGL(k_cg_grad, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
                      (const float2*)d, (const float2*)g_prev, (const float*)e->d_diag, e->tv, e->npix_p, e->d_st,
                      prime, e->d_part_i);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
                      e->direction, e->rel_tol, e->restart_c, e->d_target, e->d_hist, e->max_hist, e->P,
-                     e->d_active);
+                     e->d_active, part_q, e->nb_q);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
@@ -1087,7 +1263,7 @@ extern "C" int mrh_cg_build_diag(mrh_cg* e, const void* x, void* stream) {
   MRH_REQUIRE(e && x, MRH_EINVAL, "cg_build_diag: null argument");
   MRH_REQUIRE(e->d_diag, MRH_EINVAL, "cg_build_diag: the engine was created without a diag buffer");
   hipLaunchKernelGGL(k_build_diag, dim3(e->nb_i, e->P), dim3(RT), 0, (hipStream_t)stream, (const float2*)x,
-                     e->d_diag, e->tv, e->npix_p);
+                     e->d_diag, e->tv, e->gl, e->npix_p);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

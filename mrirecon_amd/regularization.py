@@ -17,6 +17,7 @@ import ctypes
 from dataclasses import dataclass
 from typing import List, Optional, Sequence
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -68,20 +69,83 @@ class TVND:
         return t
 
 
-class RegManager:
-    """Orchestrates the TV terms; all terms run in one fused kernel pass."""
+@dataclass(frozen=True)
+class GraphLapParams:
+    """weight = lambda; normalize "none" (L = Deg - W), "rw" (I - Deg^-1 W) or "sym"
+    (I - Deg^-1/2 W Deg^-1/2) (graspcg/regularization/graph_laplacian.py:11-20)."""
+    weight: float = 0.0
+    normalize: str = "none"
+    mm_chunk_cols: int = 65536
 
-    def __init__(self, regs: Sequence[TVND], *, compile_kernels: bool = True):
-        self._regs: List[TVND] = list(regs)
-        if len(self.terms()) > 3:
-            raise NotImplementedError("at most 3 TV terms are fused per pass")
+
+class GraphLaplacian:
+    """Graph-Laplacian term along the frame axis (graph_laplacian.py:22-321):
+    E = (lambda / 2) sum_p Re<x[:, p], (L x)[:, p]>, gradient lambda L x.  W (F, F) dense or sparse
+    COO; it is symmetrised and its diagonal zeroed as the reference does (:40-78).  The term runs in
+    the NLCG engine's kernels (``k_gl_grad``, ``k_gl_ls``), at most one per manager."""
+    Params = GraphLapParams
+
+    def __init__(self, name: str, W, params: GraphLapParams):
+        self.name = name
+        self.params = params
+        if isinstance(W, torch.Tensor):
+            W = W.detach().to("cpu")
+            W = W.to_dense() if W.is_sparse else W
+            W = W.to(torch.float64).numpy()
+        W = np.asarray(W, dtype=np.float64)
+        if W.ndim != 2 or W.shape[0] != W.shape[1]:
+            raise ValueError("W must be a square (F, F) matrix")
+        W = 0.5 * (W + W.T)
+        np.fill_diagonal(W, 0.0)
+        self._W = W
+        self._deg = np.maximum(W.sum(axis=1), 0.0)
+        if (params.normalize or "none").lower() not in ("none", "rw", "sym"):
+            raise ValueError("normalize must be 'none', 'rw' or 'sym'")
 
     @property
-    def regs(self) -> List[TVND]:
+    def frames(self) -> int:
+        return int(self._W.shape[0])
+
+    def laplacian(self) -> np.ndarray:
+        """L in float64 (graph_laplacian.py:140-154): the degree is clamped at 1e-20 where divided."""
+        W, deg, n = self._W, self._deg, self._W.shape[0]
+        norm = (self.params.normalize or "none").lower()
+        if norm == "none":
+            return np.diag(deg) - W
+        if norm == "rw":
+            return np.eye(n) - (1.0 / np.maximum(deg, 1e-20))[:, None] * W
+        dmh = 1.0 / np.sqrt(np.maximum(deg, 1e-20))
+        return np.eye(n) - dmh[:, None] * W * dmh[None, :]
+
+    def profile(self) -> np.ndarray:
+        """Diagonal majoriser per frame (graph_laplacian.py:283-321): lambda deg or lambda [deg > 0]."""
+        v = self._deg if (self.params.normalize or "none").lower() == "none" else (self._deg > 0).astype(np.float64)
+        return self.params.weight * v
+
+
+class RegManager:
+    """Orchestrates the TV terms (one fused kernel pass) and at most one graph-Laplacian term."""
+
+    def __init__(self, regs: Sequence[object], *, compile_kernels: bool = True):
+        self._regs: List[object] = list(regs)
+        if len(self.terms()) > 3:
+            raise NotImplementedError("at most 3 TV terms are fused per pass")
+        if sum(isinstance(r, GraphLaplacian) for r in self._regs) > 1:
+            raise NotImplementedError("at most one graph-Laplacian term")
+
+    @property
+    def regs(self) -> List[object]:
         return list(self._regs)
 
     def terms(self, roles: Roles = IMAGE_ROLES) -> List[_lib.TVTerm]:
-        return [t for t in (r.term(roles) for r in self._regs) if t is not None]
+        return [t for t in (r.term(roles) for r in self._regs if isinstance(r, TVND)) if t is not None]
+
+    def graph(self) -> Optional[GraphLaplacian]:
+        """The graph-Laplacian term with a non-zero weight, if any."""
+        for r in self._regs:
+            if isinstance(r, GraphLaplacian) and r.params.weight != 0.0:
+                return r
+        return None
 
     def tv_desc(self, P: int, F: int, S0: int, S1: int, roles: Roles = IMAGE_ROLES) -> _lib.TVDesc:
         d = _lib.TVDesc()
@@ -96,6 +160,8 @@ class RegManager:
     def energy_and_grad_tensors(self, x: torch.Tensor, g: Optional[torch.Tensor] = None, *,
                                 problems: int = 1) -> torch.Tensor:
         """TV energy per problem (float64, device) of x (P*F, [1,] S0, S1); g += gradient if given."""
+        if self.graph() is not None:
+            raise NotImplementedError("the graph-Laplacian term runs inside the NLCG engine only")
         if x.dtype != torch.complex64 or not x.is_contiguous():
             raise ValueError("x must be a contiguous complex64 tensor")
         if g is not None and (g.dtype != torch.complex64 or not g.is_contiguous() or g.shape != x.shape):

@@ -94,6 +94,15 @@ class NLCGEngine:
         # majoriser at x0, built by prime() (RegManager.add_diag); None: D = 1
         self.diag = torch.ones(shape, dtype=torch.float32, device=dev) if precond else None
         desc.diag = self.diag.data_ptr() if precond else None
+        gl = (regm or RegManager([])).graph()
+        if gl is not None:
+            if gl.frames != self.F:
+                raise ValueError(f"graph term has {gl.frames} frames, the problems have {self.F}")
+            self._gl_L = np.ascontiguousarray(gl.laplacian(), dtype=np.float32)
+            self._gl_prof = np.ascontiguousarray(gl.profile(), dtype=np.float32)
+            desc.gl_weight, desc.gl_F = float(gl.params.weight), int(gl.frames)
+            desc.gl_L = self._gl_L.ctypes.data
+            desc.gl_prof = self._gl_prof.ctypes.data
         if restart_c is not None and restart_c < 0:
             raise ValueError("restart_c must be >= 0 (None keeps the reference behaviour)")
         self.max_hist = int(max_hist)

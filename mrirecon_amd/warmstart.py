@@ -447,6 +447,8 @@ class ObjectiveEval:
         self.e_data = torch.zeros(self.P, dtype=torch.float64, device=dev)
         self.e_tv = torch.zeros(self.P, dtype=torch.float64, device=dev)
         S0, S1 = plan.S
+        if regm is not None and regm.graph() is not None:
+            raise NotImplementedError("objective evaluation covers TV terms only (no graph-Laplacian term)")
         self.tv = (regm or RegManager([])).tv_desc(self.P, self.F, S0, S1)
         C = plan.C
         self.k_prob = [C * int(plan.k_offsets[p * self.F]) for p in range(self.P + 1)]
@@ -490,6 +492,8 @@ class ObjectiveGrad:
         self.energy = torch.zeros(2, dtype=torch.float64, device=dev)
         if plan.B != self.F:
             raise ValueError("ObjectiveGrad handles one problem (plan frames == frames)")
+        if regm is not None and regm.graph() is not None:
+            raise NotImplementedError("objective gradients cover TV terms only (no graph-Laplacian term)")
         self.tv = (regm or RegManager([])).tv_desc(1, self.F, S0, S1)
 
     def __call__(self, x: torch.Tensor, y: torch.Tensor) -> torch.Tensor:

@@ -23,7 +23,8 @@ included, on any operator exposing ``forward`` / ``adjoint`` over
   (graspcg/solvers/utils.py:40-43), direction update FR with the first beta = 1
   quirk (graspcg/numerics/directions.py:141-162), PR+ (:75-109) or Dai-Yuan
   (:111-139), D = 1 (cg.py:95) or, with ``precond``, the regularisers' diagonal
-  majoriser (graspcg/regularization/manager.py:145-218, tv_nd.py:113-215).
+  majoriser (graspcg/regularization/manager.py:145-218, tv_nd.py:113-215);
+* graph-Laplacian term along frames (graspcg/regularization/graph_laplacian.py:22-321).
 
 Pinning (tests/test_oracle_nlcg.py): closed-form quadratic with a unitary
 operator (graspcg/tests/test_numerics_ns.py:110-194), rejection of an ascent
@@ -39,7 +40,10 @@ Wolfe conditions along TV-regularised runs of every direction rule
 (graspcg/tests/test_cg_descend_compile_ns.py:125-176); the diagonal majoriser by
 hand-evaluated entries (constant image, step edge, axis weight, single-frame
 profile) and the preconditioned rules by two properties: a constant D = 2 only
-doubles the accepted steps, and DY's identity holds with D from the majoriser.
+doubles the accepted steps, and DY's identity holds with D from the majoriser;
+the graph-Laplacian term by the chain graph's sum of squared frame differences,
+finite-difference gradients, the reference's degree-diagonal and FD tests
+(graspcg/tests/test_regularization_ns.py:195-275) and the cycle's L_sym = L_rw = L / 2.
 """
 from __future__ import annotations
 
@@ -113,6 +117,64 @@ def tv_energy_grad(x: np.ndarray, reg: TV, g: Optional[np.ndarray] = None):
     return (e * lam).sum(dtype=rdt)
 
 
+@dataclass
+class GraphLap:
+    """Graph-Laplacian term along the frame axis (graspcg/regularization/graph_laplacian.py:22-321).
+
+    W (F, F) real adjacency; normalize "none" (L = Deg - W), "rw" (I - Deg^-1 W) or "sym"
+    (I - Deg^-1/2 W Deg^-1/2)."""
+    weight: float
+    W: np.ndarray
+    normalize: str = "none"
+
+    def laplacian(self) -> np.ndarray:
+        """W symmetrised, its diagonal zeroed, degrees = row sums clamped at 0 (:43-78); the
+        normalised forms divide by the degree clamped at 1e-20 (:101-106, :140-154)."""
+        W = np.asarray(self.W, dtype=np.float64)
+        W = 0.5 * (W + W.T)
+        np.fill_diagonal(W, 0.0)
+        deg = np.maximum(W.sum(axis=1), 0.0)
+        n = W.shape[0]
+        norm = (self.normalize or "none").lower()
+        if norm == "none":
+            return np.diag(deg) - W
+        if norm == "rw":
+            return np.eye(n) - (1.0 / np.maximum(deg, 1e-20))[:, None] * W
+        if norm == "sym":
+            dmh = 1.0 / np.sqrt(np.maximum(deg, 1e-20))
+            return np.eye(n) - dmh[:, None] * W * dmh[None, :]
+        raise ValueError(self.normalize)
+
+    def profile(self) -> np.ndarray:
+        """majorizer_profile (:283-321): lambda deg (none) or lambda [deg > 0] (rw / sym)."""
+        W = np.asarray(self.W, dtype=np.float64)
+        W = 0.5 * (W + W.T)
+        np.fill_diagonal(W, 0.0)
+        deg = np.maximum(W.sum(axis=1), 0.0)
+        v = deg if (self.normalize or "none").lower() == "none" else (deg > 0).astype(np.float64)
+        return self.weight * v
+
+
+def gl_energy_grad(x: np.ndarray, reg: GraphLap, g: Optional[np.ndarray] = None):
+    """E = (lambda / 2) sum_p Re<x[:, p], (L x)[:, p]> and g += lambda L x over the frame axis
+    (graph_laplacian.py:188-242).  For "rw" L is not symmetric and lambda L x is the reference's
+    gradient, not the derivative of E."""
+    rdt = x.real.dtype
+    if reg.weight == 0.0:
+        return rdt.type(0.0)
+    F = x.shape[0]
+    X = x.reshape(F, -1)
+    Y = reg.laplacian().astype(rdt) @ X
+    lam = rdt.type(reg.weight)
+    if g is not None:
+        g += (lam * Y).reshape(x.shape)
+    return rdt.type(0.5) * lam * rdt.type(np.sum((X.conj() * Y).real))
+
+
+def reg_energy_grad(x: np.ndarray, reg, g: Optional[np.ndarray] = None):
+    return gl_energy_grad(x, reg, g) if isinstance(reg, GraphLap) else tv_energy_grad(x, reg, g)
+
+
 def _degree(n: int) -> np.ndarray:
     """Ends of an axis count once, interior positions twice (tv_nd.py:150-159, :207-209)."""
     v = np.ones(n)
@@ -132,6 +194,10 @@ def majoriser_diag(x: np.ndarray, regs: Sequence[TV]) -> np.ndarray:
     rdt = x.real.dtype
     D = np.ones(x.shape, dtype=rdt)
     for reg in regs:
+        if isinstance(reg, GraphLap):       # its frame profile (graph_laplacian.py:283-321)
+            if reg.weight != 0.0:
+                D += reg.profile().astype(rdt).reshape(-1, 1, 1)
+            continue
         if reg.weight == 0.0 or len(reg.axes) == 0:
             continue
         ws = list(reg.axis_weights) if reg.axis_weights is not None else [1.0] * len(reg.axes)
@@ -182,7 +248,7 @@ class Objective:
             xe = x + d * tt
         f_reg = rdt.type(0.0)
         for reg in self.regs:
-            f_reg = f_reg + tv_energy_grad(xe, reg, g)
+            f_reg = f_reg + reg_energy_grad(xe, reg, g)
         if self.mutate_x:
             x += d * (-tt)
         gdot = rdt.type(np.vdot(g.reshape(-1), d.reshape(-1)).real)
@@ -380,5 +446,5 @@ def objective_value(A: Callable, y: np.ndarray, x: np.ndarray, regs: Sequence[TV
     f = 0.5 * float(np.sum(_abs2(r)))
     x64 = np.asarray(x, dtype=np.complex128)
     for reg in regs:
-        f += float(tv_energy_grad(x64, reg))
+        f += float(reg_energy_grad(x64, reg))
     return f

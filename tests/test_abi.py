@@ -84,3 +84,31 @@ def test_struct_layouts_match_c(tmp_path):
         if "." in key:
             cname, field = key.split(".")
             assert getattr(offs[cname], field).offset == int(val), key
+
+
+def test_graph_laplacian_host_side():
+    """The host builds the graph term's L and majoriser profile as the oracle does (symmetrised W,
+    zero diagonal, dense or sparse input); the standalone TV op refuses a manager holding a graph
+    term, and a manager holds at most one."""
+    import numpy as np
+    import torch
+
+    from mrirecon_amd import GraphLaplacian, GraphLapParams, RegManager, TVND, TVParams
+    from oracle.nlcg import GraphLap
+
+    rng = np.random.default_rng(3)
+    W = rng.random((7, 7))
+    W[2, :] = W[:, 2] = 0.0                  # an isolated frame
+    for norm in ("none", "rw", "sym"):
+        gl = GraphLaplacian("g", torch.from_numpy(W).to_sparse(), GraphLapParams(weight=0.3, normalize=norm))
+        ref = GraphLap(0.3, W, normalize=norm)
+        assert np.allclose(gl.laplacian(), ref.laplacian(), rtol=0, atol=1e-12)
+        assert np.allclose(gl.profile(), ref.profile(), rtol=0, atol=1e-12)
+    regm = RegManager([TVND("t", TVParams(weight=0.01, axes="temporal")), gl])
+    assert regm.graph() is gl and len(regm.terms()) == 1
+    with pytest.raises(NotImplementedError):
+        regm.energy_and_grad_tensors(torch.zeros((7, 4, 4), dtype=torch.complex64))
+    with pytest.raises(NotImplementedError):
+        RegManager([gl, GraphLaplacian("h", W, GraphLapParams(weight=0.1))])
+    with pytest.raises(ValueError):
+        GraphLaplacian("bad", W, GraphLapParams(weight=0.1, normalize="lap"))

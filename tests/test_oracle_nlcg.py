@@ -12,8 +12,8 @@ import math
 import numpy as np
 import pytest
 
-from oracle.nlcg import (TV, Objective, armijo, bwd_diff, fwd_diff, majoriser_diag, nlcg, objective_value,
-                         tv_energy_grad, wolfe)
+from oracle.nlcg import (TV, GraphLap, Objective, armijo, bwd_diff, fwd_diff, gl_energy_grad, majoriser_diag, nlcg,
+                         objective_value, tv_energy_grad, wolfe)
 
 
 def fft_op():
@@ -336,3 +336,82 @@ def test_preconditioned_dai_yuan_identity_and_descent():
     assert all(r["gdot"] < 0 for r in h)
     assert [r["step"] for r in h] != [r["step"] for r in plain.history] or \
         any(abs(a["f"] - b["f"]) > 1e-9 * abs(b["f"]) for a, b in zip(h, plain.history))
+
+
+# ------------------------------------------------------------------ graph Laplacian
+
+def chain(F):
+    W = np.zeros((F, F))
+    for t in range(F - 1):
+        W[t, t + 1] = W[t + 1, t] = 1.0
+    return W
+
+
+def test_graph_laplacian_chain_is_temporal_tikhonov():
+    """A chain graph with unit weights, normalize "none": <x, L x> = sum |x_(t+1) - x_t|^2, so
+    E = (lambda / 2) * that, and lambda L x is the gradient (finite differences agree)."""
+    x = rand_c((5, 4, 3), 40)
+    reg = GraphLap(0.7, chain(5))
+    g = np.zeros_like(x)
+    E = gl_energy_grad(x, reg, g)
+    assert abs(E - 0.35 * np.sum(np.abs(np.diff(x, axis=0)) ** 2)) <= 1e-12 * E
+    d = rand_c(x.shape, 41)
+    h = 1e-6
+    fd = (gl_energy_grad(x + h * d, reg) - gl_energy_grad(x - h * d, reg)) / (2 * h)
+    assert abs(fd - np.vdot(g, d).real) <= 1e-6 * abs(fd)
+
+
+def test_graph_degree_diag():
+    """graspcg/tests/test_regularization_ns.py:195-229: chain of 5, lambda 2: D = 1 + 2 (1, 2, 2, 2, 1)
+    (the reference test starts from zeros); rw / sym add lambda on every connected frame."""
+    x = np.zeros((5, 4, 3), complex)
+    D = majoriser_diag(x, [GraphLap(2.0, chain(5))])
+    assert np.allclose(D[:, 0, 0], 1 + 2.0 * np.array([1, 2, 2, 2, 1]), atol=0)
+    W = chain(5)
+    W[4, 3] = W[3, 4] = 0.0           # frame 4 isolated
+    D = majoriser_diag(x, [GraphLap(2.0, W, normalize="sym")])
+    assert np.allclose(D[:, 1, 2], [3, 3, 3, 3, 1], atol=0)
+
+
+@pytest.mark.parametrize("normalize", ["none", "sym"])
+def test_graph_energy_grad_matches_fd(normalize):
+    """graspcg/tests/test_regularization_ns.py:231-275: random symmetric W, the slope of E along d equals
+    Re<g, d> for the symmetric Laplacians."""
+    rng = np.random.default_rng(42)
+    W = rng.random((6, 6))
+    x = rand_c((6, 4, 4), 43) * 0.05
+    d = rand_c(x.shape, 44) * 0.1
+    reg = GraphLap(1e-3, W, normalize=normalize)
+    g = np.zeros_like(x)
+    gl_energy_grad(x, reg, g)
+    h = 1e-4
+    fd = (gl_energy_grad(x + h * d, reg) - gl_energy_grad(x - h * d, reg)) / (2 * h)
+    assert abs(fd - np.vdot(g, d).real) <= 1e-9 * max(1.0, abs(fd))
+
+
+def test_graph_normalisations_on_a_regular_graph():
+    """On a cycle every degree is 2: L_sym = L_rw = L / 2, so their energies are half the plain one;
+    an asymmetric W is symmetrised first (W and its transpose give the same term)."""
+    F = 6
+    W = np.zeros((F, F))
+    for t in range(F):
+        W[t, (t + 1) % F] = W[(t + 1) % F, t] = 1.0
+    x = rand_c((F, 3, 3), 45)
+    e0 = gl_energy_grad(x, GraphLap(1.0, W))
+    for norm in ("rw", "sym"):
+        assert abs(gl_energy_grad(x, GraphLap(1.0, W, normalize=norm)) - 0.5 * e0) <= 1e-12 * e0
+    A = np.triu(np.random.default_rng(1).random((F, F)), 1)
+    assert abs(gl_energy_grad(x, GraphLap(1.0, A)) - gl_energy_grad(x, GraphLap(1.0, A.T))) <= 1e-12 * e0
+
+
+def test_nlcg_with_a_graph_term_descends():
+    """Armijo and strong-Wolfe runs with a chain-graph term and TV keep their line-search conditions."""
+    A, AH, y, x0, regs = dense_problem(47)
+    regs = regs + [GraphLap(0.05, chain(x0.shape[0]))]
+    for ls in ("armijo", "wolfe"):
+        res = nlcg(A, AH, y, x0, regs, max_iter=8, direction="prplus", restart_c=0.0, ls_name=ls, c2=0.1)
+        assert res.n_iter == 8
+        for h in res.history:
+            assert h["f_step"] <= h["f"] + 1e-4 * h["step"] * h["gdot"] + 1e-12
+            if ls == "wolfe":
+                assert abs(h["gdot_at_step"]) <= 0.1 * abs(h["gdot"]) + 1e-12
