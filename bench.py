@@ -273,6 +273,55 @@ def batched_grasp(args, dev, net, seeds, phantoms):
             "coil_chunk": int(plan.info()["coil_chunk"])}
 
 
+def solver_variants(args, dev):
+    """Milliseconds per NLCG iteration on one held-out GRASP volume for the solver options beyond the
+    default path: PR+ with the restart under Armijo, under strong Wolfe (c2 0.9 and 0.1), with the
+    regularisers' diagonal majoriser as preconditioner, and with a chain-graph Laplacian term
+    (lambda 0.01) added to the TV terms.  20 iterations from the adjoint image, HIP-graph replay,
+    median of 3 runs (the objective after 20 iterations is reported beside each)."""
+    import numpy as np
+    import torch
+
+    from mrirecon_amd import GraphLaplacian, GraphLapParams, NLCGEngine, RegManager
+    from mrirecon_amd.train_warmstart import GraspProblem, _phantom
+
+    print("bench: NLCG solver options, ms per iteration on one GRASP volume", file=sys.stderr, flush=True)
+    pb = GraspProblem(max_iter=40)
+    seed = 520_000
+    pb.measure(_phantom(seed), noise_seed=seed + 7)
+    pb.adjoint()
+    F = pb.spec["F"]
+    W = np.zeros((F, F))
+    for t in range(F - 1):
+        W[t, t + 1] = W[t + 1, t] = 1.0
+    reg_gl = RegManager(pb.regm.regs + [GraphLaplacian("gl", W, GraphLapParams(weight=0.01))])
+    variants = {"armijo": dict(), "wolfe_c2_0.9": dict(ls_name="wolfe", c2=0.9),
+                "wolfe_c2_0.1": dict(ls_name="wolfe", c2=0.1), "armijo_preconditioned": dict(precond=True),
+                "armijo_graph_laplacian": dict(regm=reg_gl)}
+    out = {}
+    iters = 20
+    for name, kw in variants.items():
+        kw = dict(kw)
+        regm = kw.pop("regm", pb.regm)
+        eng = NLCGEngine(pb.plan, pb.y, problems=1, frames=F, regm=regm, direction="prplus", restart_c=0.0,
+                         max_hist=iters + 1, **kw)
+        eng.prime(pb.x_adj)
+        eng.capture()
+        times = []
+        for _ in range(3):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            eng.prime(pb.x_adj)
+            eng.run(iters, use_graph=True, check_every=iters)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        h = eng.history()[0]
+        out[name] = {"ms_per_iteration": round(float(np.median(times)) * 1e3 / max(1, len(h)), 4),
+                     "iterations": len(h), "objective": round(h[-1]["f_step"], 4) if h else None}
+        eng.close()
+    return out
+
+
 def nufft_rate(wl, dev, reps=5):
     """Coil-frame NUFFT pairs/s of one workload on an autotuned plan (inputs in HBM)."""
     import numpy as np
@@ -404,6 +453,8 @@ def main():
     ap.add_argument("--c4-slices", type=int, default=48, help="slices of the batched uneven workload")
     ap.add_argument("--no-c4", action="store_true", help="skip the batched uneven workload")
     ap.add_argument("--no-configs", action="store_true", help="skip the c1 / c3 NUFFT rates")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the per-iteration timing of the NLCG solver options")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run a plain device copy of this many MiB (reference for PMC byte counters)")
     args = ap.parse_args()
@@ -500,6 +551,9 @@ def main():
     c4 = None
     if not args.no_c4:
         c4 = batched_uneven(args, dev)
+    variants = None
+    if not args.no_variants:
+        variants = solver_variants(args, dev)
 
     cpu = None
     if not args.no_cpu:
@@ -520,7 +574,7 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": roofline, "kernels": per_kernel, "pipeline": pipeline,
         "time_to_tol": ttt, "unet": unet, "nufft_configs": configs, "batched_uneven": c4,
-        "cpu_baseline": cpu,
+        "solver_variants": variants, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 
