@@ -347,10 +347,11 @@ def test_rel_tol_stop_rule():
     assert abs(len(hist) - ref.n_iter) <= 1
 
 
-@pytest.mark.parametrize("ls_name", ["armijo", "wolfe"])
-def test_batched_problems_equal_separate_runs(ls_name):
+@pytest.mark.parametrize("ls_name,extra", [("armijo", ""), ("wolfe", ""), ("armijo", "graph+precond")])
+def test_batched_problems_equal_separate_runs(ls_name, extra):
     """Two slices with uneven spokes in one engine == each slice alone (with either search; the
-    strong-Wolfe passes skip the slice whose search has ended)."""
+    strong-Wolfe passes skip the slice whose search has ended; the graph term and the majoriser
+    preconditioner work per slice)."""
     S, C, R, F = 48, 3, 96, 4
     spokes = [[13, 21, 17, 30], [25, 14, 40, 19]]
     ws = []
@@ -359,11 +360,22 @@ def test_batched_problems_equal_separate_runs(ls_name):
         ws.append(synth.Workload(f"s{i}", S, 1, F, C, R, 6, 2.0, sp, synth.gaussian_maps(C, S), traj, dcf, off,
                                  synth.dynamic_phantom(S, F, seed=50 + i, n_ellipses=5), lam_t=0.02,
                                  noise_seed=60 + i))
+    kw = dict(ls_name=ls_name, c2=0.1)
+    W = graph_W(F, 64)
+
+    def regs(wl):
+        regm = regs_for(wl)[0]
+        if extra:
+            regm = RegManager(regm.regs + [GraphLaplacian("gl", W, GraphLapParams(weight=0.05, normalize="sym"))])
+        return regm
+
+    if extra:
+        kw["precond"] = True
     solo = []
     for wl in ws:
         be, ora = prepare(wl)
-        regm, _ = regs_for(wl)
-        eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=F, regm=regm, ls_name=ls_name, c2=0.1)
+        regm = regs(wl)
+        eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=F, regm=regm, **kw)
         eng.prime(c64(ora.adjoint(wl.y)))
         eng.run(6)
         solo.append((eng.history()[0], eng.x.cpu().numpy()))
@@ -375,7 +387,7 @@ def test_batched_problems_equal_separate_runs(ls_name):
                k_offsets=off)
     y = np.concatenate([w.y for w in ws])
     x0 = np.concatenate([KBNufft(w.maps, w.traj, w.k_offsets, w.dcf, 6).adjoint(w.y) for w in ws])
-    eng = NLCGEngine(be.plan, c64(y), problems=2, frames=F, regm=regs_for(ws[0])[0], ls_name=ls_name, c2=0.1)
+    eng = NLCGEngine(be.plan, c64(y), problems=2, frames=F, regm=regs(ws[0]), **kw)
     eng.prime(c64(x0))
     eng.run(6)
     hb = eng.history()
