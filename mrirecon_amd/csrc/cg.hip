@@ -743,10 +743,22 @@ __global__ __launch_bounds__(RT) void k_wolfe_image(const float2* __restrict__ x
   block_reduce_store<NV>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
 }
 
+// Index of step t among the first wn candidates (-1: not evaluated).  Unrolled compares and
+// selects keep the candidate arrays in registers.
 __device__ __forceinline__ int wolfe_find(const double (&wt)[NW], int wn, double t) {
-  for (int j = 0; j < wn; ++j)
-    if (wt[j] == t) return j;
-  return -1;
+  int r = -1;
+#pragma unroll
+  for (int j = NW - 1; j >= 0; --j)
+    if (j < wn && wt[j] == t) r = j;
+  return r;
+}
+
+__device__ __forceinline__ double wolfe_pick(const double (&a)[NW], int j) {
+  double r = 0.0;
+#pragma unroll
+  for (int k = 0; k < NW; ++k)
+    if (k == j) r = a[k];
+  return r;
 }
 
 // Step kernel (one block per problem): finish the pass's sums and continue the
@@ -776,8 +788,9 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
     block_sum_partials_n<2 * NW>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 5);
   }
   if (threadIdx.x != 0) return;
-  const double* tvv = vals + 5;        // TV(x + t_j d)
-  const double* tvs = vals + 5 + NW;   // its slope
+  double tvv[NW], tvs[NW];             // TV(x + t_j d) and its slope
+#pragma unroll
+  for (int j = 0; j < NW; ++j) { tvv[j] = vals[5 + j]; tvs[j] = vals[5 + NW + j]; }
   double wt[NW];
   int wn;
   s.ls_pending = 0;
@@ -792,11 +805,13 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
       active[p] = 0;
       return;
     }
+#pragma unroll
     for (int j = 0; j < NW; ++j) wt[j] = j == 0 ? 0.0 : (double)(1 << (j - 1));
     wn = NW;
     s.wzoom = 0; s.wi = 0;
     s.w_t = 1.0; s.w_tp = 0.0; s.w_fp = s.f0; s.w_gp = s.g0d;
   } else {
+#pragma unroll
     for (int j = 0; j < NW; ++j) wt[j] = s.wt[j];
     wn = s.wn;
   }
@@ -825,8 +840,8 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
         s.ls_pending = 1;
         break;
       }
-      const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + tvv[j] + gl_value(s, gl_lam, t);
-      const double gt = s.b + t * s.c + tvs[j] + gl_slope(s, gl_lam, t);
+      const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + wolfe_pick(tvv, j) + gl_value(s, gl_lam, t);
+      const double gt = s.b + t * s.c + wolfe_pick(tvs, j) + gl_slope(s, gl_lam, t);
       if (ft > f0 + c1 * t * g0d || (s.wi > 0 && ft >= s.w_fp)) { zoom(s.w_tp, s.w_fp, s.w_gp, t); continue; }
       if (fabs(gt) <= gcurv) { accept(t, ft); return; }
       if (gt >= 0.0) { zoom(t, ft, gt, s.w_tp); continue; }
@@ -846,8 +861,8 @@ __global__ __launch_bounds__(RT) void k_wolfe_step(ProbState* st, const double* 
         s.ls_pending = 1;
         break;
       }
-      const double fm = 0.5 * (s.a + 2.0 * m * s.b + m * m * s.c) + tvv[j] + gl_value(s, gl_lam, m);
-      const double gm = s.b + m * s.c + tvs[j] + gl_slope(s, gl_lam, m);
+      const double fm = 0.5 * (s.a + 2.0 * m * s.b + m * m * s.c) + wolfe_pick(tvv, j) + gl_value(s, gl_lam, m);
+      const double gm = s.b + m * s.c + wolfe_pick(tvs, j) + gl_slope(s, gl_lam, m);
       if (fm > f0 + c1 * m * g0d || fm >= s.w_flo) {
         s.w_hi = m;
       } else {
