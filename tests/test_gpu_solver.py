@@ -524,3 +524,18 @@ def test_skipping_stopped_problems_changes_nothing():
         assert [r["step"] for r in h0[p]] == [r["step"] for r in h1[p]]
         assert [r["f_step"] for r in h0[p]] == [r["f_step"] for r in h1[p]]
     assert np.array_equal(x0n, x1n)
+
+
+def test_solver_option_limits_are_reported():
+    """Engine limits fail loudly at creation: a graph term whose frame count differs from the problems',
+    an unknown line search, a negative c2."""
+    wl = small_dynamic(seed=95, F=5)
+    be, ora = prepare(wl)
+    regm, _ = regs_for(wl)
+    bad = RegManager(regm.regs + [GraphLaplacian("gl", np.ones((4, 4)), GraphLapParams(weight=0.1))])
+    with pytest.raises(ValueError):
+        NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=bad)
+    with pytest.raises(ValueError):
+        NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, ls_name="backtrack")
+    with pytest.raises(ValueError):
+        NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, ls_name="wolfe", c2=-0.5)
