@@ -39,7 +39,7 @@ int mrh_abi_version(void);
 typedef struct mrh_nufft_plan mrh_nufft_plan;
 
 typedef struct {
-  int ndim;                 /* 2 (3-D is not built)                                   */
+  int ndim;                 /* 2 (3-D volumes use mrh_nufft3_desc below)              */
   int S0, S1;               /* image size (maps.shape[1:])                            */
   int G0, G1;               /* oversampled grid size                                  */
   int W;                    /* Kaiser-Bessel width in grid cells (2..8)                */
@@ -117,6 +117,39 @@ int mrh_nufft_forward_stage(mrh_nufft_plan* plan, int stage, const void* x, void
                             int x_has_coils, void* stream);
 int mrh_nufft_adjoint_stage(mrh_nufft_plan* plan, int stage, const void* y, void* x, int like,
                             void* stream);
+
+/* ------------------------------------------------------------------ 3-D NUFFT
+ * Volumes (S0, S1, S2) and trajectories (3, K_b) per frame: the ndim=3 case of the
+ * reference front end and its default backend (graspcg/nufft/api.py:85-89,
+ * graspcg/nufft/backends/torchkb.py:98-323 with ndim=3), same operator contract as the
+ * 2-D plan (maps, DCF policy, centred placement, KB window and deapodisation per axis).
+ * The oversampled grid lives in the plan's HBM workspace (C G0 G1 G2 complex64); one
+ * like image per frame. */
+typedef struct mrh_nufft3_plan mrh_nufft3_plan;
+
+typedef struct {
+  int S0, S1, S2;           /* volume size (maps.shape[1:])                          */
+  int G0, G1, G2;           /* grid size per axis (32 ... 1024, 2^a 3^b)               */
+  int W;                    /* KB width (2 ... 8)                                      */
+  double beta;              /* KB beta; <= 0 selects Beatty's formula                   */
+  int C;                    /* coils                                                   */
+  int B;                    /* frames                                                  */
+  const int64_t* k_offsets; /* host (B+1): sample offsets of each frame                 */
+  const float* traj;        /* host: per frame a (3, K_b) block in radians             */
+  const float* dcf;         /* host (sum K_b) or NULL                                  */
+  int dcf_mode;             /* MRH_DCF_*                                               */
+  int dcf_in_fwd;           /* 'standard' mode: multiply by dcf in A                   */
+  int dcf_in_adj;           /* 'standard' mode: multiply by dcf in A^H                 */
+  const void* maps;         /* device complex64 (C, S0, S1, S2); copied into the plan  */
+} mrh_nufft3_desc;
+
+int mrh_nufft3_plan_create(const mrh_nufft3_desc* desc, mrh_nufft3_plan** plan);
+int mrh_nufft3_plan_destroy(mrh_nufft3_plan* plan);
+/* x (B, S0, S1, S2) complex64 -> y: per frame a (C, K_b) block (torchkb.py:203-270). */
+int mrh_nufft3_forward(mrh_nufft3_plan* plan, const void* x, void* y, void* stream);
+/* y -> x (B, S0, S1, S2), coil-combined with conj(maps) (torchkb.py:273-323).  The spread
+ * adds floats atomically, so the result can differ in the last bits from run to run. */
+int mrh_nufft3_adjoint(mrh_nufft3_plan* plan, const void* y, void* x, void* stream);
 
 /* ------------------------------------------------------------------ reductions / objective */
 

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -31,6 +31,15 @@ class NufftDesc(Structure):
         ("k_offsets", POINTER(c_int64)), ("traj", POINTER(c_float)), ("dcf", POINTER(c_float)),
         ("dcf_mode", c_int), ("dcf_in_fwd", c_int), ("dcf_in_adj", c_int),
         ("maps", c_void_p), ("strip_cols", c_int),
+    ]
+
+
+class Nufft3Desc(Structure):
+    _fields_ = [
+        ("S0", c_int), ("S1", c_int), ("S2", c_int), ("G0", c_int), ("G1", c_int), ("G2", c_int), ("W", c_int),
+        ("beta", c_double), ("C", c_int), ("B", c_int),
+        ("k_offsets", POINTER(c_int64)), ("traj", POINTER(c_float)), ("dcf", POINTER(c_float)),
+        ("dcf_mode", c_int), ("dcf_in_fwd", c_int), ("dcf_in_adj", c_int), ("maps", c_void_p),
     ]
 
 
@@ -78,6 +87,10 @@ _SIGS = {
     "mrh_nufft_adjoint_masked": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "mrh_nufft_forward_stage": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_void_p]),
     "mrh_nufft_adjoint_stage": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p]),
+    "mrh_nufft3_plan_create": (c_int, [POINTER(Nufft3Desc), POINTER(c_void_p)]),
+    "mrh_nufft3_plan_destroy": (c_int, [c_void_p]),
+    "mrh_nufft3_forward": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mrh_nufft3_adjoint": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
     "mrh_reduce_workspace_len": (c_int64, []),
     "mrh_residual_energy": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int64,
                                     c_void_p, c_void_p, c_void_p]),

@@ -138,7 +138,7 @@ static void twiddles(std::vector<float2>& tw, int n) {
 extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** out) {
   MRH_REQUIRE(d && out, MRH_EINVAL, "plan_create: null argument");
   *out = nullptr;
-  MRH_REQUIRE(d->ndim == 2, MRH_EUNSUP, "only ndim=2 is implemented (got %d)", d->ndim);
+  MRH_REQUIRE(d->ndim == 2, MRH_EUNSUP, "this plan is 2-D (got ndim=%d; 3-D: mrh_nufft3_plan_create)", d->ndim);
   MRH_REQUIRE(d->S0 >= 2 && d->S1 >= 2, MRH_EINVAL, "image size must be >= 2");
   MRH_REQUIRE(d->G0 >= d->S0 && d->G1 >= d->S1, MRH_EINVAL, "grid must be at least the image size");
   MRH_REQUIRE(fft_size_ok(d->G0) && fft_size_ok(d->G1), MRH_EUNSUP,
@@ -995,4 +995,345 @@ extern "C" int mrh_nufft_adjoint_stage(mrh_nufft_plan* p, int stage, const void*
   const FrameMask fm{nullptr, 1, like};
   if (stage == 0) return dispatch_strips(p, true, (const float2*)y, nullptr, nb, like, fm, st);
   return dispatch_rows_adj(p, (float2*)x, nb, fm, st);
+}
+
+// ================================================================ 3-D NUFFT
+// Volumes (S0, S1, S2), trajectories (3, K_b) per frame.  Forward: every coil's
+// x * map * deapodisation placed centred in its zero grid (k3_pad writes every grid
+// point), one LDS FFT pass per axis (k3_fft_lines: one wavefront per grid line), KB
+// interpolation of W^3 grid points per sample (k3_interp).  Adjoint: float atomic
+// spreading into the zeroed grid (k3_spread), inverse FFT passes, crop, conj(map)
+// combine and deapodisation (k3_crop).  Same conventions as the 2-D plan: DC at grid
+// index 0, image index i at (i - S/2) mod G, footprint nodes m0 = floor(u - W/2) + 1 ...
+
+struct Samp3 {
+  int m[3];      // first footprint node per axis, reduced mod G
+  float u[3];    // distance from the sample to that node (grid units)
+  float wA, wAH;
+};
+
+struct mrh_nufft3_plan {
+  int S[3], G[3], W, C, B, np;
+  double beta;
+  int64_t Ktot;
+  KBPoly kb;
+  int64_t* h_koff;     // host copy (launches loop over frames)
+  Samp3* d_samp;
+  float* d_deap[3];
+  float2* d_tw[3];
+  float2* d_maps;
+  float2* d_grid;      // C G0 G1 G2
+};
+
+static int free_plan3(mrh_nufft3_plan* p) {
+  if (!p) return MRH_OK;
+  void* bufs[] = {p->d_samp, p->d_deap[0], p->d_deap[1], p->d_deap[2], p->d_tw[0], p->d_tw[1], p->d_tw[2],
+                  p->d_maps, p->d_grid};
+  for (void* b : bufs) if (b) (void)hipFree(b);
+  delete[] p->h_koff;
+  delete p;
+  return MRH_OK;
+}
+
+// grid[c][g] = x[r] map_c[r] deap(r) where r = (g + S/2) mod G lies inside the volume, else 0.
+__global__ __launch_bounds__(256) void k3_pad(const float2* __restrict__ x, const float2* __restrict__ maps,
+                                              const float* __restrict__ d0, const float* __restrict__ d1,
+                                              const float* __restrict__ d2, float2* __restrict__ grid, int S0,
+                                              int S1, int S2, int G0, int G1, int G2, int C) {
+  const int64_t gvol = (int64_t)G0 * G1 * G2, n = (int64_t)C * gvol;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const int c = (int)(t / gvol);
+    int64_t g = t - (int64_t)c * gvol;
+    const int g2 = (int)(g % G2); g /= G2;
+    const int g1 = (int)(g % G1);
+    const int g0 = (int)(g / G1);
+    const int r0 = (g0 + S0 / 2) % G0, r1 = (g1 + S1 / 2) % G1, r2 = (g2 + S2 / 2) % G2;
+    float2 v = make_float2(0.f, 0.f);
+    if (r0 < S0 && r1 < S1 && r2 < S2) {
+      const int64_t r = ((int64_t)r0 * S1 + r1) * S2 + r2;
+      const float2 xv = x[r], m = maps[(int64_t)c * S0 * S1 * S2 + r];
+      v = cscale(cmul(xv, m), d0[r0] * d1[r1] * d2[r2]);
+    }
+    grid[t] = v;
+  }
+}
+
+// In-place length-N DFT of every line of a (n_outer, N, n_inner) volume along its middle axis:
+// one wavefront per line (no block barriers), elements staged in a padded LDS buffer.
+template <int N, bool INV>
+__global__ __launch_bounds__(256) void k3_fft_lines(float2* __restrict__ v, const float2* __restrict__ tw0,
+                                                    int64_t n_inner, int64_t n_lines) {
+  __shared__ float2 buf[4][fft_padded(N)];
+  __shared__ float2 tw[N];
+  for (int i = threadIdx.x; i < N; i += 256) tw[i] = tw0[i];
+  __syncthreads();
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  float2* b = buf[wv];
+  for (int64_t line = (int64_t)blockIdx.x * 4 + wv; line < n_lines; line += (int64_t)gridDim.x * 4) {
+    const int64_t outer = line / n_inner, inner = line - outer * n_inner;
+    float2* base = v + outer * N * n_inner + inner;
+    for (int i = lane; i < N; i += 64) b[PIDX(i)] = base[(int64_t)i * n_inner];
+    fft_sync<true>();
+    fft_lds_wave<N, INV>(b, tw, lane);
+    fft_sync<true>();
+    for (int i = lane; i < N; i += 64) base[(int64_t)i * n_inner] = b[PIDX(i)];
+    fft_sync<true>();
+  }
+}
+
+// y[c][k] = wA sum over the W^3 footprint of w_x w_y w_z grid[c][...], one thread per (coil, sample).
+__global__ __launch_bounds__(256) void k3_interp(const float2* __restrict__ grid, const Samp3* __restrict__ sp,
+                                                 float2* __restrict__ y, KBPoly kb, int W, int G0, int G1, int G2,
+                                                 int C, int K) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)C * K) return;
+  const int c = (int)(t / K), k = (int)(t - (int64_t)c * K);
+  const Samp3 s = sp[k];
+  float w0[MRH_WMAX], w1[MRH_WMAX], w2[MRH_WMAX];
+#pragma unroll
+  for (int i = 0; i < MRH_WMAX; ++i) {
+    w0[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[0] - (float)i) : 0.f;
+    w1[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[1] - (float)i) : 0.f;
+    w2[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[2] - (float)i) : 0.f;
+  }
+  const float2* gc = grid + (int64_t)c * G0 * G1 * G2;
+  float2 acc = make_float2(0.f, 0.f);
+  for (int i = 0; i < W; ++i) {
+    int i0 = s.m[0] + i; if (i0 >= G0) i0 -= G0;
+    for (int j = 0; j < W; ++j) {
+      int i1 = s.m[1] + j; if (i1 >= G1) i1 -= G1;
+      const float2* row = gc + ((int64_t)i0 * G1 + i1) * G2;
+      const float wij = w0[i] * w1[j];
+      float2 r = make_float2(0.f, 0.f);
+      for (int l = 0; l < W; ++l) {
+        int i2 = s.m[2] + l; if (i2 >= G2) i2 -= G2;
+        const float2 gv = row[i2];
+        r.x = fmaf(gv.x, w2[l], r.x);
+        r.y = fmaf(gv.y, w2[l], r.y);
+      }
+      acc.x = fmaf(r.x, wij, acc.x);
+      acc.y = fmaf(r.y, wij, acc.y);
+    }
+  }
+  y[(int64_t)c * K + k] = cscale(acc, s.wA);
+}
+
+// grid[c][...] += wAH y[c][k] w_x w_y w_z over the footprint (float atomics), one thread per (coil, sample).
+__global__ __launch_bounds__(256) void k3_spread(const float2* __restrict__ y, const Samp3* __restrict__ sp,
+                                                 float2* __restrict__ grid, KBPoly kb, int W, int G0, int G1,
+                                                 int G2, int C, int K) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= (int64_t)C * K) return;
+  const int c = (int)(t / K), k = (int)(t - (int64_t)c * K);
+  const Samp3 s = sp[k];
+  const float2 v = cscale(y[(int64_t)c * K + k], s.wAH);
+  float w0[MRH_WMAX], w1[MRH_WMAX], w2[MRH_WMAX];
+#pragma unroll
+  for (int i = 0; i < MRH_WMAX; ++i) {
+    w0[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[0] - (float)i) : 0.f;
+    w1[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[1] - (float)i) : 0.f;
+    w2[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[2] - (float)i) : 0.f;
+  }
+  float2* gc = grid + (int64_t)c * G0 * G1 * G2;
+  for (int i = 0; i < W; ++i) {
+    int i0 = s.m[0] + i; if (i0 >= G0) i0 -= G0;
+    for (int j = 0; j < W; ++j) {
+      int i1 = s.m[1] + j; if (i1 >= G1) i1 -= G1;
+      float2* row = gc + ((int64_t)i0 * G1 + i1) * G2;
+      const float2 vij = cscale(v, w0[i] * w1[j]);
+      for (int l = 0; l < W; ++l) {
+        int i2 = s.m[2] + l; if (i2 >= G2) i2 -= G2;
+        unsafeAtomicAdd(&row[i2].x, vij.x * w2[l]);
+        unsafeAtomicAdd(&row[i2].y, vij.y * w2[l]);
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k3_zero(float2* __restrict__ v, int64_t n) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
+    v[t] = make_float2(0.f, 0.f);
+}
+
+// x[r] = deap(r) sum_c conj(map_c[r]) grid[c][(r - S/2) mod G].
+__global__ __launch_bounds__(256) void k3_crop(const float2* __restrict__ grid, const float2* __restrict__ maps,
+                                               const float* __restrict__ d0, const float* __restrict__ d1,
+                                               const float* __restrict__ d2, float2* __restrict__ x, int S0,
+                                               int S1, int S2, int G0, int G1, int G2, int C) {
+  const int64_t svol = (int64_t)S0 * S1 * S2, gvol = (int64_t)G0 * G1 * G2;
+  for (int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x; r < svol; r += (int64_t)gridDim.x * 256) {
+    int64_t q = r;
+    const int r2 = (int)(q % S2); q /= S2;
+    const int r1 = (int)(q % S1);
+    const int r0 = (int)(q / S1);
+    const int g0 = (r0 - S0 / 2 + G0) % G0, g1 = (r1 - S1 / 2 + G1) % G1, g2 = (r2 - S2 / 2 + G2) % G2;
+    const int64_t g = ((int64_t)g0 * G1 + g1) * G2 + g2;
+    float2 acc = make_float2(0.f, 0.f);
+    for (int c = 0; c < C; ++c) acc = cadd(acc, cmulc(grid[(int64_t)c * gvol + g], maps[(int64_t)c * svol + r]));
+    x[r] = cscale(acc, d0[r0] * d1[r1] * d2[r2]);
+  }
+}
+
+template <bool INV>
+static int launch_fft_axis(int N, float2* v, const float2* tw, int64_t n_inner, int64_t n_lines, hipStream_t st) {
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(65535, (n_lines + 3) / 4));
+#define MRH_FFT3_CASE(n)                                                                                   \
+  case n:                                                                                                 \
+    hipLaunchKernelGGL((k3_fft_lines<n, INV>), dim3(blocks), dim3(256), 0, st, v, tw, n_inner, n_lines);    \
+    break;
+  switch (N) {
+    MRH_FFT3_CASE(32) MRH_FFT3_CASE(64) MRH_FFT3_CASE(96) MRH_FFT3_CASE(128) MRH_FFT3_CASE(192)
+    MRH_FFT3_CASE(256) MRH_FFT3_CASE(384) MRH_FFT3_CASE(512) MRH_FFT3_CASE(768) MRH_FFT3_CASE(1024)
+    default: mrh_set_error("3-D grid size %d not built", N); return MRH_EUNSUP;
+  }
+#undef MRH_FFT3_CASE
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+// FFT of every coil's grid along all three axes (grid layout (C, G0, G1, G2), G2 fastest).
+template <bool INV>
+static int fft3(const mrh_nufft3_plan* p, hipStream_t st) {
+  const int G0 = p->G[0], G1 = p->G[1], G2 = p->G[2];
+  const int64_t tot = (int64_t)p->C * G0 * G1 * G2;
+  int rc = launch_fft_axis<INV>(G2, p->d_grid, p->d_tw[2], 1, tot / G2, st);
+  if (!rc) rc = launch_fft_axis<INV>(G1, p->d_grid, p->d_tw[1], G2, tot / G1, st);
+  if (!rc) rc = launch_fft_axis<INV>(G0, p->d_grid, p->d_tw[0], (int64_t)G1 * G2, tot / G0, st);
+  return rc;
+}
+
+static int grid_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(8192, (n + 255) / 256)); }
+
+extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan** out) {
+  MRH_REQUIRE(d && out, MRH_EINVAL, "nufft3_plan_create: null argument");
+  *out = nullptr;
+  for (int a = 0; a < 3; ++a) {
+    const int S = a == 0 ? d->S0 : a == 1 ? d->S1 : d->S2, G = a == 0 ? d->G0 : a == 1 ? d->G1 : d->G2;
+    MRH_REQUIRE(S >= 2 && G >= S, MRH_EINVAL, "axis %d: image size %d, grid %d", a, S, G);
+    MRH_REQUIRE(fft_size_ok(G), MRH_EUNSUP, "axis %d: grid size %d not built (32 ... 1024, 2^a 3^b)", a, G);
+  }
+  MRH_REQUIRE(d->W >= 2 && d->W <= MRH_WMAX, MRH_EUNSUP, "KB width %d outside [2, %d]", d->W, MRH_WMAX);
+  MRH_REQUIRE(d->C >= 1 && d->B >= 1, MRH_EINVAL, "C, B must be >= 1");
+  MRH_REQUIRE(d->k_offsets && d->traj && d->maps, MRH_EINVAL, "k_offsets, traj (host) and maps (device) are required");
+  mrh_nufft3_plan* p = new mrh_nufft3_plan();
+  std::memset(p, 0, sizeof(*p));
+  p->S[0] = d->S0; p->S[1] = d->S1; p->S[2] = d->S2;
+  p->G[0] = d->G0; p->G[1] = d->G1; p->G[2] = d->G2;
+  p->W = d->W; p->C = d->C; p->B = d->B;
+  // KB series (as the 2-D plan: beta from the first axis' oversampling unless given)
+  const double osf = (double)d->G0 / (double)d->S0;
+  const double beta = d->beta > 0 ? d->beta : M_PI * std::sqrt(std::pow((d->W / osf) * (osf - 0.5), 2) - 0.8);
+  p->beta = beta;
+  const double i0b = bessel_i0(beta), q = beta * beta / 4.0;
+  double coef[64];
+  {
+    double fact = 1.0;
+    for (int n = 0; n < 64; ++n) {
+      if (n > 0) fact *= n;
+      coef[n] = 1.0 / (fact * fact * i0b);
+    }
+  }
+  int np = 1;
+  while (np < MRH_KB_MAXNP && coef[np] * std::pow(q, np) > 1e-10) ++np;
+  if (!(coef[np] * std::pow(q, np) < 1e-9)) { free_plan3(p); mrh_set_error("KB beta %.3f too large for the series", beta); return MRH_EUNSUP; }
+  p->np = np;
+  p->kb.qscale = (float)q;
+  p->kb.two_over_w = (float)(2.0 / d->W);
+  for (int n = 0; n < MRH_KB_MAXNP; ++n) p->kb.a[n] = n < np ? (float)coef[n] : 0.f;
+  // samples
+  const int B = d->B;
+  p->h_koff = new int64_t[B + 1];
+  for (int b = 0; b <= B; ++b) p->h_koff[b] = d->k_offsets[b];
+  for (int b = 0; b < B; ++b)
+    if (p->h_koff[b + 1] < p->h_koff[b]) { free_plan3(p); mrh_set_error("k_offsets must be non-decreasing"); return MRH_EINVAL; }
+  p->Ktot = p->h_koff[B];
+  MRH_REQUIRE(p->Ktot * (int64_t)d->C < ((int64_t)1 << 31), MRH_EUNSUP, "too many coil-samples for one plan");
+  std::vector<Samp3> samp((size_t)std::max<int64_t>(1, p->Ktot));
+  for (int b = 0; b < B; ++b) {
+    const int64_t k0 = p->h_koff[b], Kb = p->h_koff[b + 1] - k0;
+    const float* tb = d->traj + 3 * k0;
+    for (int64_t k = 0; k < Kb; ++k) {
+      Samp3 s;
+      for (int a = 0; a < 3; ++a) {
+        const double u = (double)tb[a * Kb + k] * p->G[a] / (2.0 * M_PI);
+        const double f = std::floor(u - 0.5 * d->W) + 1.0;
+        s.m[a] = (int)((((int64_t)f % p->G[a]) + p->G[a]) % p->G[a]);
+        s.u[a] = (float)(u - f);
+      }
+      const float dval = d->dcf ? d->dcf[k0 + k] : 1.f;
+      s.wA = 1.f; s.wAH = 1.f;
+      if (d->dcf && d->dcf_mode == MRH_DCF_BALANCED) {
+        s.wA = s.wAH = (float)std::sqrt(std::max((double)dval, 0.0));
+      } else if (d->dcf && d->dcf_mode == MRH_DCF_STANDARD) {
+        s.wA = d->dcf_in_fwd ? dval : 1.f;
+        s.wAH = d->dcf_in_adj ? dval : 1.f;
+      }
+      samp[(size_t)(k0 + k)] = s;
+    }
+  }
+  int rc = upload(&p->d_samp, samp.data(), samp.size());
+  for (int a = 0; a < 3 && !rc; ++a) {
+    std::vector<float> dv(p->S[a]);
+    for (int i = 0; i < p->S[a]; ++i) dv[i] = (float)(1.0 / kb_ft((double)(i - p->S[a] / 2) / p->G[a], d->W, beta));
+    rc = upload(&p->d_deap[a], dv.data(), dv.size());
+    std::vector<float2> tw;
+    twiddles(tw, p->G[a]);
+    if (!rc) rc = upload(&p->d_tw[a], tw.data(), tw.size());
+  }
+  const int64_t svol = (int64_t)d->S0 * d->S1 * d->S2, gvol = (int64_t)d->G0 * d->G1 * d->G2;
+  if (!rc) rc = upload<float2>(&p->d_maps, nullptr, (size_t)(d->C * svol));
+  if (!rc) rc = upload<float2>(&p->d_grid, nullptr, (size_t)(d->C * gvol));
+  if (rc) { free_plan3(p); return rc; }
+  hipError_t err = hipMemcpy(p->d_maps, d->maps, sizeof(float2) * d->C * svol, hipMemcpyDeviceToDevice);
+  if (err != hipSuccess) { free_plan3(p); mrh_set_error("nufft3: maps copy failed: %s", hipGetErrorString(err)); return MRH_EHIP; }
+  *out = p;
+  return MRH_OK;
+}
+
+extern "C" int mrh_nufft3_plan_destroy(mrh_nufft3_plan* p) { return free_plan3(p); }
+
+extern "C" int mrh_nufft3_forward(mrh_nufft3_plan* p, const void* x, void* y, void* stream) {
+  MRH_REQUIRE(p && x && y, MRH_EINVAL, "nufft3_forward: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t svol = (int64_t)p->S[0] * p->S[1] * p->S[2], gtot = (int64_t)p->C * p->G[0] * p->G[1] * p->G[2];
+  for (int b = 0; b < p->B; ++b) {
+    const int64_t k0 = p->h_koff[b], Kb = p->h_koff[b + 1] - k0;
+    hipLaunchKernelGGL(k3_pad, dim3(grid_blocks(gtot)), dim3(256), 0, st, (const float2*)x + b * svol, p->d_maps,
+                       p->d_deap[0], p->d_deap[1], p->d_deap[2], p->d_grid, p->S[0], p->S[1], p->S[2], p->G[0],
+                       p->G[1], p->G[2], p->C);
+    MRH_CHECK_LAUNCH();
+    int rc = fft3<false>(p, st);
+    if (rc) return rc;
+    if (Kb == 0) continue;
+    const int64_t n = (int64_t)p->C * Kb;
+    hipLaunchKernelGGL(k3_interp, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, p->d_grid, p->d_samp + k0,
+                       (float2*)y + (int64_t)p->C * k0, p->kb, p->W, p->G[0], p->G[1], p->G[2], p->C, (int)Kb);
+    MRH_CHECK_LAUNCH();
+  }
+  return MRH_OK;
+}
+
+extern "C" int mrh_nufft3_adjoint(mrh_nufft3_plan* p, const void* y, void* x, void* stream) {
+  MRH_REQUIRE(p && x && y, MRH_EINVAL, "nufft3_adjoint: null argument");
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t svol = (int64_t)p->S[0] * p->S[1] * p->S[2], gtot = (int64_t)p->C * p->G[0] * p->G[1] * p->G[2];
+  for (int b = 0; b < p->B; ++b) {
+    const int64_t k0 = p->h_koff[b], Kb = p->h_koff[b + 1] - k0;
+    hipLaunchKernelGGL(k3_zero, dim3(grid_blocks(gtot)), dim3(256), 0, st, p->d_grid, gtot);
+    MRH_CHECK_LAUNCH();
+    if (Kb > 0) {
+      const int64_t n = (int64_t)p->C * Kb;
+      hipLaunchKernelGGL(k3_spread, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                         (const float2*)y + (int64_t)p->C * k0, p->d_samp + k0, p->d_grid, p->kb, p->W, p->G[0],
+                         p->G[1], p->G[2], p->C, (int)Kb);
+      MRH_CHECK_LAUNCH();
+    }
+    int rc = fft3<true>(p, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k3_crop, dim3(grid_blocks(svol)), dim3(256), 0, st, p->d_grid, p->d_maps, p->d_deap[0],
+                       p->d_deap[1], p->d_deap[2], (float2*)x + b * svol, p->S[0], p->S[1], p->S[2], p->G[0],
+                       p->G[1], p->G[2], p->C);
+    MRH_CHECK_LAUNCH();
+  }
+  return MRH_OK;
 }

@@ -166,6 +166,58 @@ class NufftPlan:
             pass
 
 
+class Nufft3Plan:
+    """Owns one 3-D C-ABI plan (samples, deapodisation, twiddles, maps, grid workspace)."""
+
+    def __init__(self, *, S: Tuple[int, int, int], G: Tuple[int, int, int], W: int, C: int,
+                 k_offsets: np.ndarray, traj_flat: np.ndarray, dcf_flat: Optional[np.ndarray],
+                 dcf_mode: str, dcf_in_fwd: bool, dcf_in_adj: bool, maps: torch.Tensor):
+        lib = _lib.load()
+        self._lib = lib
+        self.k_offsets = np.ascontiguousarray(k_offsets, dtype=np.int64)
+        self.traj_flat = np.ascontiguousarray(traj_flat, dtype=np.float32)
+        self.dcf_flat = None if dcf_flat is None else np.ascontiguousarray(dcf_flat, dtype=np.float32)
+        self.B = len(self.k_offsets) - 1
+        self.C, self.S, self.G, self.W = int(C), tuple(S), tuple(G), int(W)
+        self.maps = maps
+        d = _lib.Nufft3Desc()
+        d.S0, d.S1, d.S2 = self.S
+        d.G0, d.G1, d.G2 = self.G
+        d.W, d.beta, d.C, d.B = self.W, 0.0, self.C, self.B
+        d.k_offsets = self.k_offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+        d.traj = self.traj_flat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        d.dcf = (self.dcf_flat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+                 if self.dcf_flat is not None else None)
+        d.dcf_mode = _DCF_CODES[dcf_mode]
+        d.dcf_in_fwd, d.dcf_in_adj = int(dcf_in_fwd), int(dcf_in_adj)
+        d.maps = maps.data_ptr()
+        h = ctypes.c_void_p()
+        with torch.cuda.device(maps.device):
+            _lib.check(lib.mrh_nufft3_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self.handle = h
+
+    @property
+    def n_k(self) -> int:
+        return int(self.k_offsets[-1]) * self.C
+
+    def forward(self, x: torch.Tensor, y: torch.Tensor) -> None:
+        _lib.check(self._lib.mrh_nufft3_forward(self.handle, x.data_ptr(), y.data_ptr(), _lib.stream_of(x)))
+
+    def adjoint(self, y: torch.Tensor, x: torch.Tensor) -> None:
+        _lib.check(self._lib.mrh_nufft3_adjoint(self.handle, y.data_ptr(), x.data_ptr(), _lib.stream_of(y)))
+
+    def close(self) -> None:
+        if getattr(self, "handle", None) is not None and self.handle.value:
+            self._lib.mrh_nufft3_plan_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def _grid_size(S: int, osf: float) -> int:
     return int(math.ceil(osf * S))
 
@@ -204,38 +256,41 @@ class HipNUFFTAdapter:
                 k_offsets: Optional[Sequence[int]] = None) -> None:
         """Build the device plan.  ``traj`` is (B, nd, K) | (B, K, nd) | (nd, K), or with
         ``k_offsets`` a flat (nd, sum K_b) array of per-frame blocks."""
-        if self.ndim != 2:
-            raise NotImplementedError("the 'hip' backend implements ndim=2")
+        if self.ndim not in (2, 3):
+            raise ValueError("the 'hip' backend supports ndim in {2, 3}")
         device = torch.device(device) if device is not None else maps.device
         if device.type != "cuda":
             raise RuntimeError(f"'hip' backend needs a GPU device, got {device}")
         if dtype not in (torch.complex64, torch.complex128):
             raise ValueError(f"unsupported dtype {dtype}")
         self._im_shape = tuple(int(s) for s in im_shape)
-        C, S0, S1 = self._im_shape
+        nd = self.ndim
+        if len(self._im_shape) != nd + 1:
+            raise ValueError(f"im_shape must be (C, *S) with {nd} spatial axes, got {self._im_shape}")
+        C = self._im_shape[0]
         self._dev, self._dtype = device, dtype
         self._maps = maps.to(device=device, dtype=torch.complex64).contiguous()
-        if tuple(self._maps.shape) != (C, S0, S1):
-            raise ValueError(f"maps shape {tuple(self._maps.shape)} != {(C, S0, S1)}")
+        if tuple(self._maps.shape) != self._im_shape:
+            raise ValueError(f"maps shape {tuple(self._maps.shape)} != {self._im_shape}")
         t = traj.detach().to("cpu", torch.float32)
         scale = 2.0 * math.pi if self.traj_units == "norm" else 1.0
         if k_offsets is not None:
             off = np.asarray(k_offsets, dtype=np.int64)
-            if t.ndim != 2 or int(t.shape[0]) != 2 or int(t.shape[1]) != int(off[-1]):
-                raise ValueError("ragged traj must be (2, sum K_b) matching k_offsets")
+            if t.ndim != 2 or int(t.shape[0]) != nd or int(t.shape[1]) != int(off[-1]):
+                raise ValueError(f"ragged traj must be ({nd}, sum K_b) matching k_offsets")
             B = len(off) - 1
-            tn = t.numpy() * scale                       # (2, sum K) -> per-frame (2, K_b) blocks
+            tn = t.numpy() * scale                       # (nd, sum K) -> per-frame (nd, K_b) blocks
             flat = np.concatenate([tn[:, off[b]:off[b + 1]].reshape(-1) for b in range(B)])
         else:
             if t.ndim == 2:
-                if int(t.shape[0]) != 2 and int(t.shape[1]) == 2:
+                if int(t.shape[0]) != nd and int(t.shape[1]) == nd:
                     t = t.t()
-                if int(t.shape[0]) != 2:
-                    raise ValueError("traj has wrong nd; expected 2")
+                if int(t.shape[0]) != nd:
+                    raise ValueError(f"traj has wrong nd; expected {nd}")
                 t = t.unsqueeze(0)
             elif t.ndim == 3:
-                if int(t.shape[1]) != 2:
-                    if int(t.shape[2]) == 2:
+                if int(t.shape[1]) != nd:
+                    if int(t.shape[2]) == nd:
                         t = t.transpose(1, 2)
                     else:
                         raise ValueError("traj must be (B,nd,K) or (B,K,nd)")
@@ -243,7 +298,7 @@ class HipNUFFTAdapter:
                 raise ValueError("traj must have 2 or 3 dims")
             B, _, K = t.shape
             off = np.arange(B + 1, dtype=np.int64) * int(K)
-            flat = t.contiguous().numpy() * scale           # (B, 2, K) == per-frame (2, K) blocks
+            flat = t.contiguous().numpy() * scale           # (B, nd, K) == per-frame (nd, K) blocks
         self._B, self._k_offsets = int(B), off
         ks = np.diff(off)
         self._K = int(ks[0]) if np.all(ks == ks[0]) else -1
@@ -272,7 +327,15 @@ class HipNUFFTAdapter:
         self._alpha_scalar = None
         self._plan(1)
 
-    def _new_plan(self, like: int, strip_cols: int) -> NufftPlan:
+    def _new_plan(self, like: int, strip_cols: int):
+        if self.ndim == 3:
+            if like != 1:
+                raise NotImplementedError("3-D plans take one like image per frame")
+            C, S0, S1, S2 = self._im_shape
+            return Nufft3Plan(S=(S0, S1, S2), G=tuple(_grid_size(v, self.osf) for v in (S0, S1, S2)),
+                              W=self.kb_width, C=C, k_offsets=self._k_offsets, traj_flat=self._traj_flat,
+                              dcf_flat=self._dcf_flat, dcf_mode=self.dcf_mode, dcf_in_fwd=self.apply_dcf_in_fwd,
+                              dcf_in_adj=self.apply_dcf_in_adj, maps=self._maps)
         C, S0, S1 = self._im_shape
         return NufftPlan(S=(S0, S1), G=(_grid_size(S0, self.osf), _grid_size(S1, self.osf)),
                          W=self.kb_width, C=C, k_offsets=self._k_offsets, traj_flat=self._traj_flat,
@@ -286,6 +349,10 @@ class HipNUFFTAdapter:
         plan whose coil chunk is tuned; the fastest plan is kept (``plan.tuning`` lists all times)."""
         p = self._plans.get(like)
         if p is not None:
+            return p
+        if self.ndim == 3:                       # no strip kernels to tune
+            p = self._new_plan(like, 0)
+            self._plans[like] = p
             return p
         if not self.autotune:
             p = self._new_plan(like, self.strip_cols)
@@ -351,6 +418,8 @@ class HipNUFFTAdapter:
         if self._im_shape is None:
             raise RuntimeError("prepare() was not called")
         self._check_dev(x)
+        if self.ndim == 3:
+            return self._A3(x, out)
         C = self._im_shape[0]
         spatial = self._im_shape[1:]
         if x.ndim == 4:
@@ -388,6 +457,8 @@ class HipNUFFTAdapter:
         if self._im_shape is None:
             raise RuntimeError("prepare() was not called")
         self._check_dev(y)
+        if self.ndim == 3:
+            return self._AH3(y, out)
         C = self._im_shape[0]
         spatial = tuple(self._im_shape[1:])
         if self._K < 0:
@@ -421,6 +492,47 @@ class HipNUFFTAdapter:
             out.copy_(xb)
         return out
 
+    # ------------------------------------------------------------ 3-D operators
+    def _A3(self, x: torch.Tensor, out: Optional[torch.Tensor]) -> torch.Tensor:
+        """(B, 1, X, Y, Z) volumes -> (B, C, K) (flat per-frame (C, K_b) blocks when ragged)."""
+        C, spatial = self._im_shape[0], tuple(self._im_shape[1:])
+        if x.ndim != 5 or int(x.shape[1]) != 1 or tuple(int(v) for v in x.shape[2:]) != spatial:
+            raise ValueError(f"x must be (B, 1, *{spatial}) for ndim=3; got {tuple(x.shape)}")
+        if int(x.shape[0]) != self._B:
+            raise ValueError(f"x batch {int(x.shape[0])} != trajectory frames {self._B} (one like image)")
+        plan = self._plan(1)
+        need = (plan.n_k,) if self._K < 0 else (self._B, C, self._K)
+        if out is None:
+            out = torch.empty(need, dtype=x.dtype, device=x.device)
+        elif tuple(out.shape) != tuple(need):
+            raise ValueError(f"out has shape {tuple(out.shape)} but expected {tuple(need)}")
+        direct = out.dtype == torch.complex64 and out.is_contiguous()
+        yb = out if direct else torch.empty(need, dtype=torch.complex64, device=x.device)
+        plan.forward(self._as_c64(x), yb)
+        if not direct:
+            out.copy_(yb)
+        return out
+
+    def _AH3(self, y: torch.Tensor, out: Optional[torch.Tensor]) -> torch.Tensor:
+        """(B, C, K) or flat ragged k-space -> (B, 1, X, Y, Z)."""
+        C, spatial = self._im_shape[0], tuple(self._im_shape[1:])
+        plan = self._plan(1)
+        if int(y.numel()) != plan.n_k:
+            raise ValueError(f"y has {int(y.numel())} samples, the plan expects {plan.n_k}")
+        if y.ndim == 3 and (int(y.shape[0]) != self._B or int(y.shape[1]) != C):
+            raise ValueError(f"y must be ({self._B}, {C}, K)")
+        need = (self._B, 1) + spatial
+        if out is None:
+            out = torch.empty(need, dtype=y.dtype, device=y.device)
+        elif tuple(out.shape) != tuple(need):
+            raise ValueError(f"out has shape {tuple(out.shape)} but expected {tuple(need)}")
+        direct = out.dtype == torch.complex64 and out.is_contiguous()
+        xb = out if direct else torch.empty(need, dtype=torch.complex64, device=y.device)
+        plan.adjoint(self._as_c64(y), xb)
+        if not direct:
+            out.copy_(xb)
+        return out
+
     # ------------------------------------------------------------ calibration
     @torch.no_grad()
     def diag_AHA_profile(self) -> torch.Tensor:
@@ -437,6 +549,16 @@ class HipNUFFTAdapter:
         """alpha = A^H A(delta_centre) / sum_c |S_c|^2(centre), cached (torchkb.py:342-364)."""
         if self._alpha_scalar is not None:
             return float(self._alpha_scalar)
+        if self.ndim == 3:                       # maps applied (one image slot): A^H A delta = alpha sum |S_c|^2
+            spatial = tuple(self._im_shape[1:])
+            ctr = tuple(v // 2 for v in spatial)
+            x0 = torch.zeros((self._B, 1) + spatial, device=self._dev, dtype=torch.complex64)
+            x0[(0, 0) + ctr] = 1.0
+            z = self.AH(self.A(x0))
+            zc = float(z[(0, 0) + ctr].real.item())
+            sos = float((self._maps.abs() ** 2).sum(dim=0)[ctr].clamp_min(1e-20).item())
+            self._alpha_scalar = zc / sos
+            return self._alpha_scalar
         C, S0, S1 = self._im_shape
         ctr = (S0 // 2, S1 // 2)
         x0 = torch.zeros((self._B, C, S0, S1), device=self._dev, dtype=torch.complex64)

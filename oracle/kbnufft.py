@@ -188,6 +188,100 @@ class KBNufft:
         return out
 
 
+class KBNufft3:
+    """3-D KB-gridding SENSE operator on (B, S0, S1, S2) volumes (the ndim=3 case of the
+    reference front end, graspcg/nufft/api.py:85-89, with the 2-D conventions above applied
+    per axis).  k-space: frame b owns ``y[C*k_offsets[b] : C*k_offsets[b+1]]`` as (C, K_b);
+    ``traj`` is (3, sum K_b) in radians."""
+
+    def __init__(self, maps: np.ndarray, traj: np.ndarray, k_offsets: np.ndarray,
+                 dcf: Optional[np.ndarray], W: int, osf: float = 2.0, dcf_mode: str = "balanced"):
+        self.maps = np.asarray(maps).astype(np.complex128)
+        self.C = self.maps.shape[0]
+        self.S = tuple(int(v) for v in self.maps.shape[1:])
+        self.G = tuple(int(math.ceil(osf * v)) for v in self.S)
+        self.W = int(W)
+        self.beta = kb_beta(W, osf)
+        self.off = np.asarray(k_offsets, dtype=np.int64)
+        self.B = len(self.off) - 1
+        self.traj = np.asarray(traj, dtype=np.float64)
+        if dcf is None or dcf_mode == "none":
+            self.wA = self.wAH = None
+        elif dcf_mode == "balanced":
+            self.wA = self.wAH = np.sqrt(np.clip(np.asarray(dcf, dtype=np.float64), 0.0, None))
+        elif dcf_mode == "standard":
+            self.wA, self.wAH = None, np.asarray(dcf, dtype=np.float64)
+        else:
+            raise ValueError(dcf_mode)
+        d = [deapod_1d(S, G, self.W, self.beta) for S, G in zip(self.S, self.G)]
+        self.deapod = d[0][:, None, None] * d[1][None, :, None] * d[2][None, None, :]
+        self.idx = [(np.arange(S) - S // 2) % G for S, G in zip(self.S, self.G)]
+        self._fp = []
+        for b in range(self.B):
+            sl = slice(self.off[b], self.off[b + 1])
+            ms, ws = [], []
+            for a in range(3):
+                m, w = footprint(self.traj[a, sl] * self.G[a] / (2.0 * math.pi), self.W, self.beta)
+                ms.append((m[:, None] + np.arange(self.W)[None, :]) % self.G[a])
+                ws.append(w)
+            n, W3 = len(ms[0]), self.W ** 3
+            flat = ((ms[0][:, :, None, None] * self.G[1] + ms[1][:, None, :, None]) * self.G[2]
+                    + ms[2][:, None, None, :]).reshape(n, W3)
+            wgt = (ws[0][:, :, None, None] * ws[1][:, None, :, None] * ws[2][:, None, None, :]).reshape(n, W3)
+            self._fp.append((flat, wgt))
+
+    def forward(self, x: np.ndarray) -> np.ndarray:
+        x = np.asarray(x, dtype=np.complex128).reshape(self.B, *self.S)
+        y = np.zeros(int(self.off[-1]) * self.C, dtype=np.complex128)
+        i0, i1, i2 = np.ix_(*self.idx)
+        for b in range(self.B):
+            flat, wgt = self._fp[b]
+            grid = np.zeros((self.C,) + self.G, dtype=np.complex128)
+            grid[:, i0, i1, i2] = x[b][None] * self.maps * self.deapod[None]
+            grid = np.fft.fftn(grid, axes=(1, 2, 3)).reshape(self.C, -1)
+            vals = np.einsum("ckw,kw->ck", grid[:, flat], wgt)
+            sl = slice(self.off[b], self.off[b + 1])
+            if self.wA is not None:
+                vals = vals * self.wA[sl][None]
+            y[self.C * self.off[b]:self.C * self.off[b + 1]] = vals.reshape(-1)
+        return y
+
+    def adjoint(self, y: np.ndarray) -> np.ndarray:
+        y = np.asarray(y, dtype=np.complex128).reshape(-1)
+        out = np.zeros((self.B,) + self.S, dtype=np.complex128)
+        Gt = int(np.prod(self.G))
+        i0, i1, i2 = np.ix_(*self.idx)
+        for b in range(self.B):
+            flat, wgt = self._fp[b]
+            K = int(self.off[b + 1] - self.off[b])
+            yk = y[self.C * self.off[b]:self.C * self.off[b + 1]].reshape(self.C, K)
+            sl = slice(self.off[b], self.off[b + 1])
+            if self.wAH is not None:
+                yk = yk * self.wAH[sl][None]
+            acc = np.zeros(self.S, dtype=np.complex128)
+            for c in range(self.C):
+                contrib = (yk[c][:, None] * wgt).reshape(-1)
+                fl = flat.reshape(-1)
+                grid = (np.bincount(fl, weights=contrib.real, minlength=Gt)
+                        + 1j * np.bincount(fl, weights=contrib.imag, minlength=Gt)).reshape(self.G)
+                img = np.fft.ifftn(grid) * Gt
+                acc += img[i0, i1, i2] * np.conj(self.maps[c])
+            out[b] = acc * self.deapod
+        return out
+
+
+def nudft3_forward(maps: np.ndarray, x: np.ndarray, traj_b: np.ndarray) -> np.ndarray:
+    """FP64 3-D NUDFT of one frame (the 2-D ground truth of graspcg/tests/test_nufft_precision_gt.py:55-89
+    with a third centred coordinate): maps (C, S0, S1, S2), x (S0, S1, S2), traj_b (3, K) -> (C, K)."""
+    C = maps.shape[0]
+    S = maps.shape[1:]
+    r = [np.arange(s, dtype=np.float64) - s // 2 for s in S]
+    R = np.stack([g.reshape(-1) for g in np.meshgrid(*r, indexing="ij")], axis=1)
+    E = np.exp(-1j * (np.asarray(traj_b, dtype=np.float64).T @ R.T))
+    img = (np.asarray(maps, dtype=np.complex128) * np.asarray(x, dtype=np.complex128)[None]).reshape(C, -1)
+    return img @ E.T
+
+
 # --------------------------------------------------------------- NUDFT (closed form)
 
 def nudft_forward(maps: np.ndarray, x: np.ndarray, traj_b: np.ndarray,

@@ -55,12 +55,16 @@ C_PROBE = r"""
 #include "mrirecon_hip.h"
 #define P(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
 int main(void) {
-  printf("mrh_nufft_desc %zu\nmrh_nufft_info %zu\nmrh_tv_term %zu\nmrh_tv_desc %zu\nmrh_cg_desc %zu\nmrh_cg_hist %zu\n",
+  printf("mrh_nufft_desc %zu\nmrh_nufft_info %zu\nmrh_tv_term %zu\nmrh_tv_desc %zu\nmrh_cg_desc %zu\nmrh_cg_hist %zu\n"
+         "mrh_nufft3_desc %zu\n",
          sizeof(mrh_nufft_desc), sizeof(mrh_nufft_info), sizeof(mrh_tv_term), sizeof(mrh_tv_desc),
-         sizeof(mrh_cg_desc), sizeof(mrh_cg_hist));
+         sizeof(mrh_cg_desc), sizeof(mrh_cg_hist), sizeof(mrh_nufft3_desc));
   P(mrh_nufft_desc, beta) P(mrh_nufft_desc, k_offsets) P(mrh_nufft_desc, maps) P(mrh_nufft_desc, strip_cols)
   P(mrh_nufft_info, beta) P(mrh_nufft_info, workspace_bytes) P(mrh_nufft_info, coil_chunk)
   P(mrh_tv_desc, term) P(mrh_cg_desc, k_prob) P(mrh_cg_desc, c1) P(mrh_cg_desc, max_hist) P(mrh_cg_desc, restart_c)
+  P(mrh_cg_desc, ls_kind) P(mrh_cg_desc, c2) P(mrh_cg_desc, diag) P(mrh_cg_desc, gl_weight) P(mrh_cg_desc, gl_F)
+  P(mrh_cg_desc, gl_L) P(mrh_cg_desc, gl_prof)
+  P(mrh_nufft3_desc, beta) P(mrh_nufft3_desc, k_offsets) P(mrh_nufft3_desc, dcf_mode) P(mrh_nufft3_desc, maps)
   return 0;
 }
 """
@@ -75,11 +79,12 @@ def test_struct_layouts_match_c(tmp_path):
     out = dict(line.rsplit(" ", 1) for line in subprocess.run([str(exe)], capture_output=True, text=True,
                                                              check=True).stdout.split("\n") if line)
     sizes = {"mrh_nufft_desc": _lib.NufftDesc, "mrh_nufft_info": _lib.NufftInfo, "mrh_tv_term": _lib.TVTerm,
-             "mrh_tv_desc": _lib.TVDesc, "mrh_cg_desc": _lib.CGDesc, "mrh_cg_hist": _lib.CGHist}
+             "mrh_tv_desc": _lib.TVDesc, "mrh_cg_desc": _lib.CGDesc, "mrh_cg_hist": _lib.CGHist,
+             "mrh_nufft3_desc": _lib.Nufft3Desc}
     for cname, cls in sizes.items():
         assert int(out[cname]) == ctypes.sizeof(cls), cname
     offs = {"mrh_nufft_desc": _lib.NufftDesc, "mrh_nufft_info": _lib.NufftInfo, "mrh_tv_desc": _lib.TVDesc,
-            "mrh_cg_desc": _lib.CGDesc}
+            "mrh_cg_desc": _lib.CGDesc, "mrh_nufft3_desc": _lib.Nufft3Desc}
     for key, val in out.items():
         if "." in key:
             cname, field = key.split(".")

@@ -142,3 +142,48 @@ def test_ragged_frames_match_per_frame_nudft():
         s = slice(off[b], off[b + 1])
         ref = nudft_forward(maps, x[b:b + 1], traj[:, s], dcf[s])[0]
         assert rel_l2(y[C * off[b]:C * off[b + 1]].reshape(C, -1), ref) < 3e-5
+
+
+# ------------------------------------------------------------------ 3-D
+
+def _vol_problem(seed=5, S=(12, 10, 8), C=2, K=300):
+    rng = np.random.default_rng(seed)
+    maps = (rng.standard_normal((C,) + S) + 1j * rng.standard_normal((C,) + S)) * 0.5
+    x = rng.standard_normal(S) + 1j * rng.standard_normal(S)
+    traj = rng.uniform(-math.pi, math.pi, size=(3, K))
+    return maps, x, traj
+
+
+def test_kb_nufft3_matches_the_nudft():
+    """The 3-D restatement equals the FP64 3-D NUDFT to the KB approximation error (W = 6, osf 2),
+    and is adjoint to FP64 round-off."""
+    from oracle.kbnufft import KBNufft3, nudft3_forward
+    maps, x, traj = _vol_problem()
+    op = KBNufft3(maps, traj, np.array([0, traj.shape[1]]), None, 6, 2.0)
+    y = op.forward(x[None])
+    ref = nudft3_forward(maps, x, traj).reshape(-1)
+    assert np.linalg.norm(y - ref) / np.linalg.norm(ref) < 2e-3
+    yr = np.random.default_rng(9).standard_normal(y.shape) + 0j
+    lhs = np.vdot(y, yr)
+    rhs = np.vdot(x[None], op.adjoint(yr))
+    assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+def test_kb_nufft3_reduces_to_2d_on_a_flat_axis():
+    """A volume one voxel deep along the last axis with k_z = 0 samples is the 2-D problem: the 3-D
+    restatement equals the 2-D one times the z-axis KB factor (deapodisation and window at k = 0)."""
+    from oracle.kbnufft import KBNufft, KBNufft3, kb_beta, kb_ft, kb_phi, footprint
+    rng = np.random.default_rng(11)
+    S, C, K, W = (12, 10), 2, 200, 6
+    maps2 = rng.standard_normal((C,) + S) + 1j * rng.standard_normal((C,) + S)
+    x2 = rng.standard_normal(S) + 1j * rng.standard_normal(S)
+    t2 = rng.uniform(-math.pi, math.pi, size=(2, K))
+    y2 = KBNufft(maps2, t2, np.array([0, K]), None, W, 2.0).forward(x2[None])
+    maps3 = maps2[..., None]
+    t3 = np.concatenate([t2, np.zeros((1, K))])
+    y3 = KBNufft3(np.repeat(maps3, 1, axis=3), t3, np.array([0, K]), None, W, 2.0).forward(x2[..., None][None])
+    beta = kb_beta(W, 2.0)
+    # z: S = 1 (deapod 1 / Phi(0)), G = 2, footprint of u = 0 over nodes -2..3 folded onto 2 grid points
+    m0, w = footprint(np.zeros(1), W, beta)
+    fz = (1.0 / kb_ft(0.0, W, beta)) * w.sum()
+    assert np.linalg.norm(y3 - fz * y2) / np.linalg.norm(y2) < 1e-12
