@@ -358,6 +358,49 @@ def nufft_rate(wl, dev, reps=5):
                                          if plan.tuning else None)}
 
 
+def nufft3_rate(dev, S=128, C=8, spokes=1024, R=256, reps=5):
+    """3-D NUFFT pair on a kooshball trajectory: S^3 volume, C Gaussian-like coil maps, `spokes`
+    diametric spokes of R samples with directions on a Fibonacci sphere, KB W = 6, osf 2, no DCF.
+    Coil-volume pairs/s with inputs in HBM (HIP events on the launch stream)."""
+    import math as _m
+
+    import numpy as np
+    import torch
+
+    from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+    i = np.arange(spokes) + 0.5
+    phi = np.arccos(1.0 - 2.0 * i / spokes)
+    theta = _m.pi * (1.0 + 5.0 ** 0.5) * i
+    dirs = np.stack([np.sin(phi) * np.cos(theta), np.sin(phi) * np.sin(theta), np.cos(phi)])   # (3, spokes)
+    rad = (np.arange(R) - R / 2) / (R / 2) * _m.pi * 0.999
+    traj = (dirs[:, :, None] * rad[None, None, :]).reshape(3, -1).astype(np.float32)
+    ax = (np.arange(S) - S / 2) / S
+    X, Y, Z = np.meshgrid(ax, ax, ax, indexing="ij")
+    maps = np.stack([np.exp(-((X - 0.3 * _m.cos(2 * _m.pi * c / C)) ** 2 + (Y - 0.3 * _m.sin(2 * _m.pi * c / C)) ** 2
+                              + Z ** 2) / 0.2) for c in range(C)]).astype(np.complex64)
+    maps /= np.sqrt((np.abs(maps) ** 2).sum(axis=0, keepdims=True))
+    be = HipNUFFTAdapter(ndim=3, traj_units="rad", kb_width=6, osf=2.0, dcf_mode="none")
+    be.prepare((C, S, S, S), torch.from_numpy(traj), None, torch.from_numpy(maps).to(dev), device=dev)
+    plan = be.plan
+    x = torch.ones((1, S, S, S), dtype=torch.complex64, device=dev)
+    y = torch.empty(plan.n_k, dtype=torch.complex64, device=dev)
+    plan.forward(x, y)
+    plan.adjoint(y, x)
+    ts = []
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        plan.forward(x, y)
+        plan.adjoint(y, x)
+        e1.record()
+        e1.synchronize()
+        ts.append(e0.elapsed_time(e1))
+    ms = float(np.median(ts))
+    be.close()
+    return {"volume": f"{S}^3", "coils": C, "samples": int(traj.shape[1]), "ms_per_pair": round(ms, 3),
+            "coil_volume_pairs_per_s": round(C / (ms * 1e-3), 1)}
+
+
 def batched_uneven(args, dev):
     """Batched uneven workload: P slices x 20 frames at 256^2, 8 coils, 13..55 golden-angle spokes
     per frame x 512, all slices in the same launches (ragged k-space, per-slice stop), temporal TV 0.02.
@@ -547,7 +590,8 @@ def main():
     if not args.no_configs:
         print("bench: NUFFT rates of the single-frame configurations", file=sys.stderr, flush=True)
         configs = {"c1 (128^2, 8 coils, 64 spokes x 256, W=4)": nufft_rate(synth.config_c1(), dev)[1],
-                   "c3 (384^2, 16 coils, 402 spokes x 768, W=6)": nufft_rate(synth.config_c3(), dev)[1]}
+                   "c3 (384^2, 16 coils, 402 spokes x 768, W=6)": nufft_rate(synth.config_c3(), dev)[1],
+                   "3-D (128^3, 8 coils, 1024 kooshball spokes x 256, W=6)": nufft3_rate(dev)}
     c4 = None
     if not args.no_c4:
         c4 = batched_uneven(args, dev)

@@ -148,7 +148,7 @@ int mrh_nufft3_plan_destroy(mrh_nufft3_plan* plan);
 /* x (B, S0, S1, S2) complex64 -> y: per frame a (C, K_b) block (torchkb.py:203-270). */
 int mrh_nufft3_forward(mrh_nufft3_plan* plan, const void* x, void* y, void* stream);
 /* y -> x (B, S0, S1, S2), coil-combined with conj(maps) (torchkb.py:273-323).  The spread
- * adds floats atomically, so the result can differ in the last bits from run to run. */
+ * is a gather over footprint entries sorted by grid point at plan time (deterministic). */
 int mrh_nufft3_adjoint(mrh_nufft3_plan* plan, const void* y, void* x, void* stream);
 
 /* ------------------------------------------------------------------ reductions / objective */

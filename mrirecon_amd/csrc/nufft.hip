@@ -1001,9 +1001,10 @@ extern "C" int mrh_nufft_adjoint_stage(mrh_nufft_plan* p, int stage, const void*
 // Volumes (S0, S1, S2), trajectories (3, K_b) per frame.  Forward: every coil's
 // x * map * deapodisation placed centred in its zero grid (k3_pad writes every grid
 // point), one LDS FFT pass per axis (k3_fft_lines: one wavefront per grid line), KB
-// interpolation of W^3 grid points per sample (k3_interp).  Adjoint: float atomic
-// spreading into the zeroed grid (k3_spread), inverse FFT passes, crop, conj(map)
-// combine and deapodisation (k3_crop).  Same conventions as the 2-D plan: DC at grid
+// interpolation of W^3 grid points per sample (k3_interp).  Adjoint: the spreading
+// as its transpose, a gather per grid point over the frame's footprint entries sorted by
+// grid point at plan time (k3_gather: no atomics, deterministic, writes every grid point),
+// inverse FFT passes, crop, conj(map) combine and deapodisation (k3_crop).  Same conventions as the 2-D plan: DC at grid
 // index 0, image index i at (i - S/2) mod G, footprint nodes m0 = floor(u - W/2) + 1 ...
 
 struct Samp3 {
@@ -1023,12 +1024,16 @@ struct mrh_nufft3_plan {
   float2* d_tw[3];
   float2* d_maps;
   float2* d_grid;      // C G0 G1 G2
+  // adjoint spreading as a gather: per frame, the footprint entries of every grid point
+  int* d_goff;         // B (G0 G1 G2 + 1): entry range of grid point g of frame b (absolute)
+  int* d_gk;           // sample index within the frame
+  float* d_gw;         // KB weight x wAH
 };
 
 static int free_plan3(mrh_nufft3_plan* p) {
   if (!p) return MRH_OK;
   void* bufs[] = {p->d_samp, p->d_deap[0], p->d_deap[1], p->d_deap[2], p->d_tw[0], p->d_tw[1], p->d_tw[2],
-                  p->d_maps, p->d_grid};
+                  p->d_maps, p->d_grid, p->d_goff, p->d_gk, p->d_gw};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete[] p->h_koff;
   delete p;
@@ -1118,41 +1123,34 @@ __global__ __launch_bounds__(256) void k3_interp(const float2* __restrict__ grid
   y[(int64_t)c * K + k] = cscale(acc, s.wA);
 }
 
-// grid[c][...] += wAH y[c][k] w_x w_y w_z over the footprint (float atomics), one thread per (coil, sample).
-__global__ __launch_bounds__(256) void k3_spread(const float2* __restrict__ y, const Samp3* __restrict__ sp,
-                                                 float2* __restrict__ grid, KBPoly kb, int W, int G0, int G1,
-                                                 int G2, int C, int K) {
-  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (t >= (int64_t)C * K) return;
-  const int c = (int)(t / K), k = (int)(t - (int64_t)c * K);
-  const Samp3 s = sp[k];
-  const float2 v = cscale(y[(int64_t)c * K + k], s.wAH);
-  float w0[MRH_WMAX], w1[MRH_WMAX], w2[MRH_WMAX];
+// grid[c][g] = sum over the entries e of grid point g of gw[e] y[c][gk[e]]; coils in groups of 8
+// held in registers, so each entry is read once per group.
+__global__ __launch_bounds__(256) void k3_gather(const float2* __restrict__ y, const int* __restrict__ goff,
+                                                 const int* __restrict__ gk, const float* __restrict__ gw,
+                                                 float2* __restrict__ grid, int64_t gvol, int C, int K) {
+  for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < gvol; g += (int64_t)gridDim.x * 256) {
+    const int e0 = goff[g], e1 = goff[g + 1];
+    for (int c0 = 0; c0 < C; c0 += 8) {
+      float2 acc[8];
 #pragma unroll
-  for (int i = 0; i < MRH_WMAX; ++i) {
-    w0[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[0] - (float)i) : 0.f;
-    w1[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[1] - (float)i) : 0.f;
-    w2[i] = i < W ? kb_weight<MRH_KB_MAXNP>(kb, s.u[2] - (float)i) : 0.f;
-  }
-  float2* gc = grid + (int64_t)c * G0 * G1 * G2;
-  for (int i = 0; i < W; ++i) {
-    int i0 = s.m[0] + i; if (i0 >= G0) i0 -= G0;
-    for (int j = 0; j < W; ++j) {
-      int i1 = s.m[1] + j; if (i1 >= G1) i1 -= G1;
-      float2* row = gc + ((int64_t)i0 * G1 + i1) * G2;
-      const float2 vij = cscale(v, w0[i] * w1[j]);
-      for (int l = 0; l < W; ++l) {
-        int i2 = s.m[2] + l; if (i2 >= G2) i2 -= G2;
-        unsafeAtomicAdd(&row[i2].x, vij.x * w2[l]);
-        unsafeAtomicAdd(&row[i2].y, vij.y * w2[l]);
+      for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
+      for (int e = e0; e < e1; ++e) {
+        const int k = gk[e];
+        const float w = gw[e];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (c0 + q < C) {
+            const float2 v = y[(int64_t)(c0 + q) * K + k];
+            acc[q].x = fmaf(w, v.x, acc[q].x);
+            acc[q].y = fmaf(w, v.y, acc[q].y);
+          }
+        }
       }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (c0 + q < C) grid[(int64_t)(c0 + q) * gvol + g] = acc[q];
     }
   }
-}
-
-__global__ __launch_bounds__(256) void k3_zero(float2* __restrict__ v, int64_t n) {
-  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256)
-    v[t] = make_float2(0.f, 0.f);
 }
 
 // x[r] = deap(r) sum_c conj(map_c[r]) grid[c][(r - S/2) mod G].
@@ -1272,6 +1270,66 @@ extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan*
     }
   }
   int rc = upload(&p->d_samp, samp.data(), samp.size());
+  // gather lists: every footprint entry (grid point, sample, weight x wAH), counting-sorted by
+  // grid point within each frame; the weights are the KB series in FP64
+  {
+    const int W = d->W;
+    const int64_t gvol = (int64_t)d->G0 * d->G1 * d->G2;
+    const int64_t n_ent = p->Ktot * W * W * W;
+    if (n_ent >= ((int64_t)1 << 31) || (int64_t)B * (gvol + 1) >= ((int64_t)1 << 31)) {
+      free_plan3(p);
+      mrh_set_error("nufft3: %lld footprint entries / %lld grid offsets exceed 32-bit indexing",
+                    (long long)n_ent, (long long)B * (gvol + 1));
+      return MRH_EUNSUP;
+    }
+    auto kbw = [&](double dd) {
+      const double t = 2.0 * dd / W, qq = q * std::max(1.0 - t * t, 0.0);
+      double v = coef[np - 1];
+      for (int n = np - 2; n >= 0; --n) v = v * qq + coef[n];
+      return v;
+    };
+    std::vector<int> goff((size_t)(B * (gvol + 1)), 0);
+    std::vector<int> gk((size_t)std::max<int64_t>(1, n_ent));
+    std::vector<float> gw((size_t)std::max<int64_t>(1, n_ent));
+    int64_t base = 0;
+    std::vector<int> fill;
+    for (int b = 0; b < B && !rc; ++b) {
+      const int64_t k0 = p->h_koff[b], Kb = p->h_koff[b + 1] - k0;
+      int* off = goff.data() + (size_t)b * (gvol + 1);
+      auto gidx = [&](const Samp3& s, int i, int j, int l) {
+        int i0 = s.m[0] + i; if (i0 >= d->G0) i0 -= d->G0;
+        int i1 = s.m[1] + j; if (i1 >= d->G1) i1 -= d->G1;
+        int i2 = s.m[2] + l; if (i2 >= d->G2) i2 -= d->G2;
+        return ((int64_t)i0 * d->G1 + i1) * d->G2 + i2;
+      };
+      for (int64_t k = 0; k < Kb; ++k)
+        for (int i = 0; i < W; ++i)
+          for (int j = 0; j < W; ++j)
+            for (int l = 0; l < W; ++l) off[gidx(samp[(size_t)(k0 + k)], i, j, l) + 1]++;
+      off[0] = (int)base;
+      for (int64_t g = 0; g < gvol; ++g) off[g + 1] += off[g];
+      fill.assign(off, off + gvol);
+      for (int64_t k = 0; k < Kb; ++k) {
+        const Samp3& s = samp[(size_t)(k0 + k)];
+        for (int i = 0; i < W; ++i) {
+          const double wi = kbw(s.u[0] - i);
+          for (int j = 0; j < W; ++j) {
+            const double wij = wi * kbw(s.u[1] - j);
+            for (int l = 0; l < W; ++l) {
+              const int64_t g = gidx(s, i, j, l);
+              const int e = fill[(size_t)g]++;
+              gk[(size_t)e] = (int)k;
+              gw[(size_t)e] = (float)(wij * kbw(s.u[2] - l) * s.wAH);
+            }
+          }
+        }
+      }
+      base = off[gvol];
+    }
+    if (!rc) rc = upload(&p->d_goff, goff.data(), goff.size());
+    if (!rc) rc = upload(&p->d_gk, gk.data(), gk.size());
+    if (!rc) rc = upload(&p->d_gw, gw.data(), gw.size());
+  }
   for (int a = 0; a < 3 && !rc; ++a) {
     std::vector<float> dv(p->S[a]);
     for (int i = 0; i < p->S[a]; ++i) dv[i] = (float)(1.0 / kb_ft((double)(i - p->S[a] / 2) / p->G[a], d->W, beta));
@@ -1319,15 +1377,10 @@ extern "C" int mrh_nufft3_adjoint(mrh_nufft3_plan* p, const void* y, void* x, vo
   const int64_t svol = (int64_t)p->S[0] * p->S[1] * p->S[2], gtot = (int64_t)p->C * p->G[0] * p->G[1] * p->G[2];
   for (int b = 0; b < p->B; ++b) {
     const int64_t k0 = p->h_koff[b], Kb = p->h_koff[b + 1] - k0;
-    hipLaunchKernelGGL(k3_zero, dim3(grid_blocks(gtot)), dim3(256), 0, st, p->d_grid, gtot);
+    const int64_t gvol = gtot / p->C;
+    hipLaunchKernelGGL(k3_gather, dim3(grid_blocks(gvol)), dim3(256), 0, st, (const float2*)y + (int64_t)p->C * k0,
+                       p->d_goff + (int64_t)b * (gvol + 1), p->d_gk, p->d_gw, p->d_grid, gvol, p->C, (int)Kb);
     MRH_CHECK_LAUNCH();
-    if (Kb > 0) {
-      const int64_t n = (int64_t)p->C * Kb;
-      hipLaunchKernelGGL(k3_spread, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                         (const float2*)y + (int64_t)p->C * k0, p->d_samp + k0, p->d_grid, p->kb, p->W, p->G[0],
-                         p->G[1], p->G[2], p->C, (int)Kb);
-      MRH_CHECK_LAUNCH();
-    }
     int rc = fft3<true>(p, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k3_crop, dim3(grid_blocks(svol)), dim3(256), 0, st, p->d_grid, p->d_maps, p->d_deap[0],
