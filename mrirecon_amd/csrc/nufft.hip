@@ -1028,12 +1028,13 @@ struct mrh_nufft3_plan {
   int* d_goff;         // B (G0 G1 G2 + 1): entry range of grid point g of frame b (absolute)
   int* d_gk;           // sample index within the frame
   float* d_gw;         // KB weight x wAH
+  float2* d_yt;        // one frame's k-space sample-major (K_b, C), read by the gather
 };
 
 static int free_plan3(mrh_nufft3_plan* p) {
   if (!p) return MRH_OK;
   void* bufs[] = {p->d_samp, p->d_deap[0], p->d_deap[1], p->d_deap[2], p->d_tw[0], p->d_tw[1], p->d_tw[2],
-                  p->d_maps, p->d_grid, p->d_goff, p->d_gk, p->d_gw};
+                  p->d_maps, p->d_grid, p->d_goff, p->d_gk, p->d_gw, p->d_yt};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete[] p->h_koff;
   delete p;
@@ -1086,6 +1087,37 @@ __global__ __launch_bounds__(256) void k3_fft_lines(float2* __restrict__ v, cons
   }
 }
 
+// The same transform for a middle axis whose inner extent is a multiple of TI: a workgroup stages
+// an N x TI tile of TI neighbouring lines (rows of TI contiguous elements, so the global reads and
+// writes are whole 8 TI-byte segments) in LDS, line-major with an odd line stride (conflict-free
+// staging), and its wavefronts transform the tile's lines in place.
+template <int N, bool INV, int TI>
+__global__ __launch_bounds__(256) void k3_fft_tiles(float2* __restrict__ v, const float2* __restrict__ tw0,
+                                                    int64_t n_inner, int64_t n_tiles) {
+  constexpr int LS = fft_padded(N) + 1;
+  __shared__ float2 tile[TI * LS];
+  __shared__ float2 tw[N];
+  for (int i = threadIdx.x; i < N; i += 256) tw[i] = tw0[i];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t per_outer = n_inner / TI;
+  for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+    const int64_t outer = t / per_outer, inner0 = (t - outer * per_outer) * TI;
+    float2* base = v + outer * N * n_inner + inner0;
+    __syncthreads();                       // twiddles staged / the previous tile written back
+    for (int e = threadIdx.x; e < N * TI; e += 256) {
+      const int i = e / TI, j = e - i * TI;
+      tile[j * LS + PIDX(i)] = base[(int64_t)i * n_inner + j];
+    }
+    __syncthreads();
+    for (int j = wv; j < TI; j += 4) fft_lds_wave<N, INV>(tile + j * LS, tw, lane);
+    __syncthreads();
+    for (int e = threadIdx.x; e < N * TI; e += 256) {
+      const int i = e / TI, j = e - i * TI;
+      base[(int64_t)i * n_inner + j] = tile[j * LS + PIDX(i)];
+    }
+  }
+}
+
 // y[c][k] = wA sum over the W^3 footprint of w_x w_y w_z grid[c][...], one thread per (coil, sample).
 __global__ __launch_bounds__(256) void k3_interp(const float2* __restrict__ grid, const Samp3* __restrict__ sp,
                                                  float2* __restrict__ y, KBPoly kb, int W, int G0, int G1, int G2,
@@ -1123,11 +1155,22 @@ __global__ __launch_bounds__(256) void k3_interp(const float2* __restrict__ grid
   y[(int64_t)c * K + k] = cscale(acc, s.wA);
 }
 
-// grid[c][g] = sum over the entries e of grid point g of gw[e] y[c][gk[e]]; coils in groups of 8
+// yt[k][c] = y[c][k]: one frame's k-space sample-major, so that the gather reads a sample's
+// coils from one cache line.
+__global__ __launch_bounds__(256) void k3_transpose_y(const float2* __restrict__ y, float2* __restrict__ yt, int C,
+                                                      int K) {
+  const int64_t n = (int64_t)C * K;
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
+    const int k = (int)(t / C), c = (int)(t - (int64_t)k * C);
+    yt[t] = y[(int64_t)c * K + k];
+  }
+}
+
+// grid[c][g] = sum over the entries e of grid point g of gw[e] yt[gk[e]][c]; coils in groups of 8
 // held in registers, so each entry is read once per group.
-__global__ __launch_bounds__(256) void k3_gather(const float2* __restrict__ y, const int* __restrict__ goff,
+__global__ __launch_bounds__(256) void k3_gather(const float2* __restrict__ yt, const int* __restrict__ goff,
                                                  const int* __restrict__ gk, const float* __restrict__ gw,
-                                                 float2* __restrict__ grid, int64_t gvol, int C, int K) {
+                                                 float2* __restrict__ grid, int64_t gvol, int C) {
   for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < gvol; g += (int64_t)gridDim.x * 256) {
     const int e0 = goff[g], e1 = goff[g + 1];
     for (int c0 = 0; c0 < C; c0 += 8) {
@@ -1135,12 +1178,12 @@ __global__ __launch_bounds__(256) void k3_gather(const float2* __restrict__ y, c
 #pragma unroll
       for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
       for (int e = e0; e < e1; ++e) {
-        const int k = gk[e];
+        const float2* ys = yt + (int64_t)gk[e] * C + c0;
         const float w = gw[e];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           if (c0 + q < C) {
-            const float2 v = y[(int64_t)(c0 + q) * K + k];
+            const float2 v = ys[q];
             acc[q].x = fmaf(w, v.x, acc[q].x);
             acc[q].y = fmaf(w, v.y, acc[q].y);
           }
@@ -1174,10 +1217,17 @@ __global__ __launch_bounds__(256) void k3_crop(const float2* __restrict__ grid, 
 
 template <bool INV>
 static int launch_fft_axis(int N, float2* v, const float2* tw, int64_t n_inner, int64_t n_lines, hipStream_t st) {
-  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(65535, (n_lines + 3) / 4));
+  constexpr int TI = 16;
+  const bool tiled = n_inner % TI == 0;
+  const int64_t n_tiles = n_lines / TI;
+  const int blocks = tiled ? (int)std::max<int64_t>(1, std::min<int64_t>(65535, n_tiles))
+                           : (int)std::max<int64_t>(1, std::min<int64_t>(65535, (n_lines + 3) / 4));
 #define MRH_FFT3_CASE(n)                                                                                   \
   case n:                                                                                                 \
-    hipLaunchKernelGGL((k3_fft_lines<n, INV>), dim3(blocks), dim3(256), 0, st, v, tw, n_inner, n_lines);    \
+    if (tiled)                                                                                            \
+      hipLaunchKernelGGL((k3_fft_tiles<n, INV, TI>), dim3(blocks), dim3(256), 0, st, v, tw, n_inner, n_tiles); \
+    else                                                                                                  \
+      hipLaunchKernelGGL((k3_fft_lines<n, INV>), dim3(blocks), dim3(256), 0, st, v, tw, n_inner, n_lines);  \
     break;
   switch (N) {
     MRH_FFT3_CASE(32) MRH_FFT3_CASE(64) MRH_FFT3_CASE(96) MRH_FFT3_CASE(128) MRH_FFT3_CASE(192)
@@ -1326,6 +1376,9 @@ extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan*
       }
       base = off[gvol];
     }
+    int64_t kmax = 1;
+    for (int b = 0; b < B; ++b) kmax = std::max<int64_t>(kmax, p->h_koff[b + 1] - p->h_koff[b]);
+    if (!rc) rc = upload<float2>(&p->d_yt, nullptr, (size_t)(kmax * d->C));
     if (!rc) rc = upload(&p->d_goff, goff.data(), goff.size());
     if (!rc) rc = upload(&p->d_gk, gk.data(), gk.size());
     if (!rc) rc = upload(&p->d_gw, gw.data(), gw.size());
@@ -1378,8 +1431,13 @@ extern "C" int mrh_nufft3_adjoint(mrh_nufft3_plan* p, const void* y, void* x, vo
   for (int b = 0; b < p->B; ++b) {
     const int64_t k0 = p->h_koff[b], Kb = p->h_koff[b + 1] - k0;
     const int64_t gvol = gtot / p->C;
-    hipLaunchKernelGGL(k3_gather, dim3(grid_blocks(gvol)), dim3(256), 0, st, (const float2*)y + (int64_t)p->C * k0,
-                       p->d_goff + (int64_t)b * (gvol + 1), p->d_gk, p->d_gw, p->d_grid, gvol, p->C, (int)Kb);
+    if (Kb > 0) {
+      hipLaunchKernelGGL(k3_transpose_y, dim3(grid_blocks((int64_t)p->C * Kb)), dim3(256), 0, st,
+                         (const float2*)y + (int64_t)p->C * k0, p->d_yt, p->C, (int)Kb);
+      MRH_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(k3_gather, dim3(grid_blocks(gvol)), dim3(256), 0, st, (const float2*)p->d_yt,
+                       p->d_goff + (int64_t)b * (gvol + 1), p->d_gk, p->d_gw, p->d_grid, gvol, p->C);
     MRH_CHECK_LAUNCH();
     int rc = fft3<true>(p, st);
     if (rc) return rc;
