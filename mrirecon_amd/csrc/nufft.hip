@@ -1029,12 +1029,19 @@ struct mrh_nufft3_plan {
   int* d_gk;           // sample index within the frame
   float* d_gw;         // KB weight x wAH
   float2* d_yt;        // one frame's k-space sample-major (K_b, C), read by the gather
+  // grid points with more than GATHER_PART entries: their lists in parts of at most that many
+  // (summed into d_part, then added in part order), so no thread walks a long list
+  int4* d_heavy;       // per heavy point: (g, first part, end part, 0), all frames
+  int2* d_parts;       // per part: entry range [begin, end)
+  float2* d_part;      // partial sums (max parts of a frame) x C
+  std::vector<int64_t> h_hoff, h_poff;   // per frame: heavy-point / part offsets (B + 1)
 };
 
 static int free_plan3(mrh_nufft3_plan* p) {
   if (!p) return MRH_OK;
   void* bufs[] = {p->d_samp, p->d_deap[0], p->d_deap[1], p->d_deap[2], p->d_tw[0], p->d_tw[1], p->d_tw[2],
-                  p->d_maps, p->d_grid, p->d_goff, p->d_gk, p->d_gw, p->d_yt};
+                  p->d_maps, p->d_grid, p->d_goff, p->d_gk, p->d_gw, p->d_yt, p->d_heavy, p->d_parts,
+                  p->d_part};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete[] p->h_koff;
   delete p;
@@ -1166,33 +1173,66 @@ __global__ __launch_bounds__(256) void k3_transpose_y(const float2* __restrict__
   }
 }
 
+#define GATHER_PART 64   // longest entry list one gather thread walks
+
+// out[q * ostride] = sum over entries [e0, e1) of gw[e] yt[gk[e]][c0 + q] for the coils of one group.
+__device__ __forceinline__ void gather_range(const float2* __restrict__ yt, const int* __restrict__ gk,
+                                             const float* __restrict__ gw, int e0, int e1, int c0, int C,
+                                             float2* __restrict__ out, int64_t ostride) {
+  float2 acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
+  for (int e = e0; e < e1; ++e) {
+    const float2* ys = yt + (int64_t)gk[e] * C + c0;
+    const float w = gw[e];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      if (c0 + q < C) {
+        const float2 v = ys[q];
+        acc[q].x = fmaf(w, v.x, acc[q].x);
+        acc[q].y = fmaf(w, v.y, acc[q].y);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (c0 + q < C) out[q * ostride] = acc[q];
+}
+
 // grid[c][g] = sum over the entries e of grid point g of gw[e] yt[gk[e]][c]; coils in groups of 8
-// held in registers, so each entry is read once per group.
+// held in registers, so each entry is read once per group.  Points with more than GATHER_PART
+// entries are left to k3_gather_parts / k3_gather_heavy.
 __global__ __launch_bounds__(256) void k3_gather(const float2* __restrict__ yt, const int* __restrict__ goff,
                                                  const int* __restrict__ gk, const float* __restrict__ gw,
                                                  float2* __restrict__ grid, int64_t gvol, int C) {
   for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < gvol; g += (int64_t)gridDim.x * 256) {
     const int e0 = goff[g], e1 = goff[g + 1];
-    for (int c0 = 0; c0 < C; c0 += 8) {
-      float2 acc[8];
-#pragma unroll
-      for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
-      for (int e = e0; e < e1; ++e) {
-        const float2* ys = yt + (int64_t)gk[e] * C + c0;
-        const float w = gw[e];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          if (c0 + q < C) {
-            const float2 v = ys[q];
-            acc[q].x = fmaf(w, v.x, acc[q].x);
-            acc[q].y = fmaf(w, v.y, acc[q].y);
-          }
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (c0 + q < C) grid[(int64_t)(c0 + q) * gvol + g] = acc[q];
-    }
+    if (e1 - e0 > GATHER_PART) continue;
+    for (int c0 = 0; c0 < C; c0 += 8) gather_range(yt, gk, gw, e0, e1, c0, C, grid + (int64_t)c0 * gvol + g, gvol);
+  }
+}
+
+// One part of a long list: part[i][c] = its entries' sum (parts of a frame, in list order).
+__global__ __launch_bounds__(256) void k3_gather_parts(const float2* __restrict__ yt, const int2* __restrict__ parts,
+                                                       const int* __restrict__ gk, const float* __restrict__ gw,
+                                                       float2* __restrict__ part, int n_parts, int C) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_parts) return;
+  const int2 r = parts[i];
+  for (int c0 = 0; c0 < C; c0 += 8) gather_range(yt, gk, gw, r.x, r.y, c0, C, part + (int64_t)i * C + c0, 1);
+}
+
+// A long list's grid point: the sum of its parts in order (deterministic).
+__global__ __launch_bounds__(256) void k3_gather_heavy(const int4* __restrict__ heavy, const float2* __restrict__ part,
+                                                       float2* __restrict__ grid, int64_t gvol, int n_heavy, int p_base,
+                                                       int C) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n_heavy) return;
+  const int4 h = heavy[i];
+  for (int c = 0; c < C; ++c) {
+    float2 acc = make_float2(0.f, 0.f);
+    for (int q = h.y; q < h.z; ++q) acc = cadd(acc, part[(int64_t)(q - p_base) * C + c]);
+    grid[(int64_t)c * gvol + h.x] = acc;
   }
 }
 
@@ -1263,8 +1303,7 @@ extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan*
   MRH_REQUIRE(d->W >= 2 && d->W <= MRH_WMAX, MRH_EUNSUP, "KB width %d outside [2, %d]", d->W, MRH_WMAX);
   MRH_REQUIRE(d->C >= 1 && d->B >= 1, MRH_EINVAL, "C, B must be >= 1");
   MRH_REQUIRE(d->k_offsets && d->traj && d->maps, MRH_EINVAL, "k_offsets, traj (host) and maps (device) are required");
-  mrh_nufft3_plan* p = new mrh_nufft3_plan();
-  std::memset(p, 0, sizeof(*p));
+  mrh_nufft3_plan* p = new mrh_nufft3_plan();   // value-initialised: pointers null, vectors empty
   p->S[0] = d->S0; p->S[1] = d->S1; p->S[2] = d->S2;
   p->G[0] = d->G0; p->G[1] = d->G1; p->G[2] = d->G2;
   p->W = d->W; p->C = d->C; p->B = d->B;
@@ -1376,6 +1415,29 @@ extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan*
       }
       base = off[gvol];
     }
+    // long lists, split into parts
+    std::vector<int4> heavy;
+    std::vector<int2> parts;
+    int64_t max_parts = 1;
+    p->h_hoff.assign(1, 0);
+    p->h_poff.assign(1, 0);
+    for (int b = 0; b < B; ++b) {
+      const int* off = goff.data() + (size_t)b * (gvol + 1);
+      const int64_t p_first = (int64_t)parts.size();
+      for (int64_t g = 0; g < gvol; ++g) {
+        const int e0 = off[g], e1 = off[g + 1];
+        if (e1 - e0 <= GATHER_PART) continue;
+        const int q0 = (int)parts.size();
+        for (int e = e0; e < e1; e += GATHER_PART) parts.push_back(make_int2(e, std::min(e1, e + GATHER_PART)));
+        heavy.push_back(make_int4((int)g, q0, (int)parts.size(), 0));
+      }
+      max_parts = std::max<int64_t>(max_parts, (int64_t)parts.size() - p_first);
+      p->h_hoff.push_back((int64_t)heavy.size());
+      p->h_poff.push_back((int64_t)parts.size());
+    }
+    if (!rc) rc = upload(&p->d_heavy, heavy.empty() ? nullptr : heavy.data(), std::max<size_t>(1, heavy.size()));
+    if (!rc) rc = upload(&p->d_parts, parts.empty() ? nullptr : parts.data(), std::max<size_t>(1, parts.size()));
+    if (!rc) rc = upload<float2>(&p->d_part, nullptr, (size_t)(max_parts * d->C));
     int64_t kmax = 1;
     for (int b = 0; b < B; ++b) kmax = std::max<int64_t>(kmax, p->h_koff[b + 1] - p->h_koff[b]);
     if (!rc) rc = upload<float2>(&p->d_yt, nullptr, (size_t)(kmax * d->C));
@@ -1439,6 +1501,15 @@ extern "C" int mrh_nufft3_adjoint(mrh_nufft3_plan* p, const void* y, void* x, vo
     hipLaunchKernelGGL(k3_gather, dim3(grid_blocks(gvol)), dim3(256), 0, st, (const float2*)p->d_yt,
                        p->d_goff + (int64_t)b * (gvol + 1), p->d_gk, p->d_gw, p->d_grid, gvol, p->C);
     MRH_CHECK_LAUNCH();
+    const int n_parts = (int)(p->h_poff[b + 1] - p->h_poff[b]), n_heavy = (int)(p->h_hoff[b + 1] - p->h_hoff[b]);
+    if (n_heavy > 0) {
+      hipLaunchKernelGGL(k3_gather_parts, dim3((n_parts + 255) / 256), dim3(256), 0, st, (const float2*)p->d_yt,
+                         p->d_parts + p->h_poff[b], p->d_gk, p->d_gw, p->d_part, n_parts, p->C);
+      MRH_CHECK_LAUNCH();
+      hipLaunchKernelGGL(k3_gather_heavy, dim3((n_heavy + 255) / 256), dim3(256), 0, st, p->d_heavy + p->h_hoff[b],
+                         (const float2*)p->d_part, p->d_grid, gvol, n_heavy, (int)p->h_poff[b], p->C);
+      MRH_CHECK_LAUNCH();
+    }
     int rc = fft3<true>(p, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k3_crop, dim3(grid_blocks(svol)), dim3(256), 0, st, p->d_grid, p->d_maps, p->d_deap[0],
