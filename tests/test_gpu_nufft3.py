@@ -119,3 +119,25 @@ def test_nufft3_kooshball_centre_lists():
     print(f"[3-D kooshball] centre grid point entries {centre_entries}, adjoint rel={ea:.2e}")
     assert ea <= 1e-5
     assert torch.equal(xa, xb)
+
+
+def test_nufft3_through_the_reference_front_end():
+    """NUFFT(maps (C, X, Y, Z), traj, dcf, axis, NUFFTConfig(ndim=3)) -- the reference's public entry
+    (graspcg/nufft/api.py:85-89) -- in a user axis order equals the adapter's operator."""
+    from mrirecon_amd.nufft.api import NUFFT, NUFFTConfig
+    from mrirecon_amd.nufft.layout import AxisSpec
+    S, C, K = (16, 16, 32), 2, 500
+    be, ora, x, off = problem(S, C, [K], seed=81)
+    traj = torch.from_numpy(be._traj_flat.reshape(1, 3, K).copy()).to(DEV)
+    dcf = torch.from_numpy(be._dcf_flat.reshape(1, K).copy()).to(DEV)
+    axis = AxisSpec(image=("B", "coil", "X", "Y", "Z"), kspace=("B", "K", "coil"), image_fft=("X", "Y", "Z"))
+    op = NUFFT(be._maps, traj, dcf, axis=axis, config=NUFFTConfig(ndim=3, traj_units="rad"), device=DEV)
+    xg = torch.from_numpy(x).to(DEV).reshape(1, 1, *S)
+    yu = op.A(xg)                                             # (1, K, C) user order
+    assert tuple(yu.shape) == (1, K, C)
+    y1 = be.A(xg)
+    assert torch.equal(yu.permute(0, 2, 1).contiguous(), y1)
+    xb = op.AH(yu)
+    torch.cuda.synchronize()
+    assert tuple(xb.shape) == (1, 1) + S
+    assert rel_l2(xb.cpu().numpy(), be.AH(y1).cpu().numpy()) <= 1e-6
