@@ -208,7 +208,7 @@ def batched_grasp(args, dev, net, seeds, phantoms):
     from mrirecon_amd import NLCGEngine, RegManager, TVND, TVParams, synth
     from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
     from mrirecon_amd.train_warmstart import GRASP, spokes_for
-    from mrirecon_amd.warmstart import ObjectiveEval, warm_start
+    from mrirecon_amd.warmstart import ObjectiveEval, launch_warm_start
 
     g = GRASP
     P, F, S, C = len(seeds), g["F"], g["S"], g["C"]
@@ -259,10 +259,11 @@ def batched_grasp(args, dev, net, seeds, phantoms):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan.adjoint(y, xa)
-    x0, acc, _, _ = warm_start(net, obj, obj_adj, xa, nvol=P, out=x0)
-    solve(x0, obj.Ax if acc.all() else None)
+    ch = launch_warm_start(net, obj, obj_adj, xa, nvol=P, out=x0)
+    solve(ch.x0, ch.Ax0)
     torch.cuda.synchronize()
     t_warm = time.perf_counter() - t0
+    acc, _, _ = ch.read()
     _, it_w = eng.status()
     return {"volumes": P, "frames": P * F, "plain_seconds": round(t_plain, 4), "warm_seconds": round(t_warm, 4),
             "speedup": round(t_plain / t_warm, 3), "plain_volumes_per_second": round(P / t_plain, 2),
@@ -411,7 +412,7 @@ def batched_uneven(args, dev):
     import torch
 
     from mrirecon_amd import NLCGEngine, RegManager, TVND, TVParams, synth
-    from mrirecon_amd.warmstart import ObjectiveEval, UNet2DT, warm_start
+    from mrirecon_amd.warmstart import ObjectiveEval, UNet2DT, launch_warm_start
 
     print(f"bench: batched uneven workload ({args.c4_slices} slices x 20 frames)", file=sys.stderr, flush=True)
     wl = synth.config_c4(n_slices=args.c4_slices)
@@ -462,10 +463,11 @@ def batched_uneven(args, dev):
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     plan.adjoint(y, xa)
-    x0, acc, f_w, f_a = warm_start(net, obj, obj_adj, xa, nvol=P, out=x0)
-    solve(x0, obj.Ax if acc.all() else None)
+    ch = launch_warm_start(net, obj, obj_adj, xa, nvol=P, out=x0)
+    solve(ch.x0, ch.Ax0)
     torch.cuda.synchronize()
     t_warm = time.perf_counter() - t0
+    acc, f_w, f_a = ch.read()
     st_w, it_w = eng.status()
     h_w = eng.history()
     f_warm = np.array([rows[-1]["f_step"] for rows in h_w])

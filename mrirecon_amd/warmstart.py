@@ -5,7 +5,9 @@ There is no reference code for the learned part.  The reference's only hook is
 (graspcg/solvers/cg.py:48,142); this module fills it.  The warm start replaces
 only the initial point: NLCG still runs to its own stop rule, and
 :func:`select_warm_start` falls back to the adjoint start for every problem
-whose objective at the network output is worse than at the adjoint image.
+whose objective at the network output is worse than at the adjoint image
+(:func:`launch_warm_start` makes the same choice on the device, without a host
+read between the network and the solver).
 
 Network (``levels`` L, ``base`` channels c_l = base 2^l, BF16 activations
 channels-last (volumes * frames, H, W, C), FP32 accumulation):
@@ -444,8 +446,8 @@ class ObjectiveEval:
         self.r = torch.empty_like(self.y)
         dev = y.device
         self.work = torch.zeros(int(self.lib.mrh_reduce_workspace_len()), dtype=torch.float64, device=dev)
-        self.e_data = torch.zeros(self.P, dtype=torch.float64, device=dev)
-        self.e_tv = torch.zeros(self.P, dtype=torch.float64, device=dev)
+        self.e = torch.zeros(2 * self.P, dtype=torch.float64, device=dev)     # data terms, then TV terms
+        self.e_data, self.e_tv = self.e[:self.P], self.e[self.P:]
         S0, S1 = plan.S
         if regm is not None and regm.graph() is not None:
             raise NotImplementedError("objective evaluation covers TV terms only (no graph-Laplacian term)")
@@ -471,7 +473,13 @@ class ObjectiveEval:
 
     def read(self) -> np.ndarray:
         """f per problem of the last launched evaluation (synchronises with its stream)."""
-        return 0.5 * self.e_data.cpu().numpy() + self.e_tv.cpu().numpy()
+        e = self.e.cpu().numpy()
+        return 0.5 * e[:self.P] + e[self.P:]
+
+    def value_device(self) -> torch.Tensor:
+        """f per problem as an FP64 device tensor (no host synchronisation).  Equal bit for
+        bit to :meth:`read`: the product by 0.5 is exact, and the sum is one rounded add."""
+        return self.e_data * 0.5 + self.e_tv
 
 
 class ObjectiveGrad:
@@ -530,16 +538,34 @@ def select_warm_start(objective: ObjectiveEval, x_adj: torch.Tensor, x_warm: tor
     return x0, acc, f_w, f_a
 
 
-def warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveEval, x_adj: torch.Tensor,
-               nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None
-               ) -> Tuple[torch.Tensor, np.ndarray, np.ndarray, np.ndarray]:
-    """Network + fallback check with f(x_adj) evaluated on a side stream while the network runs.
+class WarmStartChoice:
+    """The per-problem start chosen on the device by :func:`launch_warm_start`.
+
+    ``x0`` is the start image and ``Ax0`` (the buffer ``objective.Ax``) holds A x0 for
+    every problem, accepted or not.  :meth:`read` synchronises and returns
+    (accepted[P] bool, f_warm[P], f_adj[P])."""
+
+    def __init__(self, x0: torch.Tensor, Ax0: torch.Tensor, f_w: torch.Tensor, f_a: torch.Tensor):
+        self.x0, self.Ax0, self._f_w, self._f_a = x0, Ax0, f_w, f_a
+
+    def read(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
+        f_w = self._f_w.cpu().numpy()
+        f_a = self._f_a.cpu().numpy()
+        return f_w <= f_a, f_w, f_a
+
+
+def launch_warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveEval, x_adj: torch.Tensor,
+                      nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None) -> WarmStartChoice:
+    """Network + fallback check with f(x_adj) evaluated on a side stream while the network runs,
+    and the choice per problem made on the device, so nothing waits for the host after the
+    network's input scales are read (the solver's first iteration queues right behind).
 
     ``objective`` and ``objective_adj`` are two evaluators of the same problem (separate
     buffers).  The network (MFMA-bound) and the adjoint image's objective (a forward NUFFT,
     memory / LDS-bound) overlap; f(x_warm) runs after both, since the two NUFFTs share the
-    plan's workspace.  Same result as ``select_warm_start(objective, x_adj, net.infer(...))``,
-    and ``objective.Ax`` holds A x0 when every problem accepted."""
+    plan's workspace.  Problem p starts from x_warm unless f(x_warm) > f(x_adj); its slice of
+    ``objective.Ax`` is then replaced by A x_adj from ``objective_adj``, so ``objective.Ax``
+    holds A x0 for every problem."""
     main = torch.cuda.current_stream(x_adj.device)
     side = side_stream or torch.cuda.Stream(device=x_adj.device)
     ready = torch.cuda.Event()
@@ -550,13 +576,25 @@ def warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveE
     x_warm = net.infer(x_adj, nvol=nvol)
     main.wait_stream(side)
     objective.launch(x_warm)
-    f_a = objective_adj.read()
-    f_w = objective.read()
-    acc = f_w <= f_a
-    x0 = x_warm if out is None else out.copy_(x_warm)
-    if not acc.all():
-        per = x_adj.shape[0] // objective.P
-        for p in np.nonzero(~acc)[0]:
-            x0[p * per:(p + 1) * per].copy_(x_adj[p * per:(p + 1) * per])
-    return x0, acc, f_w, f_a
+    f_w, f_a = objective.value_device(), objective_adj.value_device()
+    acc = f_w <= f_a                                       # device bool per problem
+    x0 = x_warm if out is None else out
+    per = x_adj.shape[0] // objective.P
+    for p in range(objective.P):
+        xs = slice(p * per, (p + 1) * per)
+        torch.where(acc[p], x_warm[xs], x_adj[xs], out=x0[xs])
+        a, b = objective.k_prob[p], objective.k_prob[p + 1]
+        torch.where(acc[p], objective.Ax[a:b], objective_adj.Ax[a:b], out=objective.Ax[a:b])
+    return WarmStartChoice(x0, objective.Ax, f_w, f_a)
+
+
+def warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveEval, x_adj: torch.Tensor,
+               nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None
+               ) -> Tuple[torch.Tensor, np.ndarray, np.ndarray, np.ndarray]:
+    """:func:`launch_warm_start` followed by the host read of its choice: returns
+    (x0, accepted[P] bool, f_warm[P], f_adj[P]), the result of
+    ``select_warm_start(objective, x_adj, net.infer(...))``; ``objective.Ax`` holds A x0."""
+    ch = launch_warm_start(net, objective, objective_adj, x_adj, nvol=nvol, out=out, side_stream=side_stream)
+    acc, f_w, f_a = ch.read()
+    return ch.x0, acc, f_w, f_a
 

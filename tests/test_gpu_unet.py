@@ -432,11 +432,12 @@ def test_trained_warm_start_gain_on_held_out_grasp_volumes():
 
 
 def test_overlapped_warm_start_equals_sequential_selection():
-    """warm_start (network on the main stream, f(A^H y) on a side stream) returns what
-    select_warm_start returns for the same network output."""
+    """warm_start (network on the main stream, f(A^H y) on a side stream, the choice made on
+    the device) returns what select_warm_start returns for the same network output, and
+    leaves A x0 in the objective's Ax buffer for rejected and accepted starts alike."""
     from mrirecon_amd import RegManager, TVND, TVParams
     from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
-    from mrirecon_amd.warmstart import warm_start
+    from mrirecon_amd.warmstart import launch_warm_start, warm_start
     P, F, S, C = 2, 4, 64, 4
     traj, dcf, off = synth.radial_frames([13] * (P * F), 128)
     x_true = np.concatenate([synth.dynamic_phantom(S, F, seed=90 + p, n_ellipses=5) for p in range(P)])
@@ -457,4 +458,18 @@ def test_overlapped_warm_start_equals_sequential_selection():
     r0, racc, rf_w, rf_a = select_warm_start(ObjectiveEval(plan, y, regm, P, F), x_adj, xw, out=torch.empty_like(x_adj))
     assert torch.equal(x0, r0) and acc.tolist() == racc.tolist()
     assert np.allclose(f_w, rf_w, rtol=1e-12) and np.allclose(f_a, rf_a, rtol=1e-12)
+    Ax = torch.empty_like(y)
+    plan.forward(x0, Ax)
+    assert torch.equal(o1.Ax, Ax), "objective.Ax must hold A x0 whichever start each problem took"
+    # second call on the same evaluators, with a network whose output layer is zero:
+    # x_warm == x_adj, so f_warm == f_adj and every problem keeps the warm start
+    params = net.parameters()
+    params["out"][0][:] = 0.0
+    params["out"][1][:] = 0.0
+    net.set_parameters(params)
+    ch = launch_warm_start(net, o1, o2, x_adj, nvol=P, out=torch.empty_like(x_adj))
+    acc2, f_w2, f_a2 = ch.read()
+    assert acc2.all() and np.array_equal(f_w2, f_a2) and torch.equal(ch.x0, x_adj)
+    plan.forward(ch.x0, Ax)
+    assert ch.Ax0 is o1.Ax and torch.equal(o1.Ax, Ax)
 
