@@ -402,6 +402,36 @@ def nufft3_rate(dev, S=128, C=8, spokes=1024, R=256, reps=5):
             "coil_volume_pairs_per_s": round(C / (ms * 1e-3), 1)}
 
 
+def tbasis_rate(dev, T=34, K=8, S=256, vols=1, reps=20):
+    """Temporal-basis map u = U V and its adjoint g_V = U^H g_u at GRASP size (T frames, K basis
+    images of S^2 for each of ``vols`` volumes sharing U): ms per call (HIP events on the launch
+    stream) and GB/s of algorithmic bytes, (K + T) vols S^2 complex64 values per call."""
+    import numpy as np
+    import torch
+
+    from mrirecon_amd.mapping import TemporalBasisOp
+    g = torch.Generator(device="cpu")
+    g.manual_seed(3)
+    op = TemporalBasisOp(torch.randn((T, K), generator=g, dtype=torch.complex64), device=dev)
+    V = torch.randn((K, vols, S, S), generator=g, dtype=torch.complex64).to(dev)
+    u = torch.empty((T, vols, S, S), dtype=torch.complex64, device=dev)
+    gV = torch.empty_like(V)
+    res = {"frames": T, "basis": K, "image": f"{S}^2", "volumes": vols}
+    for name, call in (("forward", lambda: op.forward_apply(V, out=u)),
+                       ("adjoint", lambda: op.adjoint_apply(u, gV, accumulate=False))):
+        call()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            call()
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        res[f"{name}_ms"] = round(ms, 4)
+        res[f"{name}_GBps"] = round((K + T) * vols * S * S * 8 / (ms * 1e-3) / 1e9, 1)
+    return res
+
+
 def batched_uneven(args, dev):
     """Batched uneven workload: P slices x 20 frames at 256^2, 8 coils, 13..55 golden-angle spokes
     per frame x 512, all slices in the same launches (ragged k-space, per-slice stop), temporal TV 0.02.
@@ -597,9 +627,10 @@ def main():
     c4 = None
     if not args.no_c4:
         c4 = batched_uneven(args, dev)
-    variants = None
+    variants, tbasis = None, None
     if not args.no_variants:
         variants = solver_variants(args, dev)
+        tbasis = {"one volume": tbasis_rate(dev), "16 volumes": tbasis_rate(dev, vols=16)}
 
     cpu = None
     if not args.no_cpu:
@@ -620,7 +651,7 @@ def main():
                    "parallelism": "1 GPU"},
         "roofline": roofline, "kernels": per_kernel, "pipeline": pipeline,
         "time_to_tol": ttt, "unet": unet, "nufft_configs": configs, "batched_uneven": c4,
-        "solver_variants": variants, "cpu_baseline": cpu,
+        "solver_variants": variants, "temporal_basis": tbasis, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
 

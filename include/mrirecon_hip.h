@@ -390,6 +390,24 @@ int mrh_unet_adam(float* wm, const float* gw, float* m, float* v, void* wf, void
                   const float* gb, float* mb, float* vb, int ntap, int Cout, int Cout_pad, int Cin,
                   float lr, float beta1, float beta2, float eps, int step, void* stream);
 
+/* ---------------------------------------------------------------- low-rank temporal basis
+ * u = U V with U (T, K) complex64 row-major (frame t's weights on K basis images), V (K, N)
+ * complex64 coefficient images and u (T, N) complex64 frames; N counts every non-frame element.
+ * Replaces TemporalBasisOp (graspcg/regularization/mapping.py:112-231).  Fixed-order FP32 sums
+ * (bitwise repeatable); 1 <= K <= MRH_TBASIS_MAX_K, T >= 1, N >= 0. */
+#define MRH_TBASIS_MAX_K 16
+/* u = U V.  Replaces TemporalBasisOp.forward_apply (mapping.py:129-147). */
+int mrh_tbasis_forward(const void* U, int T, int K, const void* V, int64_t N, void* u, void* stream);
+/* gV = U^H gu, or gV += U^H gu with accumulate != 0 (the reference adds into g_V).
+ * Replaces TemporalBasisOp.adjoint_apply (mapping.py:149-164). */
+int mrh_tbasis_adjoint(const void* U, int T, int K, const void* gu, int64_t N, void* gV, int accumulate,
+                       void* stream);
+/* diag_V[k, n] += sum_t |U[t, k]|^2 ku[t, n] (FP32).  ku is a (T, N) field (profile = 0,
+ * TemporalBasisOp.diag_push, mapping.py:170-199) or a per-frame profile of T values broadcast
+ * over n (profile != 0, TemporalBasisOp.diag_push_profile, mapping.py:200-231). */
+int mrh_tbasis_diag(const void* U, int T, int K, const float* ku, int profile, int64_t N, float* diag_V,
+                    void* stream);
+
 #ifdef __cplusplus
 }
 #endif

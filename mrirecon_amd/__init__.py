@@ -11,6 +11,7 @@ from .nufft.layout import AxisSpec
 from .nufft.api import NUFFT, NUFFTConfig
 from .regularization import TVND, TVParams, RegManager, GraphLaplacian, GraphLapParams
 from .solver import CGSolver, CGConfig, NLCGEngine
+from .mapping import TemporalBasisOp, MappedTV
 
-__all__ = ["NUFFT", "NUFFTConfig", "AxisSpec", "TVND", "TVParams", "RegManager", "GraphLaplacian", "GraphLapParams", "CGSolver",
+__all__ = ["TemporalBasisOp", "MappedTV", "NUFFT", "NUFFTConfig", "AxisSpec", "TVND", "TVParams", "RegManager", "GraphLaplacian", "GraphLapParams", "CGSolver",
            "CGConfig", "NLCGEngine", "Roles", "IMAGE_ROLES"]

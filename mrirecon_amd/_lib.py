@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 9
+ABI_VERSION = 10
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -124,6 +124,9 @@ _SIGS = {
                                   c_float, c_float, c_void_p]),
     "mrh_unet_fill": (c_int, [c_void_p, c_int64, c_float, c_void_p]),
     "mrh_unet_adam": (c_int, [c_void_p] * 10 + [c_int] * 4 + [c_float] * 4 + [c_int, c_void_p]),
+    "mrh_tbasis_forward": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_void_p]),
+    "mrh_tbasis_adjoint": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int64, c_void_p, c_int, c_void_p]),
+    "mrh_tbasis_diag": (c_int, [c_void_p, c_int, c_int, c_void_p, c_int, c_int64, c_void_p, c_void_p]),
 }
 
 _ERR = {1: ValueError, 2: RuntimeError, 3: MemoryError, 4: NotImplementedError}

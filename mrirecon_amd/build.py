@@ -18,7 +18,7 @@ CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 BUILD = PKG / "_build"
 LIB = PKG / "libmrirecon_hip.so"
-SOURCES = ["errors.hip", "nufft.hip", "cg.hip", "unet.hip"]
+SOURCES = ["errors.hip", "nufft.hip", "cg.hip", "unet.hip", "tbasis.hip"]
 
 ARCH = os.environ.get("MRH_OFFLOAD_ARCH", "gfx950")
 CXXFLAGS = [

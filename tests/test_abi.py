@@ -47,6 +47,14 @@ def test_errors_are_reported_without_a_gpu():
     assert rc == 4
     with pytest.raises(NotImplementedError):
         _lib.check(rc)
+    # temporal basis: sizes are checked before any pointer or launch; N = 0 is a no-op
+    rc = lib.mrh_tbasis_forward(None, 4, 17, None, 10, None, None)
+    assert rc == 1 and b"K = 17" in lib.mrh_last_error()
+    rc = lib.mrh_tbasis_adjoint(None, 0, 2, None, 10, None, 1, None)
+    assert rc == 1 and b"T = 0 frames" in lib.mrh_last_error()
+    assert lib.mrh_tbasis_diag(None, 3, 2, None, 1, 0, None, None) == 0
+    rc = lib.mrh_tbasis_diag(None, 3, 2, None, 1, 5, None, None)
+    assert rc == 1 and b"null" in lib.mrh_last_error()
 
 
 C_PROBE = r"""

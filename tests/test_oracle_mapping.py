@@ -1,0 +1,94 @@
+"""Pins of the temporal-basis oracle (oracle/mapping.py) -- CPU only."""
+import numpy as np
+
+from oracle.mapping import (mapped_tv_add_diag, mapped_tv_energy_grad, tb_adjoint, tb_diag_push,
+                            tb_diag_push_profile, tb_forward)
+from oracle.nlcg import TV, tv_energy_grad
+
+
+def _c(rng, *shape):
+    return rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+
+
+def test_reference_diag_push_temporal_basis_values():
+    """graspcg/tests/test_reg_diag_mapped.py:92-126: U = [[1,0,0],[0,2,0]], spatial TV with
+    lambda = 1e-3 -> k = 2 lambda * 2; diag_V = (k, 4k, 0) per coefficient image."""
+    U = np.array([[1.0, 0.0, 0.0], [0.0, 2.0, 0.0]], dtype=np.complex128)
+    diag_V = np.zeros((3, 3, 2))
+    mapped_tv_add_diag(U, TV(weight=1e-3, axes=(1, 2), eps=1e-3), diag_V)
+    k = 2.0 * 1e-3 * 2.0
+    assert np.allclose(diag_V[0], k, atol=1e-12) and np.allclose(diag_V[1], 4 * k, atol=1e-12)
+    assert np.all(diag_V[2] == 0)
+
+
+def test_explicit_loops():
+    rng = np.random.default_rng(0)
+    T, K, S0, S1 = 4, 3, 2, 3
+    U, V, W = _c(rng, T, K), _c(rng, K, S0, S1), _c(rng, T, S0, S1)
+    u = tb_forward(U, V)
+    gV = tb_adjoint(U, W)
+    ku = rng.random((T, S0, S1))
+    D = tb_diag_push(U, ku, np.zeros((K, S0, S1)))
+    v = rng.random(T)
+    Dp = tb_diag_push_profile(U, v, np.zeros((K, S0, S1)))
+    for i in range(S0):
+        for j in range(S1):
+            for t in range(T):
+                assert np.isclose(u[t, i, j], sum(U[t, k] * V[k, i, j] for k in range(K)))
+            for k in range(K):
+                assert np.isclose(gV[k, i, j], sum(np.conj(U[t, k]) * W[t, i, j] for t in range(T)))
+                assert np.isclose(D[k, i, j], sum(abs(U[t, k]) ** 2 * ku[t, i, j] for t in range(T)))
+                assert np.isclose(Dp[k, i, j], sum(abs(U[t, k]) ** 2 * v[t] for t in range(T)))
+
+
+def test_adjoint_identity():
+    """<U V, W> = <V, U^H W>: fails for a dropped conjugate or a transposed U."""
+    rng = np.random.default_rng(1)
+    T, K = 7, 3
+    U, V, W = _c(rng, T, K), _c(rng, K, 5, 4), _c(rng, T, 5, 4)
+    lhs = np.vdot(W, tb_forward(U, V))
+    rhs = np.vdot(tb_adjoint(U, W), V)
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+    bad = (U.T @ W.reshape(T, -1)).reshape(K, 5, 4)           # planted mistake: no conjugate
+    assert abs(np.vdot(bad, V) - lhs) > 1e-3 * abs(lhs)
+
+
+def test_identity_basis_reduces_to_plain_tv():
+    rng = np.random.default_rng(2)
+    T = 5
+    V = _c(rng, T, 6, 6)
+    regs = [TV(weight=0.02, axes=(0,), eps=1e-3), TV(weight=0.005, axes=(1, 2), eps=1e-3)]
+    E, gV = mapped_tv_energy_grad(np.eye(T, dtype=np.complex128), V, regs)
+    g = np.zeros_like(V)
+    E0 = sum(float(tv_energy_grad(V, r, g)) for r in regs)
+    assert np.isclose(E, E0, rtol=1e-14) and np.allclose(gV, g, rtol=1e-14, atol=1e-15)
+
+
+def test_mapped_tv_gradient_by_finite_differences():
+    rng = np.random.default_rng(3)
+    T, K = 6, 2
+    U, V = _c(rng, T, K) / np.sqrt(K), _c(rng, K, 5, 5)
+    regs = [TV(weight=0.02, axes=(0,), eps=1e-2), TV(weight=0.01, axes=(1, 2), eps=1e-2, isotropic=False)]
+    E, gV = mapped_tv_energy_grad(U, V, regs)
+    for s in range(3):
+        d = _c(np.random.default_rng(10 + s), K, 5, 5)
+        h = 1e-6
+        Ep, _ = mapped_tv_energy_grad(U, V + h * d, regs)
+        Em, _ = mapped_tv_energy_grad(U, V - h * d, regs)
+        fd = (Ep - Em) / (2 * h)
+        # E is real, so dE along d is Re<g_V, d> with g_V the conjugate (Wirtinger) gradient
+        assert np.isclose(fd, np.vdot(gV, d).real, rtol=1e-6, atol=1e-9)
+
+
+def test_add_diag_with_image_diagonal_counts_the_frame_profile_too():
+    """wrappers.py:62-105: the profile push is terminal for a temporal basis, so the constant
+    push runs as well; a temporal TV then adds 2 lambda (degree(t) + 1) |U_tk|^2 summed over t."""
+    U = np.array([[1.0, 0.0], [1.0, 1.0], [0.0, 2.0]], dtype=np.complex128)
+    lam = 0.01
+    reg = TV(weight=lam, axes=(0,), eps=1e-3)
+    only_b = mapped_tv_add_diag(U, reg, np.zeros((2, 2, 2)))
+    both = mapped_tv_add_diag(U, reg, np.zeros((2, 2, 2)), image_diag=True)
+    deg = np.array([1.0, 2.0, 1.0])
+    U2 = np.abs(U) ** 2
+    assert np.allclose(only_b[:, 0, 0], 2 * lam * U2.sum(0))
+    assert np.allclose(both[:, 0, 0], 2 * lam * (U2 * (deg + 1.0)[:, None]).sum(0))
