@@ -347,6 +347,7 @@ struct mrh_cg {
   double* d_part_i;   // [P][nb_i][PI_STRIDE]
   double* d_part_k;   // [P][nb_k][PK_STRIDE]
   ProbState* d_st;
+  ProbState* h_st;      // pinned host copy for status reads (one async copy + one stream sync)
   mrh_cg_hist* d_hist;  // [max_hist][P]
   double* d_target;     // [P] objective targets (-inf: none)
   int* d_active;        // [P] 1 while the problem iterates (read by the masked NUFFT entry points)
@@ -1128,6 +1129,8 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
   ALLOC(e->d_target, sizeof(double) * e->P);
   ALLOC(e->d_active, sizeof(int) * e->P);
+  err = hipHostMalloc((void**)&e->h_st, sizeof(ProbState) * e->P, hipHostMallocDefault);
+  if (err != hipSuccess) { e->h_st = nullptr; mrh_cg_destroy(e); mrh_set_error("cg_create: hipHostMalloc failed: %s", hipGetErrorString(err)); return MRH_ENOMEM; }
   if (e->gl.lam != 0.f) {
     const int plane = e->S0 * e->S1;
     e->nb_q = std::max(1, std::min(2048, (plane + GL_TILE - 1) / GL_TILE));
@@ -1162,6 +1165,7 @@ extern "C" int mrh_cg_destroy(mrh_cg* e) {
   void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist, e->d_target, e->d_active, e->d_gl,
                   e->d_part_q};
   for (void* b : bufs) if (b) (void)hipFree(b);
+  if (e->h_st) (void)hipHostFree(e->h_st);
   delete e;
   return MRH_OK;
 }
@@ -1301,12 +1305,12 @@ extern "C" int mrh_cg_set_targets(mrh_cg* e, const double* targets, void* stream
 
 extern "C" int mrh_cg_read_status(mrh_cg* e, int* status, int* iters, void* stream) {
   MRH_REQUIRE(e, MRH_EINVAL, "cg_read_status: null engine");
+  // stream order puts the copy after the caller's work; one sync then covers both
+  MRH_HIP(hipMemcpyAsync(e->h_st, e->d_st, sizeof(ProbState) * e->P, hipMemcpyDeviceToHost, (hipStream_t)stream));
   MRH_HIP(hipStreamSynchronize((hipStream_t)stream));
-  std::vector<ProbState> h(e->P);
-  MRH_HIP(hipMemcpy(h.data(), e->d_st, sizeof(ProbState) * e->P, hipMemcpyDeviceToHost));
   for (int p = 0; p < e->P; ++p) {
-    if (status) status[p] = h[p].status;
-    if (iters) iters[p] = h[p].iter;
+    if (status) status[p] = e->h_st[p].status;
+    if (iters) iters[p] = e->h_st[p].iter;
   }
   return MRH_OK;
 }
