@@ -549,9 +549,8 @@ class WarmStartChoice:
         self.x0, self.Ax0, self._f_w, self._f_a = x0, Ax0, f_w, f_a
 
     def read(self) -> Tuple[np.ndarray, np.ndarray, np.ndarray]:
-        f_w = self._f_w.cpu().numpy()
-        f_a = self._f_a.cpu().numpy()
-        return f_w <= f_a, f_w, f_a
+        f = torch.stack((self._f_w, self._f_a)).cpu().numpy()       # one copy, one sync
+        return f[0] <= f[1], f[0], f[1]
 
 
 def launch_warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveEval, x_adj: torch.Tensor,
@@ -579,10 +578,11 @@ def launch_warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: Obj
     f_w, f_a = objective.value_device(), objective_adj.value_device()
     acc = f_w <= f_a                                       # device bool per problem
     x0 = x_warm if out is None else out
-    per = x_adj.shape[0] // objective.P
-    for p in range(objective.P):
-        xs = slice(p * per, (p + 1) * per)
-        torch.where(acc[p], x_warm[xs], x_adj[xs], out=x0[xs])
+    P = objective.P
+    sel = acc.view(P, *([1] * x_adj.dim()))                 # every problem holds the same frame count
+    torch.where(sel, x_warm.view(P, -1, *x_adj.shape[1:]), x_adj.view(P, -1, *x_adj.shape[1:]),
+                out=x0.view(P, -1, *x_adj.shape[1:]))
+    for p in range(P):                                     # k-space segments are ragged
         a, b = objective.k_prob[p], objective.k_prob[p + 1]
         torch.where(acc[p], objective.Ax[a:b], objective_adj.Ax[a:b], out=objective.Ax[a:b])
     return WarmStartChoice(x0, objective.Ax, f_w, f_a)
