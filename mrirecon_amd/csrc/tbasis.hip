@@ -99,7 +99,9 @@ int tb_dispatch(int K, A... a) {
     TB_CASE(1) TB_CASE(2) TB_CASE(3) TB_CASE(4) TB_CASE(5) TB_CASE(6) TB_CASE(7) TB_CASE(8)
     TB_CASE(9) TB_CASE(10) TB_CASE(11) TB_CASE(12) TB_CASE(13) TB_CASE(14) TB_CASE(15) TB_CASE(16)
 #undef TB_CASE
-    default: return MRH_EINVAL;
+    default:
+      mrh_set_error("tbasis: K = %d basis images, need 1..%d", K, MRH_TBASIS_MAX_K);
+      return MRH_EINVAL;
   }
 }
 
