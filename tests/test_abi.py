@@ -125,3 +125,15 @@ def test_graph_laplacian_host_side():
         RegManager([gl, GraphLaplacian("h", W, GraphLapParams(weight=0.1))])
     with pytest.raises(ValueError):
         GraphLaplacian("bad", W, GraphLapParams(weight=0.1, normalize="lap"))
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    """No CPU fallback: with the library absent, the first use raises ImportError."""
+    import sys
+    env = dict(os.environ, MRH_LIB=str(tmp_path / "absent.so"))
+    code = ("from mrirecon_amd import _lib\n"
+            "try:\n    _lib.load()\nexcept ImportError as e:\n    print('ImportError:', e)\n")
+    out = subprocess.run([sys.executable, "-c", code], cwd=ROOT, env=env, capture_output=True, text=True,
+                         timeout=300)
+    assert out.returncode == 0, out.stderr
+    assert "ImportError:" in out.stdout and "no CPU fallback" in out.stdout
