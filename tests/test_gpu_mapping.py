@@ -121,3 +121,20 @@ def test_limits():
         op.forward_apply(torch.zeros((3, 4), dtype=torch.complex64, device=DEV))      # K = 2 rows expected
     empty = op.forward_apply(torch.zeros((2, 0), dtype=torch.complex64, device=DEV))
     assert empty.shape == (3, 0)
+
+
+def test_identity_basis_equals_plain_tv_bitwise():
+    """U = I: u = V exactly (the other terms are exact zeros), so TV through the map must give the
+    plain TV's energy and gradient bit for bit."""
+    from mrirecon_amd import RegManager, TVND, TVParams
+    from mrirecon_amd.mapping import MappedTV, TemporalBasisOp
+    rng = np.random.default_rng(11)
+    T, S = 12, 48
+    V = torch.from_numpy(_c(rng, T, S, S)).to(DEV)
+    terms = [TVND("t", TVParams(weight=0.02, eps=1e-3, axes="temporal")),
+             TVND("s", TVParams(weight=0.004, eps=1e-3, axes="spatial"))]
+    gV = torch.zeros_like(V)
+    E = MappedTV(terms, TemporalBasisOp(torch.eye(T))).energy_grad(V, gV)
+    g = torch.zeros_like(V)
+    E0 = RegManager(terms).energy_and_grad_tensors(V, g, problems=1)[0]
+    assert float(E) == float(E0) and torch.equal(gV, g)
