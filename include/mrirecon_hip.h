@@ -321,6 +321,11 @@ int mrh_cg_active_flags(mrh_cg* eng, const int** flags);
 int mrh_cg_read_status(mrh_cg* eng, int* status, int* iters, void* stream);
 /* Copy history (host mrh_cg_hist[n_iter * P], iteration-major) to host. */
 int mrh_cg_read_history(mrh_cg* eng, mrh_cg_hist* hist, int n_iter, void* stream);
+/* Status (int[P]), iteration counts (int[P]) and each problem's latest complete history
+ * record (host mrh_cg_hist[P]; zeros before the first iteration) in one copy and one stream
+ * synchronisation: what the reference's per-iteration convergence test reads
+ * (graspcg/solvers/cg.py:228-244 with graspcg/solvers/utils.py:40-43).  Any pointer may be NULL. */
+int mrh_cg_read_latest(mrh_cg* eng, int* status, int* iters, mrh_cg_hist* last, void* stream);
 
 /* ------------------------------------------------------------------ U-Net warm start
  *

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 10
+ABI_VERSION = 11
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -109,6 +109,7 @@ _SIGS = {
     "mrh_cg_build_diag": (c_int, [c_void_p, c_void_p, c_void_p]),
     "mrh_cg_read_status": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), c_void_p]),
     "mrh_cg_read_history": (c_int, [c_void_p, POINTER(CGHist), c_int, c_void_p]),
+    "mrh_cg_read_latest": (c_int, [c_void_p, POINTER(c_int), POINTER(c_int), POINTER(CGHist), c_void_p]),
     "mrh_unet_conv_fwd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 9 + [c_void_p]),
     "mrh_unet_conv_bwd_weight": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p] + [c_int] * 8 + [c_void_p]),
     "mrh_unet_relu_bwd": (c_int, [c_void_p, c_void_p, c_int64, c_void_p]),

@@ -349,6 +349,8 @@ struct mrh_cg {
   ProbState* d_st;
   ProbState* h_st;      // pinned host copy for status reads (one async copy + one stream sync)
   mrh_cg_hist* d_hist;  // [max_hist][P]
+  mrh_cg_hist* d_last;  // [P] each problem's latest complete record (mrh_cg_read_latest)
+  mrh_cg_hist* h_last;  // pinned host copy of d_last
   double* d_target;     // [P] objective targets (-inf: none)
   int* d_active;        // [P] 1 while the problem iterates (read by the masked NUFFT entry points)
   float* d_diag;        // caller's diagonal preconditioner (P F S0 S1 floats) or null
@@ -1128,9 +1130,12 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   ALLOC(e->d_st, sizeof(ProbState) * e->P);
   ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
   ALLOC(e->d_target, sizeof(double) * e->P);
+  ALLOC(e->d_last, sizeof(mrh_cg_hist) * e->P);
   ALLOC(e->d_active, sizeof(int) * e->P);
   err = hipHostMalloc((void**)&e->h_st, sizeof(ProbState) * e->P, hipHostMallocDefault);
   if (err != hipSuccess) { e->h_st = nullptr; mrh_cg_destroy(e); mrh_set_error("cg_create: hipHostMalloc failed: %s", hipGetErrorString(err)); return MRH_ENOMEM; }
+  err = hipHostMalloc((void**)&e->h_last, sizeof(mrh_cg_hist) * e->P, hipHostMallocDefault);
+  if (err != hipSuccess) { e->h_last = nullptr; mrh_cg_destroy(e); mrh_set_error("cg_create: hipHostMalloc failed: %s", hipGetErrorString(err)); return MRH_ENOMEM; }
   if (e->gl.lam != 0.f) {
     const int plane = e->S0 * e->S1;
     e->nb_q = std::max(1, std::min(2048, (plane + GL_TILE - 1) / GL_TILE));
@@ -1163,9 +1168,10 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
 extern "C" int mrh_cg_destroy(mrh_cg* e) {
   if (!e) return MRH_OK;
   void* bufs[] = {e->d_kprob, e->d_part_i, e->d_part_k, e->d_st, e->d_hist, e->d_target, e->d_active, e->d_gl,
-                  e->d_part_q};
+                  e->d_part_q, e->d_last};
   for (void* b : bufs) if (b) (void)hipFree(b);
   if (e->h_st) (void)hipHostFree(e->h_st);
+  if (e->h_last) (void)hipHostFree(e->h_last);
   delete e;
   return MRH_OK;
 }
@@ -1321,5 +1327,38 @@ extern "C" int mrh_cg_read_history(mrh_cg* e, mrh_cg_hist* hist, int n_iter, voi
   MRH_HIP(hipStreamSynchronize((hipStream_t)stream));
   if (n_iter == 0) return MRH_OK;
   MRH_HIP(hipMemcpy(hist, e->d_hist, sizeof(mrh_cg_hist) * (size_t)n_iter * e->P, hipMemcpyDeviceToHost));
+  return MRH_OK;
+}
+
+// last[p] = problem p's latest complete history record (zeros before the first one).
+__global__ void k_gather_latest(const ProbState* __restrict__ st, const mrh_cg_hist* __restrict__ hist,
+                                int max_hist, int P, mrh_cg_hist* __restrict__ last) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  const int n = min(st[p].iter, max_hist);
+  mrh_cg_hist h;
+  if (n > 0) {
+    h = hist[(int64_t)(n - 1) * P + p];
+  } else {
+    double* z = (double*)&h;
+    for (size_t i = 0; i < sizeof(mrh_cg_hist) / sizeof(double); ++i) z[i] = 0.0;
+  }
+  last[p] = h;
+}
+
+extern "C" int mrh_cg_read_latest(mrh_cg* e, int* status, int* iters, mrh_cg_hist* last, void* stream) {
+  MRH_REQUIRE(e, MRH_EINVAL, "cg_read_latest: null engine");
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_gather_latest, dim3((e->P + 63) / 64), dim3(64), 0, st, e->d_st, e->d_hist, e->max_hist, e->P,
+                     e->d_last);
+  MRH_CHECK_LAUNCH();
+  MRH_HIP(hipMemcpyAsync(e->h_st, e->d_st, sizeof(ProbState) * e->P, hipMemcpyDeviceToHost, st));
+  MRH_HIP(hipMemcpyAsync(e->h_last, e->d_last, sizeof(mrh_cg_hist) * e->P, hipMemcpyDeviceToHost, st));
+  MRH_HIP(hipStreamSynchronize(st));
+  for (int p = 0; p < e->P; ++p) {
+    if (status) status[p] = e->h_st[p].status;
+    if (iters) iters[p] = e->h_st[p].iter;
+    if (last) last[p] = e->h_last[p];
+  }
   return MRH_OK;
 }

@@ -129,6 +129,8 @@ class MappedTV:
     @torch.no_grad()
     def energy_grad(self, V: torch.Tensor, g_V: torch.Tensor) -> torch.Tensor:
         """Energy of the terms at u = U V (FP64 device scalar); g_V += U^H grad_u."""
+        if not (V.dim() == 3 or (V.dim() == 4 and int(V.shape[1]) == 1)):
+            raise ValueError(f"V must be (K, S0, S1) or (K, 1, S0, S1) for TV on its frames, got {tuple(V.shape)}")
         u = self.op.forward_apply(V)
         g_u = torch.zeros_like(u)
         E = self.regm.energy_and_grad_tensors(u, g_u, problems=1)
@@ -147,6 +149,8 @@ class MappedTV:
             axes = tuple(self.roles.resolve_axes(p.axes))
             if p.weight == 0.0 or not axes:
                 continue
+            if p.axis_weights is not None and len(p.axis_weights) != len(axes):
+                continue    # the reference pushes no profile and k = 0 then (tv_nd.py:176-179,197-199)
             ws = list(p.axis_weights) if p.axis_weights is not None else [1.0] * len(axes)
             if image_diag and 0 in axes:
                 w0 = ws[axes.index(0)]

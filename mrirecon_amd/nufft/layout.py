@@ -1,8 +1,14 @@
-"""Axis grammar for user layouts (same contract as graspcg/nufft/layout.py:11-194).
+"""Axis labels of user tensors and the layout plans derived from them.
 
-Image tensors are (B, Like..., [coil=1], FFT...) in any label order; k-space
-tensors are (B, Like..., coil, K).  Like axes are every label that is neither
-'B', the coil label nor an FFT label.
+Interface of graspcg/nufft/layout.py:11-194.  An image tensor carries the
+labels of ``AxisSpec.image`` (batch 'B', any number of Like labels, an optional
+size-1 coil axis and the FFT axes, in any order); a k-space tensor carries
+``AxisSpec.kspace`` (batch, Like labels, the coil axis and the sample axis).
+Every label that is none of batch, coil or an FFT label is a Like label, and
+the Like axes are folded into the batch by the operator.  Field names, both
+keyword styles of ``AxisSpec`` (``image_labels=`` or ``image=``), the default
+labels ('coil', ('K',)) and the exception types are the reference's; the code
+is this package's.
 """
 from __future__ import annotations
 
@@ -12,42 +18,65 @@ from typing import Dict, Optional, Sequence, Tuple
 
 import torch
 
+_BATCH = "B"
+
 
 class AxisSpec:
-    """User axis labels; accepts the new (``image_labels=...``) or legacy (``image=...``) names
-    (graspcg/nufft/layout.py:28-60)."""
+    """Labels of image and k-space tensors.  Each field may be given under its long keyword
+    (``image_labels``, ``kspace_labels``, ``image_fft_labels``, ``kspace_fft_labels``,
+    ``coil_label``) or its short one (``image``, ``kspace``, ``image_fft``, ``kspace_fft``,
+    ``coil``); the long one wins when both are given."""
 
-    def __init__(self, *, image_labels=None, kspace_labels=None, image_fft_labels=None,
-                 kspace_fft_labels=None, coil_label=None, image=None, kspace=None, image_fft=None,
-                 kspace_fft=None, coil=None):
-        image_labels = image_labels if image_labels is not None else image
-        kspace_labels = kspace_labels if kspace_labels is not None else kspace
-        image_fft_labels = image_fft_labels if image_fft_labels is not None else image_fft
-        kspace_fft_labels = kspace_fft_labels if kspace_fft_labels is not None else kspace_fft
-        coil_label = coil_label if coil_label is not None else (coil if coil is not None else "coil")
-        if image_labels is None or kspace_labels is None or image_fft_labels is None:
-            raise TypeError("AxisSpec requires image(_labels)=..., kspace(_labels)=..., image_fft(_labels)=...")
-        self.image_labels = tuple(image_labels)
-        self.kspace_labels = tuple(kspace_labels)
-        self.image_fft_labels = tuple(image_fft_labels)
-        self.kspace_fft_labels = tuple(kspace_fft_labels) if kspace_fft_labels is not None else ("K",)
-        self.coil_label = str(coil_label)
+    _FIELDS = (("image_labels", "image"), ("kspace_labels", "kspace"),
+               ("image_fft_labels", "image_fft"), ("kspace_fft_labels", "kspace_fft"),
+               ("coil_label", "coil"))
+    _DEFAULTS = {"kspace_fft_labels": ("K",), "coil_label": "coil"}
 
+    def __init__(self, **kw):
+        known = {n for pair in self._FIELDS for n in pair}
+        extra = sorted(set(kw) - known)
+        if extra:
+            raise TypeError(f"AxisSpec got unexpected keyword(s) {extra}")
+        given: Dict[str, object] = {}
+        for long_name, short_name in self._FIELDS:
+            v = kw.get(long_name)
+            if v is None:
+                v = kw.get(short_name)
+            if v is None:
+                v = self._DEFAULTS.get(long_name)
+            given[long_name] = v
+        missing = [f"{ln} (or {sn})" for ln, sn in self._FIELDS if given[ln] is None]
+        if missing:
+            raise TypeError("AxisSpec needs labels for " + ", ".join(missing))
+        self.image_labels: Tuple[str, ...] = tuple(given["image_labels"])
+        self.kspace_labels: Tuple[str, ...] = tuple(given["kspace_labels"])
+        self.image_fft_labels: Tuple[str, ...] = tuple(given["image_fft_labels"])
+        self.kspace_fft_labels: Tuple[str, ...] = tuple(given["kspace_fft_labels"])
+        self.coil_label: str = str(given["coil_label"])
+
+    # short names (read-only)
     image = property(lambda self: self.image_labels)
     kspace = property(lambda self: self.kspace_labels)
     image_fft = property(lambda self: self.image_fft_labels)
     kspace_fft = property(lambda self: self.kspace_fft_labels)
     coil = property(lambda self: self.coil_label)
 
+    def _like(self, labels: Sequence[str], fft: Sequence[str]) -> Tuple[str, ...]:
+        structural = set(fft) | {_BATCH, self.coil_label}
+        return tuple(lbl for lbl in labels if lbl not in structural)
+
     def like_image(self) -> Tuple[str, ...]:
-        return tuple(l for l in self.image if l not in self.image_fft and l not in ("B", self.coil))
+        """Like labels of image tensors, in label order."""
+        return self._like(self.image_labels, self.image_fft_labels)
 
     def like_kspace(self) -> Tuple[str, ...]:
-        return tuple(l for l in self.kspace if l not in self.kspace_fft and l not in ("B", self.coil))
+        """Like labels of k-space tensors, in label order."""
+        return self._like(self.kspace_labels, self.kspace_fft_labels)
 
     def __repr__(self) -> str:
-        return (f"AxisSpec(image={self.image}, kspace={self.kspace}, image_fft={self.image_fft}, "
-                f"coil={self.coil!r})")
+        return (f"AxisSpec(image={self.image_labels}, kspace={self.kspace_labels}, "
+                f"image_fft={self.image_fft_labels}, kspace_fft={self.kspace_fft_labels}, "
+                f"coil={self.coil_label!r})")
 
 
 @dataclass(frozen=True)
@@ -57,7 +86,7 @@ class ImageLayoutPlan:
     like_labels: Tuple[str, ...]
     like_sizes: Tuple[int, ...]
     L_other: int
-    spatial: Tuple[int, ...]
+    spatial: Tuple[int, ...]          # sizes of the FFT axes, in image_fft order
 
 
 @dataclass(frozen=True)
@@ -68,55 +97,64 @@ class KspaceLayoutPlan:
     like_sizes: Tuple[int, ...]
     L_other: int
     C: int
-    K: int
+    K: int                            # size of the first k-space FFT label
 
 
-def _sizes(t: torch.Tensor, labels: Sequence[str]) -> Dict[str, int]:
-    if t.ndim != len(labels):
-        raise ValueError(f"tensor has {t.ndim} dims but the axis spec names {len(labels)}: {labels}")
-    return {lbl: int(t.shape[i]) for i, lbl in enumerate(labels)}
+def _label_sizes(t: torch.Tensor, labels: Sequence[str], what: str) -> Dict[str, int]:
+    if t.dim() != len(labels):
+        raise ValueError(f"{what} tensor of shape {tuple(t.shape)} does not match its {len(labels)} labels {tuple(labels)}")
+    return dict(zip(labels, (int(n) for n in t.shape)))
 
 
-def plan_image_layout(x: torch.Tensor, axis: AxisSpec) -> ImageLayoutPlan:
-    s = _sizes(x, axis.image)
-    if axis.coil in s and s[axis.coil] != 1:
-        raise ValueError("Image is expected coil-combined; axis.coil dim must be size 1 if present.")
+def _fold_like(like_labels: Tuple[str, ...], L_sizes: Sequence[int]) -> Tuple[Tuple[int, ...], int]:
+    """Sizes of the Like labels from a size list: one size per label, or a single total that the
+    first label takes (the others 1), or nothing when there are no Like labels."""
+    n = len(like_labels)
+    if n == 0:
+        return (), 1
+    sizes = [int(v) for v in L_sizes]
+    if len(sizes) == 1 and n > 1:
+        sizes = sizes + [1] * (n - 1)
+    if len(sizes) != n:
+        raise ValueError(f"got {len(L_sizes)} Like sizes for the Like labels {like_labels}; "
+                         f"give one per label or a single total")
+    return tuple(sizes), math.prod(sizes)
+
+
+def plan_image_layout(x_user: torch.Tensor, axis: AxisSpec) -> ImageLayoutPlan:
+    sz = _label_sizes(x_user, axis.image_labels, "image")
+    ncoil = sz.get(axis.coil_label, 1)
+    if ncoil != 1:
+        raise ValueError(f"images are coil-combined: the '{axis.coil_label}' axis must have size 1, not {ncoil}")
     like = axis.like_image()
-    ls = tuple(s[l] for l in like)
-    return ImageLayoutPlan(axis, s.get("B", 1), like, ls, int(math.prod(ls)) if ls else 1,
-                           tuple(s[l] for l in axis.image_fft))
+    ls = tuple(sz[lbl] for lbl in like)
+    return ImageLayoutPlan(axis=axis, B=sz.get(_BATCH, 1), like_labels=like, like_sizes=ls,
+                           L_other=math.prod(ls), spatial=tuple(sz[lbl] for lbl in axis.image_fft_labels))
 
 
-def plan_kspace_layout(y: torch.Tensor, axis: AxisSpec) -> KspaceLayoutPlan:
-    s = _sizes(y, axis.kspace)
+def plan_kspace_layout(y_user: torch.Tensor, axis: AxisSpec) -> KspaceLayoutPlan:
+    sz = _label_sizes(y_user, axis.kspace_labels, "k-space")
     like = axis.like_kspace()
-    ls = tuple(s[l] for l in like)
-    return KspaceLayoutPlan(axis, s.get("B", 1), like, ls, int(math.prod(ls)) if ls else 1,
-                            s[axis.coil], s[axis.kspace_fft[0]])
-
-
-def _like_sizes(labels: Tuple[str, ...], L_sizes: Sequence[int]) -> Tuple[Tuple[int, ...], int]:
-    if len(labels) == 0:
-        return tuple(), 1
-    if len(L_sizes) == len(labels):
-        ls = tuple(int(v) for v in L_sizes)
-        return ls, int(math.prod(ls))
-    if len(L_sizes) == 1:   # product assigned to the first like label (layout.py:164-168)
-        ls = [1] * len(labels)
-        ls[0] = int(L_sizes[0])
-        return tuple(ls), int(L_sizes[0])
-    raise ValueError("L_sizes length must be 0, 1, or match number of like labels.")
+    ls = tuple(sz[lbl] for lbl in like)
+    return KspaceLayoutPlan(axis=axis, B=sz.get(_BATCH, 1), like_labels=like, like_sizes=ls,
+                            L_other=math.prod(ls), C=sz[axis.coil_label], K=sz[axis.kspace_fft_labels[0]])
 
 
 def plan_image_layout_from_sizes(B: int, L_sizes: Sequence[int], spatial: Sequence[int],
                                  axis: AxisSpec) -> ImageLayoutPlan:
     like = axis.like_image()
-    ls, L = _like_sizes(like, L_sizes)
-    return ImageLayoutPlan(axis, int(B), like, ls, L, tuple(int(v) for v in spatial))
+    ls, L = _fold_like(like, L_sizes)
+    return ImageLayoutPlan(axis=axis, B=int(B), like_labels=like, like_sizes=ls, L_other=int(L),
+                           spatial=tuple(int(s) for s in spatial))
 
 
 def plan_kspace_layout_from_sizes(B: int, L_sizes: Sequence[int], C: int, K: int,
                                   axis: AxisSpec) -> KspaceLayoutPlan:
     like = axis.like_kspace()
-    ls, L = _like_sizes(like, L_sizes)
-    return KspaceLayoutPlan(axis, int(B), like, ls, L, int(C), int(K))
+    ls, L = _fold_like(like, L_sizes)
+    return KspaceLayoutPlan(axis=axis, B=int(B), like_labels=like, like_sizes=ls, L_other=int(L),
+                            C=int(C), K=int(K))
+
+
+__all__ = ["AxisSpec", "ImageLayoutPlan", "KspaceLayoutPlan", "plan_image_layout", "plan_kspace_layout",
+           "plan_image_layout_from_sizes", "plan_kspace_layout_from_sizes"]

@@ -166,8 +166,12 @@ class RegManager:
             raise ValueError("x must be a contiguous complex64 tensor")
         if g is not None and (g.dtype != torch.complex64 or not g.is_contiguous() or g.shape != x.shape):
             raise ValueError("g must be a contiguous complex64 tensor shaped like x")
+        if not (x.dim() == 3 or (x.dim() == 4 and int(x.shape[1]) == 1)):
+            raise ValueError(f"x must be (P*F, S0, S1) or (P*F, 1, S0, S1), got shape {tuple(x.shape)}")
         S0, S1 = int(x.shape[-2]), int(x.shape[-1])
         B = int(x.shape[0])
+        if B * S0 * S1 != x.numel():
+            raise ValueError(f"x of shape {tuple(x.shape)} is not a stack of S0 x S1 frames")
         if B % problems:
             raise ValueError("frames not divisible by the number of problems")
         desc = self.tv_desc(problems, B // problems, S0, S1)

@@ -207,6 +207,17 @@ class NLCGEngine:
         _lib.check(self._lib.mrh_cg_read_status(self.handle, st, it, self._st()))
         return np.array(st[:], dtype=np.int32), np.array(it[:], dtype=np.int32)
 
+    def latest(self):
+        """Status, iteration counts and each problem's latest history record (gnorm, xnorm,
+        stepnorm) from one copy and one stream synchronisation."""
+        st = (ctypes.c_int * self.P)()
+        it = (ctypes.c_int * self.P)()
+        last = (_lib.CGHist * self.P)()
+        _lib.check(self._lib.mrh_cg_read_latest(self.handle, st, it, last, self._st()))
+        rows = [dict(gnorm=math.sqrt(max(h.rho, 0.0)), xnorm=math.sqrt(max(h.xnorm2, 0.0)),
+                     stepnorm=math.sqrt(max(h.dnorm2, 0.0)), f=h.f0, f_step=h.ft, step=h.t) for h in last]
+        return np.array(st[:], dtype=np.int32), np.array(it[:], dtype=np.int32), rows
+
     def active(self) -> np.ndarray:
         return self.status()[0] == _lib.CG_ACTIVE
 
@@ -344,13 +355,13 @@ class CGSolver:
         for k in range(max_iter):
             eng.run(1, use_graph=self.cfg.use_graph, check_every=0)
             self.iter = k
-            status, iters = eng.status()
+            # one read per iteration: status plus each problem's latest record
+            status, iters, last = eng.latest()
             if not (status == _lib.CG_ACTIVE).any():
                 break
             # reference convergence test (graspcg/solvers/utils.py:40-43)
-            h = eng.history()
-            if all(rows and rows[-1]["gnorm"] <= (1 + rows[-1]["xnorm"]) * tol_g
-                   and rows[-1]["stepnorm"] <= (1 + rows[-1]["xnorm"]) * tol_step for rows in h):
+            if all(n > 0 and r["gnorm"] <= (1 + r["xnorm"]) * tol_g and r["stepnorm"] <= (1 + r["xnorm"]) * tol_step
+                   for n, r in zip(iters, last)):
                 break
         return self
 

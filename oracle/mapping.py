@@ -85,6 +85,8 @@ def mapped_tv_add_diag(U: np.ndarray, reg: TV, diag_V: np.ndarray, image_diag: b
     """Transformed.add_diag for one TV term through a temporal basis (wrappers.py:39-105)."""
     if reg.weight == 0.0 or len(reg.axes) == 0:
         return diag_V
+    if reg.axis_weights is not None and len(reg.axis_weights) != len(reg.axes):
+        return diag_V           # no profile and k = 0 for mismatched weights (tv_nd.py:176-179,197-199)
     ws = list(reg.axis_weights) if reg.axis_weights is not None else [1.0] * len(reg.axes)
     T = U.shape[0]
     if image_diag and 0 in reg.axes:                                   # (A), then fall through

@@ -92,3 +92,39 @@ def test_add_diag_with_image_diagonal_counts_the_frame_profile_too():
     U2 = np.abs(U) ** 2
     assert np.allclose(only_b[:, 0, 0], 2 * lam * U2.sum(0))
     assert np.allclose(both[:, 0, 0], 2 * lam * (U2 * (deg + 1.0)[:, None]).sum(0))
+
+
+def test_add_diag_skips_a_term_whose_axis_weights_do_not_match_its_axes():
+    """tv_nd.py:176-179,197-199: mismatched axis_weights give k = 0 and no frame profile."""
+    U = np.array([[1.0, 0.0], [1.0, 1.0], [0.0, 2.0]], dtype=np.complex128)
+    reg = TV(weight=0.01, axes=(0, 1, 2), eps=1e-3, axis_weights=(1.0, 2.0))
+    for image_diag in (False, True):
+        assert np.all(mapped_tv_add_diag(U, reg, np.zeros((2, 2, 2)), image_diag=image_diag) == 0)
+    ok = TV(weight=0.01, axes=(0, 1, 2), eps=1e-3, axis_weights=(1.0, 2.0, 1.0))
+    assert np.allclose(mapped_tv_add_diag(U, ok, np.zeros((2, 2, 2)))[:, 0, 0],
+                       2 * 0.01 * 6.0 * (np.abs(U) ** 2).sum(0))
+
+
+def test_tv_entry_points_refuse_frame_stacks_of_the_wrong_shape():
+    """RegManager / MappedTV reject tensors that are not (frames, [1,] S0, S1) before any launch
+    (a (T, N) or (K, C, S0, S1) stack would be read with the wrong frame size)."""
+    import pytest
+    import torch
+
+    from mrirecon_amd.mapping import MappedTV
+    from mrirecon_amd.regularization import RegManager, TVND, TVParams
+
+    terms = [TVND("t", TVParams(weight=0.02, axes="temporal"))]
+    regm = RegManager(terms)
+    for shape in ((6, 40), (6, 2, 5, 4), (6,), (6, 1, 5, 4, 1)):
+        with pytest.raises(ValueError):
+            regm.energy_and_grad_tensors(torch.zeros(shape, dtype=torch.complex64))
+
+    class _NoLaunch:
+        def forward_apply(self, V):
+            raise AssertionError("reached the kernels")
+
+    mt = MappedTV(terms, _NoLaunch())
+    for shape in ((3, 40), (3, 2, 5, 4)):
+        with pytest.raises(ValueError):
+            mt.energy_grad(torch.zeros(shape, dtype=torch.complex64), torch.zeros(shape, dtype=torch.complex64))
