@@ -93,7 +93,8 @@ class KBNufft:
 
     def __init__(self, maps: np.ndarray, traj: np.ndarray, k_offsets: np.ndarray,
                  dcf: Optional[np.ndarray], W: int, osf: float = 2.0,
-                 dcf_mode: str = "balanced", like: int = 1, dtype=np.float64):
+                 dcf_mode: str = "balanced", like: int = 1, dtype=np.float64,
+                 dcf_in_fwd: bool = False, dcf_in_adj: bool = True):
         self.rdt = np.dtype(dtype)
         self.cdt = np.complex128 if self.rdt == np.float64 else np.complex64
         self.maps = np.asarray(maps).astype(self.cdt)
@@ -111,8 +112,10 @@ class KBNufft:
         elif dcf_mode == "balanced":
             w = np.sqrt(np.clip(np.asarray(dcf, dtype=np.float64), 0.0, None))
             self.wA = self.wAH = w
-        elif dcf_mode == "standard":
-            self.wA, self.wAH = None, np.asarray(dcf, dtype=np.float64)
+        elif dcf_mode == "standard":     # dcf on the sides the flags name (torchkb.py:192-197)
+            d = np.asarray(dcf, dtype=np.float64)
+            self.wA = d if dcf_in_fwd else None
+            self.wAH = d if dcf_in_adj else None
         else:
             raise ValueError(dcf_mode)
         dx = deapod_1d(self.S0, self.G0, self.W, self.beta)

@@ -313,3 +313,27 @@ def test_masked_forward_adjoint_skip_inactive_problems():
     assert torch.equal(xa[F:], xa_full[F:]) and bool((xa[:F] == 5.0).all())
     with pytest.raises(ValueError):
         plan.forward_masked(x, y, active.data_ptr(), 4)     # 4 does not divide 6 frames
+
+
+@pytest.mark.parametrize("mode,fwd,adj", [("none", False, True), ("standard", False, True),
+                                          ("standard", True, True), ("standard", True, False)])
+def test_dcf_modes_match_oracle(mode, fwd, adj):
+    """The 2-D DCF policies of torchkb.py:185-199 on c1 ('none'; 'standard' with apply_dcf_in_fwd
+    both ways and apply_dcf_in_adj off) against the FP64 oracle of the same policy."""
+    wl = synth.config_c1()
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", dcf_mode=mode, apply_dcf_in_fwd=fwd, apply_dcf_in_adj=adj,
+                         kb_width=wl.W, osf=wl.osf)
+    be.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(wl.traj), torch.from_numpy(wl.dcf), c64(wl.maps),
+               device=DEV, k_offsets=wl.k_offsets)
+    ora = KBNufft(wl.maps, wl.traj, wl.k_offsets, wl.dcf, wl.W, wl.osf, dcf_mode=mode, dcf_in_fwd=fwd,
+                  dcf_in_adj=adj)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(wl.x_true), y)
+    yr = crand(be.plan.n_k, 31)
+    xa = torch.empty((1, wl.S, wl.S), dtype=torch.complex64, device=DEV)
+    be.plan.adjoint(c64(yr), xa)
+    torch.cuda.synchronize()
+    e_f = rel_l2(y.cpu().numpy(), ora.forward(wl.x_true))
+    e_a = rel_l2(xa.cpu().numpy(), ora.adjoint(yr))
+    print(f"[c1 dcf {mode} fwd={fwd} adj={adj}] forward rel={e_f:.3e} adjoint rel={e_a:.3e}")
+    assert e_f <= TOL and e_a <= TOL

@@ -539,3 +539,32 @@ def test_solver_option_limits_are_reported():
         NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, ls_name="backtrack")
     with pytest.raises(ValueError):
         NLCGEngine(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, ls_name="wolfe", c2=-0.5)
+
+
+def test_latest_record_read_equals_the_history_tail():
+    """mrh_cg_read_latest: status, counts and each problem's last record in one read, equal to
+    the last row of the full history (two problems that stop at different iterations)."""
+    wl = small_dynamic(seed=33, F=4)
+    be, ora = prepare(wl)
+    regm, _ = regs_for(wl)
+    y = np.concatenate([wl.y, wl.y * 0.5])
+    traj = np.concatenate([wl.traj, wl.traj], axis=1)
+    off = np.concatenate([wl.k_offsets, wl.k_offsets[1:] + wl.k_offsets[-1]])
+    be2 = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=wl.W, osf=wl.osf)
+    be2.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(traj), torch.from_numpy(np.concatenate([wl.dcf] * 2)),
+                c64(wl.maps), device=DEV, k_offsets=off)
+    eng = NLCGEngine(be2.plan, c64(y), problems=2, frames=wl.B, regm=regm, direction="prplus", rel_tol=1e-3,
+                     restart_c=0.0)
+    st0, it0, last0 = eng.latest()
+    assert list(it0) == [0, 0] and last0[0]["gnorm"] == 0.0
+    x0 = np.concatenate([ora.adjoint(wl.y), ora.adjoint(wl.y * 0.5)])
+    eng.prime(c64(x0))
+    eng.run(40)
+    st, it, last = eng.latest()
+    hist = eng.history()
+    s2, i2 = eng.status()
+    assert list(st) == list(s2) and list(it) == list(i2)
+    for p in range(2):
+        h = hist[p][-1]
+        for k in ("gnorm", "xnorm", "stepnorm", "f", "f_step", "step"):
+            assert last[p][k] == h[k], (p, k)

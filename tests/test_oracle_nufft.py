@@ -187,3 +187,29 @@ def test_kb_nufft3_reduces_to_2d_on_a_flat_axis():
     m0, w = footprint(np.zeros(1), W, beta)
     fz = (1.0 / kb_ft(0.0, W, beta)) * w.sum()
     assert np.linalg.norm(y3 - fz * y2) / np.linalg.norm(y2) < 1e-12
+
+
+@pytest.mark.parametrize("mode,fwd,adj", [("none", False, True), ("standard", False, True),
+                                          ("standard", True, True), ("standard", True, False),
+                                          ("standard", False, False)])
+def test_dcf_policy_matches_the_weighted_nudft(mode, fwd, adj):
+    """DCF policy of torchkb.py:185-199: 'none' weights neither side; 'standard' multiplies by
+    dcf (not its square root) on the sides apply_dcf_in_fwd / apply_dcf_in_adj name.  Pinned by
+    the unweighted NUDFT with the weights applied by hand (W = 6 gridding error <= 3e-5)."""
+    B, L, C, S, K, maps, traj, dcf, x = reference_precision_problem(2)
+    op = KBNufft(maps, flat_traj(traj), np.arange(B + 1) * K, dcf.reshape(-1), 6, 2.0, like=L,
+                 dcf_mode=mode, dcf_in_fwd=fwd, dcf_in_adj=adj)
+    use_f = mode == "standard" and fwd
+    use_a = mode == "standard" and adj
+    y_un = np.stack([nudft_forward(maps, x[b], traj[b]) for b in range(B)])          # (B, L, C, K)
+    ygt = y_un * (dcf[:, None, None, :] if use_f else 1.0)
+    rng = np.random.default_rng(3)
+    yr = rng.standard_normal(y_un.shape) + 1j * rng.standard_normal(y_un.shape)
+    xgt = np.stack([nudft_adjoint(maps, yr[b] * (dcf[b][None, None] if use_a else 1.0), traj[b])
+                    for b in range(B)])
+    assert rel_l2(op.forward(x.reshape(B * L, S, S)), ygt) <= 3e-5
+    assert rel_l2(op.adjoint(yr.reshape(-1)), xgt) <= 3e-5
+    # planted mistake: the balanced square root on a 'standard' side fails the pin
+    if use_f:
+        bad = y_un * np.sqrt(dcf)[:, None, None, :]
+        assert rel_l2(op.forward(x.reshape(B * L, S, S)), bad) > 1e-2
