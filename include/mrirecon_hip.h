@@ -198,6 +198,17 @@ typedef struct {
 int mrh_tv_energy_grad(const mrh_tv_desc* desc, const void* x, void* g, double* work,
                        double* energy, void* stream);
 
+/* mrh_tv_energy_grad without the workspace argument and without any allocation: the block
+ * partials live in a static device array of 1 MiB per GPU (131072 doubles, the length
+ * mrh_reduce_workspace_len() gives), whose address is looked up once per device on the first
+ * call.  Same kernels and partition as mrh_tv_energy_grad, so the results are identical.
+ * 1 <= desc->P <= 65535, else MRH_EINVAL.  Calls that overlap on one GPU (other streams or
+ * threads) share the array and race.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). */
+int mrh_tv_energy_grad_noalloc(const mrh_tv_desc* desc, const void* x, void* g, double* energy,
+                               void* stream);
+
 /* ------------------------------------------------------------------ device NLCG engine */
 
 typedef struct mrh_cg mrh_cg;

@@ -96,6 +96,7 @@ _SIGS = {
                                     c_void_p, c_void_p, c_void_p]),
     "mrh_dot_real": (c_int, [c_void_p, c_void_p, c_int64, c_void_p, c_void_p, c_void_p]),
     "mrh_tv_energy_grad": (c_int, [POINTER(TVDesc), c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "mrh_tv_energy_grad_noalloc": (c_int, [POINTER(TVDesc), c_void_p, c_void_p, c_void_p, c_void_p]),
     "mrh_cg_create": (c_int, [POINTER(CGDesc), POINTER(c_void_p)]),
     "mrh_cg_destroy": (c_int, [c_void_p]),
     "mrh_cg_reset": (c_int, [c_void_p, c_void_p]),

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #define RT 256            // threads per reduction block
@@ -300,6 +301,57 @@ extern "C" int mrh_tv_energy_grad(const mrh_tv_desc* desc, const void* x, void* 
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_tv_energy_grad, dim3(nb, desc->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
                      tv, npix_p, work);
+  MRH_CHECK_LAUNCH();
+  hipLaunchKernelGGL(k_finish_per_problem, dim3(desc->P), dim3(RT), 0, st, (const double*)work, nb, energy);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+// ---------------------------------------------------------------- TV without allocation
+// mrh_tv_energy_grad_noalloc keeps its block partials in this file-scope array instead of a
+// caller's workspace.  Host code must not pass `tv_scratch` itself to a kernel (in host code the
+// name is a stand-in, not a device address): each device's address comes from
+// hipGetSymbolAddress, fetched once per device on its first call and kept below.
+#define TV_SCRATCH_DOUBLES (RED_BLOCKS * 64)   // 1 MiB per device, as mrh_reduce_workspace_len()
+#define TV_SCRATCH_MAX_DEVICES 64
+static __device__ __align__(256) double tv_scratch[TV_SCRATCH_DOUBLES];
+static double* g_tv_scratch_ptr[TV_SCRATCH_MAX_DEVICES];
+static std::mutex g_tv_scratch_mu;
+
+static int tv_scratch_for_current_device(double** out) {
+  int dev = -1;
+  MRH_HIP(hipGetDevice(&dev));
+  MRH_REQUIRE(dev >= 0 && dev < TV_SCRATCH_MAX_DEVICES, MRH_EUNSUP, "tv_energy_grad_noalloc: device %d beyond %d",
+              dev, TV_SCRATCH_MAX_DEVICES);
+  std::lock_guard<std::mutex> lock(g_tv_scratch_mu);
+  if (!g_tv_scratch_ptr[dev]) {
+    void* p = nullptr;
+    MRH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(tv_scratch)));
+    g_tv_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_tv_scratch_ptr[dev];
+  return MRH_OK;
+}
+
+extern "C" int mrh_tv_energy_grad_noalloc(const mrh_tv_desc* desc, const void* x, void* g, double* energy,
+                                          void* stream) {
+  MRH_REQUIRE(desc && x && energy, MRH_EINVAL, "tv_energy_grad_noalloc: null argument");
+  MRH_REQUIRE(desc->P >= 1 && desc->P <= 65535, MRH_EINVAL,
+              "tv_energy_grad_noalloc: %d problems; the static scratch serves 1 to 65535", desc->P);
+  TVDev tv;
+  int rc = make_tvdev(desc, &tv);
+  if (rc) return rc;
+  double* work = nullptr;
+  rc = tv_scratch_for_current_device(&work);
+  if (rc) return rc;
+  // the same partition as mrh_tv_energy_grad (whose workspace has the same length), so both
+  // entry points return identical sums; every partial is written before it is read
+  const int64_t npix_p = (int64_t)desc->F * desc->S0 * desc->S1;
+  int nb = red_blocks(npix_p);
+  nb = std::max(1, std::min<int>(nb, (int)(TV_SCRATCH_DOUBLES / desc->P)));
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_tv_energy_grad, dim3(nb, desc->P), dim3(RT), 0, st, (const float2*)x, (float2*)g, tv,
+                     npix_p, work);
   MRH_CHECK_LAUNCH();
   hipLaunchKernelGGL(k_finish_per_problem, dim3(desc->P), dim3(RT), 0, st, (const double*)work, nb, energy);
   MRH_CHECK_LAUNCH();
