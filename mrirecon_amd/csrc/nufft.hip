@@ -564,6 +564,12 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
   }
 }
 
+// Running max of |re|, |im| over the weighted samples; a non-finite sample makes it +inf
+// (fmaxf alone would drop a NaN).
+__device__ __forceinline__ float fmax_finite(float m, float2 v) {
+  return (isfinite(v.x) && isfinite(v.y)) ? fmaxf(m, fmaxf(fabsf(v.x), fabsf(v.y))) : INFINITY;
+}
+
 // Deterministic fixed-point accumulation for the adjoint.  Per strip and coil
 // the weighted samples are scaled by 2^sc so that the largest component is
 // below 2^30; every contribution is rounded once to an integer (a quantum of
@@ -695,12 +701,12 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
 #pragma unroll
     for (int q = 0; q < STRIP_EPT; ++q) {
       v[q] = vn[q];
-      m = fmaxf(m, fmaxf(fabsf(v[q].x), fabsf(v[q].y)));
+      m = fmax_finite(m, v[q]);
     }
     if (c + 1 < ce) fetch(c + 1);
     for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
       const float2 vo = cscale(yc[em[e].y], ef[e].w);
-      m = fmaxf(m, fmaxf(fabsf(vo.x), fabsf(vo.y)));
+      m = fmax_finite(m, vo);
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
@@ -709,10 +715,14 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
     float vmax = 0.f;
 #pragma unroll
     for (int w = 0; w < NTH / 64; ++w) vmax = fmaxf(vmax, wmax[w]);
-    int ex;
-    frexpf(vmax, &ex);   // vmax < 2^ex
+    // vmax = +inf: a weighted sample of the strip is not finite; nothing is spread and the
+    // strip's output for this coil is NaN (as the NaN or inf would reach every grid point of
+    // the strip's columns through the inverse FFT)
+    const bool bad = !(vmax < INFINITY);
+    int ex = 0;
+    if (!bad) frexpf(vmax, &ex);   // vmax < 2^ex
     const int sc = 30 - ex;
-    if (vmax > 0.f) {
+    if (vmax > 0.f && !bad) {
 #pragma unroll
       for (int q = 0; q < STRIP_EPT; ++q)
         if (F[q].k >= 0)
@@ -744,7 +754,7 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
       const int ix = e / TC, j = e - ix * TC;
       const int r = ix >= h0 ? ix - h0 : ix - h0 + G0;
       float2* sp = fbuf + 2 * j * CS + PIDX(r);
-      Tb[e] = *sp;
+      Tb[e] = bad ? make_float2(__int_as_float(0x7fc00000), __int_as_float(0x7fc00000)) : *sp;
       *sp = make_float2(0.f, 0.f);
     }
   }

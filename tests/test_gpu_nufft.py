@@ -337,3 +337,27 @@ def test_dcf_modes_match_oracle(mode, fwd, adj):
     e_a = rel_l2(xa.cpu().numpy(), ora.adjoint(yr))
     print(f"[c1 dcf {mode} fwd={fwd} adj={adj}] forward rel={e_f:.3e} adjoint rel={e_a:.3e}")
     assert e_f <= TOL and e_a <= TOL
+
+
+@pytest.mark.parametrize("bad", [float("nan"), float("inf")])
+def test_non_finite_sample_poisons_only_its_frame(bad):
+    """A NaN or inf k-space sample makes the adjoint image of its frame NaN (as it would through
+    the reference's torch gridding and inverse FFT) and leaves every other frame unchanged."""
+    S, C, R = 48, 3, 96
+    spokes = [13, 21, 17]
+    traj, dcf, off = synth.radial_frames(spokes, R)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=6)
+    be.prepare((C, S, S), torch.from_numpy(traj), torch.from_numpy(dcf), c64(synth.gaussian_maps(C, S)),
+               device=DEV, k_offsets=off)
+    plan = be.plan
+    yr = c64(crand(plan.n_k, 41))
+    clean = torch.empty((3, S, S), dtype=torch.complex64, device=DEV)
+    plan.adjoint(yr, clean)
+    k = C * int(off[1]) + 2 * int(off[2] - off[1]) + 100        # frame 1, coil 2, sample 100
+    ybad = yr.clone()
+    ybad[k] = complex(bad, 0.0)
+    out = torch.empty_like(clean)
+    plan.adjoint(ybad, out)
+    torch.cuda.synchronize()
+    assert bool(torch.isnan(out[1]).all())
+    assert torch.equal(out[0], clean[0]) and torch.equal(out[2], clean[2])
