@@ -21,7 +21,6 @@ import argparse
 import json
 import math
 import os
-import platform
 import sys
 import time
 
@@ -72,60 +71,118 @@ def load_traffic(kernel):
         return None
 
 
-def cpu_baseline(wl, frames=8, reps=4):
-    """Oracle (numpy FP32 port, single thread) NUFFT pairs on a bounded sample of frames."""
+def cpu_baseline(gpu_c1=None, plain_iterations=None):
+    """The pure-torch CPU port of the operator and the reference NLCG loop (oracle/torch_cpu.py:
+    torch.fft, gathers and index_add_, complex64, FP32 reductions), timed on every core this
+    process may use: NUFFT pairs/s at c1, c2 and c3 (one warm-up, median of 3); NLCG to the 1e-4
+    relative objective change at c1 in full (PR+ with the descent restart, Armijo that evaluates
+    A^H at every trial, as the reference loop does); on the GRASP workload c2 the prime step and 3
+    full-volume iterations, measured, and the time to tolerance they imply at the GPU's median
+    plain iteration count."""
     import numpy as np
-    from oracle.kbnufft import KBNufft
-    off = wl.k_offsets
-    idx = np.concatenate([np.arange(off[b], off[b + 1]) for b in range(frames)])
-    ora = KBNufft(wl.maps, wl.traj[:, idx], off[:frames + 1], wl.dcf[idx], wl.W, wl.osf, dtype=np.float32)
-    x = wl.x_true[:frames]
-    y = ora.forward(x)
-    ora.adjoint(y)                                   # warm-up
-    ts = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        y = ora.forward(x)
-        ora.adjoint(y)
-        ts.append(time.perf_counter() - t0)
-    t = sorted(ts)[len(ts) // 2]
-    ncf = frames * wl.n_coils
-    return {"value": ncf / t, "unit": "coil-frame NUFFT pairs/s", "cores": 1, "kind": "port",
-            "sample": f"{frames} of {wl.B} frames x {wl.n_coils} coils ({ncf} coil-frames), median of {reps} "
-                      f"forward+adjoint pairs of the numpy FP32 oracle (single-threaded pocketfft/bincount) "
-                      f"on {platform.processor() or platform.machine()}",
-            "seconds_per_pair_sample": t}
+    import torch
 
-
-def cpu_nlcg_estimate(wl, plain_iterations, frames=4, iters=2):
-    """CPU time-to-tolerance estimate: the oracle's reference-semantics NLCG (PR+ with the
-    descent restart, Armijo backtracking that re-evaluates A^H at every trial) timed for a few
-    iterations on a frame sample, scaled to the whole volume and the GPU run's iteration count."""
-    import numpy as np
     from mrirecon_amd import synth
-    from oracle.kbnufft import KBNufft
-    from oracle.nlcg import TV, nlcg
-    off = wl.k_offsets
-    idx = np.concatenate([np.arange(off[b], off[b + 1]) for b in range(frames)])
-    ora = KBNufft(wl.maps, wl.traj[:, idx], off[:frames + 1], wl.dcf[idx], wl.W, wl.osf, dtype=np.float32)
-    y = ora.forward(wl.x_true[:frames])
-    y = (y + synth.complex_noise(y.shape, 0.01 * float(np.abs(y).max()), 5)).astype(np.complex64)
-    x0 = ora.adjoint(y)
-    regs = [TV(wl.lam_t, (0,)), TV(wl.lam_s, (1, 2))]
+    from oracle.torch_cpu import TorchKBNufft, nlcg
+
+    cores = len(os.sched_getaffinity(0))
+    torch.set_num_threads(cores)
+    rates, ops = {}, {}
+    for name in ("c1", "c2", "c3"):
+        wl = getattr(synth, f"config_{name}")()
+        op = TorchKBNufft(wl.maps, wl.traj, wl.k_offsets, wl.dcf, wl.W, wl.osf)
+        x = torch.from_numpy(wl.x_true)
+        op.adjoint(op.forward(x))
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            op.adjoint(op.forward(x))
+            ts.append(time.perf_counter() - t0)
+        rates[name] = round(wl.B * wl.n_coils / float(np.median(ts)), 2)
+        ops[name] = (wl, op)
+        print(f"bench: CPU torch port {name}: {rates[name]} coil-frame pairs/s ({cores} threads)",
+              file=sys.stderr, flush=True)
+
+    def problem(name):
+        wl, op = ops[name]
+        y = op.forward(torch.from_numpy(wl.x_true))
+        y = y + torch.from_numpy(synth.complex_noise(tuple(y.shape), 0.01 * float(y.abs().max()), wl.noise_seed))
+        regs = [(wl.lam_t, (0,))] * (wl.lam_t > 0) + [(wl.lam_s, (1, 2))] * (wl.lam_s > 0)
+        return wl, op, y, regs
+
+    wl, op, y1, regs = problem("c1")
     t0 = time.perf_counter()
-    res = nlcg(ora.forward, ora.adjoint, y, x0, regs, max_iter=iters, direction="prplus", restart_c=0.0,
-               mutate_x=False)
-    t = time.perf_counter() - t0
-    per_iter = t / max(1, res.n_iter)
-    return {"seconds_per_iteration_sample": round(per_iter, 3), "sample_frames": frames,
-            "iterations_timed": res.n_iter,
-            "time_to_tol_estimate_seconds": round(per_iter * wl.B / frames * plain_iterations, 1),
-            "estimate": f"{res.n_iter} oracle NLCG iterations on {frames} of {wl.B} frames (single thread, FP32 "
-                        f"numpy), scaled by {wl.B}/{frames} frames and by the GPU plain run's median "
-                        f"{plain_iterations} iterations"}
+    _, h1 = nlcg(op.forward, op.adjoint, y1, op.adjoint(y1), regs, max_iter=200, direction="prplus", restart_c=0.0,
+                 rel_tol=1e-4)
+    t_c1 = time.perf_counter() - t0
+    wl, op, y2, regs = problem("c2")
+    t0 = time.perf_counter()
+    _, h2 = nlcg(op.forward, op.adjoint, y2, op.adjoint(y2), regs, max_iter=3, direction="prplus", restart_c=0.0)
+    t_c2 = time.perf_counter() - t0
+    per_it = t_c2 / max(1, len(h2))
+    nl = {"c1_time_to_tol_s": round(t_c1, 3), "c1_iterations": len(h1),
+          "c2_seconds_for_3_iterations": round(t_c2, 3), "c2_seconds_per_iteration": round(per_it, 3)}
+    if gpu_c1:
+        nl["c1_gpu_time_to_tol_s"] = gpu_c1["seconds"]
+        nl["c1_gpu_iterations"] = gpu_c1["iterations"]
+    if plain_iterations:
+        nl["c2_time_to_tol_estimate_s"] = round(per_it * plain_iterations, 1)
+        nl["c2_estimate"] = (f"measured s/iteration (prime included) x the GPU plain run's median "
+                             f"{plain_iterations} iterations")
+    return {"value": rates["c2"], "unit": "coil-frame NUFFT pairs/s", "cores": cores, "kind": "port",
+            "sample": f"whole GRASP problem (408 coil-frames), median of 3 pairs, torch CPU port, {cores} threads",
+            "pairs_per_s": rates, "nlcg": nl}
 
 
-def time_to_tolerance(args, dev):
+def gpu_c1_time_to_tol(dev):
+    """c1 on the GPU from the same data as the CPU port's c1 run (y = the torch port's forward of the
+    phantom + 1 % noise, seed 102): adjoint + NLCG to the 1e-4 relative change (PR+ with restart)."""
+    import numpy as np
+    import torch
+
+    from mrirecon_amd import NLCGEngine, RegManager, TVND, TVParams, synth
+    from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+    from oracle.torch_cpu import TorchKBNufft
+
+    wl = synth.config_c1()
+    op = TorchKBNufft(wl.maps, wl.traj, wl.k_offsets, wl.dcf, wl.W, wl.osf)
+    y = op.forward(torch.from_numpy(wl.x_true))
+    y = (y + torch.from_numpy(synth.complex_noise(tuple(y.shape), 0.01 * float(y.abs().max()), wl.noise_seed)))
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=wl.W, osf=wl.osf)
+    be.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(wl.traj), torch.from_numpy(wl.dcf),
+               torch.from_numpy(wl.maps).to(dev), device=dev, k_offsets=wl.k_offsets)
+    yd = y.to(dev)
+    eng = NLCGEngine(be.plan, yd, problems=1, frames=1, regm=RegManager([TVND("s", TVParams(weight=wl.lam_s))]),
+                     direction="prplus", rel_tol=1e-4, restart_c=0.0, max_hist=201)
+    xa = torch.empty((1, wl.S, wl.S), dtype=torch.complex64, device=dev)
+    be.plan.adjoint(yd, xa)
+    eng.prime(xa)
+    eng.capture()
+    ts = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        be.plan.adjoint(yd, xa)
+        eng.prime(xa)
+        it = eng.run(200)
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return {"seconds": round(float(np.median(ts)), 5), "iterations": int(eng.status()[1][0])}
+
+
+def pregen_phantoms(seeds, workers):
+    """GRASP phantoms of the held-out seeds, generated in forked worker processes before this
+    process touches the GPU."""
+    import multiprocessing as mp
+
+    from mrirecon_amd.train_warmstart import _phantom
+    if workers <= 1 or len(seeds) < 4:
+        return [_phantom(s) for s in seeds]
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_phantom, seeds, chunksize=4)
+
+
+def time_to_tolerance(args, dev, seeds, phantoms):
     """Plain vs warm-started NLCG on held-out GRASP volumes, plus the U-Net's MFMA rate.
 
     Both runs: PR+ with the beta = 0 descent restart, Armijo (c1 1e-4, halving
@@ -143,13 +200,13 @@ def time_to_tolerance(args, dev):
     print(f"bench: time-to-1e-4 on {args.tol_volumes} held-out GRASP volumes (plain vs warm start)",
           file=sys.stderr, flush=True)
     pb = GraspProblem(max_iter=args.tol_iters)
-    seeds = list(range(520_000, 520_000 + args.tol_volumes))
-    phantoms = [_phantom(s) for s in seeds]
     net = UNet2DT.load(weights, device=dev)
     pb.measure(phantoms[0], noise_seed=seeds[0] + 7)      # warm-up: workspaces, graphs
     run_warm(pb, net)
     res = evaluate(pb, net, seeds, phantoms, log=lambda *a: print(*a, file=sys.stderr, flush=True))
     rows = res.pop("rows")
+    ratio = np.array([r["plain_s"] / r["warm_s"] for r in rows])
+    per_volume = [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows]
     ttt = {"volumes": len(rows), "seeds": f"{seeds[0]}..{seeds[-1]} (not in the training set)",
            "solver": "PR+ with beta=0 descent restart, Armijo c1=1e-4 halving from t=1 (20 trials), "
                      "temporal TV 0.02 + spatial TV 0.002 (eps 1e-3), FP64 reductions, HIP-graph iteration",
@@ -167,10 +224,16 @@ def time_to_tolerance(args, dev):
            "warm_final_objective_rel_to_plain_median": round(res["median_rel_f_gap"], 6),
            "warm_start_objective_over_plain_final_median": round(res["median_f_warm_start_over_plain_final"], 4),
            "network_and_fallback_check_ms_median": round(float(np.median([r["warm_start_ms"] for r in rows])), 3),
-           "weights": "mrirecon_amd/weights/unet_grasp.npz", "per_volume": [
-               {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows]}
+           "weights": "mrirecon_amd/weights/unet_grasp.npz",
+           "speedup_quartiles": [round(float(q), 3) for q in np.percentile(ratio, [25, 50, 75])],
+           "speedup_min_max": [round(float(ratio.min()), 3), round(float(ratio.max()), 3)],
+           "volumes_below_3x": int((ratio < 3.0).sum()),
+           "warm_final_f_above_plain": int(sum(r["f_warm_final"] > r["f_plain_final"] for r in rows)),
+           "iterations_saved_median": float(np.median([r["plain_iters"] - r["warm_iters"] for r in rows])),
+           "per_volume": per_volume}
     if not args.no_batched_tol:
-        ttt["batched"] = batched_grasp(args, dev, net, seeds, phantoms)
+        nb = min(16, len(seeds))
+        ttt["batched"] = batched_grasp(args, dev, net, seeds[:nb], phantoms[:nb])
     # U-Net forward alone: dense MFMA FLOPs over HIP-event time on the launch stream
     x = pb.x_adj
     for _ in range(3):
@@ -432,6 +495,78 @@ def tbasis_rate(dev, T=34, K=8, S=256, vols=1, reps=20):
     return res
 
 
+def c3_precision(dev, iters=20):
+    """Precision study on c3 (384^2, 16 coils, 402 golden-angle spokes x 768, W = 6, x ~ CN(0, 1)
+    seed 301): the FP32 NUFFT pair against the FP64 oracle (relative L2, max-abs over max), and
+    NLCG (PR+ with the descent restart, Armijo, spatial TV 0.01, y = A x + 1 % noise) for 20
+    iterations with the engine's FP64 inner products / beta / line-search values against the
+    all-FP32 variant (the reference's precision, graspcg/solvers/cg.py:89-90,
+    graspcg/ops/dot.py:54-71), both final objectives evaluated in FP64 on the device."""
+    import numpy as np
+    import torch
+
+    from mrirecon_amd import NLCGEngine, RegManager, TVND, TVParams, synth
+    from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
+    from mrirecon_amd.warmstart import ObjectiveEval
+    from oracle.kbnufft import KBNufft, rel_l2
+
+    print("bench: c3 precision study (operator vs FP64 oracle, NLCG FP64 vs FP32 reductions)", file=sys.stderr,
+          flush=True)
+    wl = synth.config_c3()
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=wl.W, osf=wl.osf)
+    be.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(wl.traj), torch.from_numpy(wl.dcf),
+               torch.from_numpy(wl.maps).to(dev), device=dev, k_offsets=wl.k_offsets)
+    plan = be.plan
+    x = torch.from_numpy(wl.x_true).to(dev)
+    y = torch.empty(plan.n_k, dtype=torch.complex64, device=dev)
+    plan.forward(x, y)
+    rng = np.random.default_rng(303)
+    yr = (rng.standard_normal(plan.n_k) + 1j * rng.standard_normal(plan.n_k)).astype(np.complex64)
+    xa = torch.empty_like(x)
+    plan.adjoint(torch.from_numpy(yr).to(dev), xa)
+    ora = KBNufft(wl.maps, wl.traj, wl.k_offsets, wl.dcf, wl.W, wl.osf)
+    yf, xr = ora.forward(wl.x_true), ora.adjoint(yr)
+    y_np, xa_np = y.cpu().numpy(), xa.cpu().numpy()
+    op = {"forward_rel_l2": float(rel_l2(y_np, yf)), "adjoint_rel_l2": float(rel_l2(xa_np, xr)),
+          "forward_max_abs_over_max": float(np.abs(y_np - yf).max() / np.abs(yf).max()),
+          "adjoint_max_abs_over_max": float(np.abs(xa_np - xr).max() / np.abs(xr).max())}
+    g = torch.Generator(device=dev)
+    g.manual_seed(302)
+    nz = torch.randn((2, plan.n_k), generator=g, device=dev, dtype=torch.float32)
+    y += torch.complex(nz[0], nz[1]) * (0.01 * float(y.abs().max()) / math.sqrt(2.0))
+    regm = RegManager([TVND("tv_s", TVParams(weight=0.01, axes="spatial"))])
+    x0 = torch.empty_like(x)
+    plan.adjoint(y, x0)
+    fev = ObjectiveEval(plan, y, regm, problems=1, frames=1)
+    res = {}
+    for name, fp32 in (("fp64", False), ("fp32", True)):
+        eng = NLCGEngine(plan, y, problems=1, frames=1, regm=regm, direction="prplus", restart_c=0.0,
+                         max_hist=iters + 1, fp32_reductions=fp32)
+        eng.prime(x0)
+        eng.capture()
+        eng.prime(x0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        eng.run(iters)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        h = eng.history()[0]
+        res[name] = {"objective": float(fev(eng.x)[0]), "iterations": len(h), "steps": [r["step"] for r in h],
+                     "ms_per_iteration": round(dt * 1e3 / max(1, len(h)), 4)}
+        eng.close()
+    f64, f32 = res["fp64"]["objective"], res["fp32"]["objective"]
+    same = 0
+    for a, b in zip(res["fp64"]["steps"], res["fp32"]["steps"]):
+        if a != b:
+            break
+        same += 1
+    return {"operator": {k: float(f"{v:.3e}") for k, v in op.items()},
+            "nlcg": {"iterations": iters, "objective_fp64_reductions": round(f64, 6),
+                     "objective_fp32_reductions": round(f32, 6), "rel_gap_fp32_minus_fp64": float(f"{(f32 - f64) / f64:.3e}"),
+                     "identical_leading_steps": same, "ms_per_iteration_fp64": res["fp64"]["ms_per_iteration"],
+                     "ms_per_iteration_fp32": res["fp32"]["ms_per_iteration"]}}
+
+
 def batched_uneven(args, dev):
     """Batched uneven workload: P slices x 20 frames at 256^2, 8 coils, 13..55 golden-angle spokes
     per frame x 512, all slices in the same launches (ragged k-space, per-slice stop), temporal TV 0.02.
@@ -522,7 +657,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU oracle baseline")
     ap.add_argument("--no-tol", action="store_true", help="skip the time-to-tolerance NLCG run")
     ap.add_argument("--tol-iters", type=int, default=200)
-    ap.add_argument("--tol-volumes", type=int, default=16, help="held-out volumes of the time-to-tolerance runs")
+    ap.add_argument("--tol-volumes", type=int, default=200,
+                    help="held-out volumes of the time-to-tolerance runs (seeds 600000 + i)")
+    ap.add_argument("--workers", type=int, default=16, help="processes generating the held-out phantoms")
+    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+                    help="file for the full results (per-volume rows, autotuner times, ...)")
     ap.add_argument("--no-batched-tol", action="store_true",
                     help="skip the run of the same volumes as one batched problem set")
     ap.add_argument("--c4-slices", type=int, default=48, help="slices of the batched uneven workload")
@@ -537,6 +676,12 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding of "
               "frames / slices is a design note, not built in this tree.", file=sys.stderr)
         sys.exit(2)
+
+    tol_seeds = list(range(600_000, 600_000 + args.tol_volumes))
+    phantoms = None
+    if not args.no_tol and not args.copy_probe_mib:
+        print(f"bench: generating {len(tol_seeds)} held-out GRASP phantoms", file=sys.stderr, flush=True)
+        phantoms = pregen_phantoms(tol_seeds, min(args.workers, os.cpu_count() or 1))
 
     import numpy as np
     import torch
@@ -616,7 +761,7 @@ def main():
 
     ttt, unet = None, None
     if not args.no_tol:
-        ttt, unet = time_to_tolerance(args, dev)
+        ttt, unet = time_to_tolerance(args, dev, tol_seeds, phantoms)
 
     configs = None
     if not args.no_configs:
@@ -624,6 +769,7 @@ def main():
         configs = {"c1 (128^2, 8 coils, 64 spokes x 256, W=4)": nufft_rate(synth.config_c1(), dev)[1],
                    "c3 (384^2, 16 coils, 402 spokes x 768, W=6)": nufft_rate(synth.config_c3(), dev)[1],
                    "3-D (128^3, 8 coils, 1024 kooshball spokes x 256, W=6)": nufft3_rate(dev)}
+    c3p = None if args.no_configs else c3_precision(dev)
     c4 = None
     if not args.no_c4:
         c4 = batched_uneven(args, dev)
@@ -634,26 +780,51 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        print("bench: CPU oracle baseline on a bounded sample", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(wl)
-        if ttt:
-            cpu["nlcg"] = cpu_nlcg_estimate(wl, ttt["plain_iterations_median"])
+        print("bench: CPU baseline (torch CPU port) and c1 NLCG on the GPU", file=sys.stderr, flush=True)
+        c1_gpu = gpu_c1_time_to_tol(dev)
+        cpu = cpu_baseline(c1_gpu, ttt["plain_iterations_median"] if ttt else None)
 
-    out = {
+    detail = {"roofline": roofline, "kernels": per_kernel, "pipeline": pipeline, "time_to_tol": ttt, "unet": unet,
+              "nufft_configs": configs, "c3_precision": c3p, "batched_uneven": c4, "solver_variants": variants,
+              "temporal_basis": tbasis, "cpu_baseline": cpu}
+    try:
+        os.makedirs(os.path.dirname(os.path.abspath(args.detail)), exist_ok=True)
+        with open(args.detail, "w") as fh:
+            json.dump(detail, fh, indent=1)
+    except OSError as e:
+        print(f"bench: could not write {args.detail}: {e}", file=sys.stderr)
+    print("bench detail: " + json.dumps({k: v for k, v in detail.items() if k != "time_to_tol"}), file=sys.stderr)
+    ttt_line = None
+    if ttt:
+        ttt_line = {k: ttt[k] for k in ("volumes", "plain_ms_median", "plain_iterations_median", "warm_ms_median",
+                                        "warm_iterations_median", "speedup_median", "speedup_quartiles",
+                                        "speedup_min_max", "volumes_below_3x", "matched_objective_speedup_median",
+                                        "matched_objective_reached_fraction", "fallback_rate",
+                                        "iterations_saved_median", "warm_final_f_above_plain",
+                                        "network_and_fallback_check_ms_median")}
+        if ttt.get("batched"):
+            ttt_line["batched16_speedup"] = ttt["batched"]["speedup"]
+    cpu_line = None
+    if cpu:
+        cpu_line = dict(cpu)
+    line = {
         "metric": METRIC, "value": round(value, 1), "unit": "coil-frame NUFFT pairs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
         "data": "synthetic (seeded GRASP phantom, golden-angle radial, Gaussian coil maps)",
         "config": {"workload": "GRASP dynamic 256x256 x 34 frames, 12 coils, 21 golden-angle spokes x 512 "
                                "samples per frame, KB W=6, osf 2, complex64",
-                   "coil_frames": ncf, "samples_per_frame": int(wl.uniform_K), "strip_cols": info["strip_cols"],
-                   "kb_terms": info["kb_terms"], "coil_chunk": info["coil_chunk"],
-                   "parallelism": "1 GPU"},
-        "roofline": roofline, "kernels": per_kernel, "pipeline": pipeline,
-        "time_to_tol": ttt, "unet": unet, "nufft_configs": configs, "batched_uneven": c4,
-        "solver_variants": variants, "temporal_basis": tbasis, "cpu_baseline": cpu,
+                   "coil_frames": ncf, "samples_per_frame": int(wl.uniform_K), "parallelism": "1 GPU"},
+        "roofline": {k: roofline[k] for k in ("bound", "kernel", "achieved", "peak", "unit", "frac", "traffic")},
+        "cpu_baseline": cpu_line,
+        "kernels_ms": {n: per_kernel[n]["avg_ms"] for n in names},
+        "time_to_tol": ttt_line,
+        "unet_mfma_frac": unet["mfma_frac"] if unet else None,
+        "batched_uneven_speedup": c4["speedup"] if c4 else None,
+        "c3_fp32_vs_fp64_objective_gap": c3p["nlcg"]["rel_gap_fp32_minus_fp64"] if c3p else None,
+        "detail": args.detail,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(line), flush=True)
 
 
 if __name__ == "__main__":

@@ -252,6 +252,11 @@ typedef struct {
                                symmetrised graph)                                      */
   const float* gl_prof;     /* host F: its diagonal majoriser per frame (lambda deg or
                                lambda [deg > 0], graph_laplacian.py:283-321)            */
+  int fp32_reductions;      /* 1: every inner product, norm, energy, beta and line-search
+                               value in FP32, as the reference computes them
+                               (graspcg/solvers/cg.py:89-90, graspcg/ops/dot.py:54-71) -- for
+                               precision studies; Armijo search without a graph term only.
+                               0: FP64 (default)                                          */
 } mrh_cg_desc;
 
 /* Per-problem, per-iteration record copied out by mrh_cg_read_history. */

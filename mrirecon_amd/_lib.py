@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 11
+ABI_VERSION = 12
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -66,7 +66,7 @@ class CGDesc(Structure):
                 ("ls_beta", c_double), ("ls_max_iter", c_int), ("rel_tol", c_double), ("max_hist", c_int),
                 ("restart_c", c_double), ("ls_kind", c_int), ("c2", c_double),
                 ("diag", c_void_p), ("gl_weight", c_double), ("gl_F", c_int), ("gl_L", c_void_p),
-                ("gl_prof", c_void_p)]
+                ("gl_prof", c_void_p), ("fp32_reductions", c_int)]
 
 
 class CGHist(Structure):

@@ -31,21 +31,30 @@
 #define GL_TILE 64        // pixels per workgroup tile of the graph-Laplacian passes
 
 // ---------------------------------------------------------------- block reduction
-template <int NV>
-__device__ __forceinline__ void block_reduce_store(double (&v)[NV], double* out) {
-  __shared__ double sh[NV][RT / 64];
+// R is the accumulation type: double (the engine's FP64 reductions) or float (the all-FP32
+// variant of the precision study, whose partials are stored as doubles holding float values).
+template <typename R>
+__device__ __forceinline__ R warp_sum(R v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+template <int NV, typename R = double>
+__device__ __forceinline__ void block_reduce_store(R (&v)[NV], double* out) {
+  __shared__ R sh[NV][RT / 64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    double w = warp_sum_d(v[k]);
+    R w = warp_sum<R>(v[k]);
     if (lane == 0) sh[k][wid] = w;
   }
   __syncthreads();
   if (threadIdx.x < NV) {
-    double s = 0.0;
+    R s = R(0);
 #pragma unroll
     for (int w = 0; w < RT / 64; ++w) s += sh[threadIdx.x][w];
-    out[threadIdx.x] = s;
+    out[threadIdx.x] = (double)s;
   }
 }
 
@@ -68,31 +77,31 @@ __device__ double block_sum_partials(const double* part, int n, int stride, int 
 // Sums of values [v0, v0 + NV) over n block partials (stride doubles apart) in one
 // sweep: each thread adds its partials for every value, then one block reduction.
 // Thread 0 receives the sums in out.
-template <int NV>
+template <int NV, typename R = double>
 __device__ void block_sum_partials_n(const double* part, int n, int stride, int v0, double* out) {
-  __shared__ double sh[NV][RT / 64];
-  double s[NV];
+  __shared__ R sh[NV][RT / 64];
+  R s[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) s[k] = 0.0;
+  for (int k = 0; k < NV; ++k) s[k] = R(0);
   for (int i = threadIdx.x; i < n; i += RT) {
     const double* q = part + (int64_t)i * stride + v0;
 #pragma unroll
-    for (int k = 0; k < NV; ++k) s[k] += q[k];
+    for (int k = 0; k < NV; ++k) s[k] += (R)q[k];
   }
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
-    const double w = warp_sum_d(s[k]);
+    const R w = warp_sum<R>(s[k]);
     if (lane == 0) sh[k][wid] = w;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      double t = 0.0;
+      R t = R(0);
 #pragma unroll
       for (int w = 0; w < RT / 64; ++w) t += sh[k][w];
-      out[k] = t;
+      out[k] = (double)t;
     }
   }
   __syncthreads();
@@ -411,7 +420,15 @@ struct mrh_cg {
   double* d_part_q;     // [P][nb_q][PQ_STRIDE] graph-term partials
   int nb_q;
   size_t gl_lds_grad, gl_lds_ls;   // dynamic LDS bytes of k_gl_grad / k_gl_ls
+  int fp32;             // all-FP32 reductions (precision study), else FP64
 };
+
+// Launch kernel template K<R> with R = float for an engine with FP32 reductions, else double.
+#define LAUNCH_R(e, K, ...)                                                            \
+  do {                                                                                 \
+    if ((e)->fp32) hipLaunchKernelGGL(K<float>, __VA_ARGS__);                          \
+    else hipLaunchKernelGGL(K<double>, __VA_ARGS__);                                   \
+  } while (0)
 
 // g += lambda L x; block partial: E.
 __global__ __launch_bounds__(256) void k_gl_grad(const float2* __restrict__ x, float2* __restrict__ g, GLDev gl,
@@ -541,22 +558,23 @@ __global__ __launch_bounds__(RT) void k_cg_residual(const float2* __restrict__ A
 }
 
 // Pass 1 of the line search: a = |Ax - y|^2, b = Re<Ax - y, Ad>, c = |Ad|^2 per problem.
+template <typename R>
 __global__ __launch_bounds__(RT) void k_ls_kspace(const float2* __restrict__ Ax, const float2* __restrict__ Ad,
                                                   const float2* __restrict__ y, const int64_t* __restrict__ kprob,
                                                   const ProbState* __restrict__ st, double* part) {
   const int p = blockIdx.y;
-  double v[3] = {0.0, 0.0, 0.0};
+  R v[3] = {R(0), R(0), R(0)};
   if (st[p].status == MRH_CG_ACTIVE) {
     const int64_t k0 = kprob[p], k1 = kprob[p + 1];
     for (int64_t i = k0 + blockIdx.x * (int64_t)RT + threadIdx.x; i < k1; i += (int64_t)gridDim.x * RT) {
       const float2 ax = Ax[i], yy = y[i], ad = Ad[i];
       const float rx = ax.x - yy.x, ry = ax.y - yy.y;
-      v[0] += (double)rx * rx + (double)ry * ry;
-      v[1] += (double)rx * ad.x + (double)ry * ad.y;
-      v[2] += (double)ad.x * ad.x + (double)ad.y * ad.y;
+      v[0] += (R)rx * rx + (R)ry * ry;
+      v[1] += (R)rx * ad.x + (R)ry * ad.y;
+      v[2] += (R)ad.x * ad.x + (R)ad.y * ad.y;
     }
   }
-  block_reduce_store<3>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PK_STRIDE);
+  block_reduce_store<3, R>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PK_STRIDE);
 }
 
 // Pass 2 of the line search: Re<g, d>, |d|^2 (first pass) and TV(x + t_j d) for the
@@ -564,7 +582,7 @@ __global__ __launch_bounds__(RT) void k_ls_kspace(const float2* __restrict__ Ax,
 // steps, which is where Armijo almost always stops; the second pass covers the rest
 // and does work only for problems whose first pass found no step.  Each candidate's
 // sum is accumulated exactly as in a single pass over all candidates.
-template <int J0, int J1>
+template <int J0, int J1, typename R>
 __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, const float2* __restrict__ d,
                                                  const float2* __restrict__ g, TVDev tv, int64_t npix_p,
                                                  double ls_beta, const ProbState* __restrict__ st, double* part) {
@@ -572,9 +590,9 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
   constexpr int NV = (FIRST ? 2 : 0) + (J1 - J0), V0 = FIRST ? 2 : 0;
   const int p = blockIdx.y;
   if (!FIRST && !(st[p].status == MRH_CG_ACTIVE && st[p].ls_pending)) return;   // partials unread: skip
-  double v[NV];
+  R v[NV];
 #pragma unroll
-  for (int j = 0; j < NV; ++j) v[j] = 0.0;
+  for (int j = 0; j < NV; ++j) v[j] = R(0);
   float tj[NT];
   tj[0] = 0.f;
   {
@@ -590,8 +608,8 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
       const int64_t n = (int64_t)p * npix_p + i;
       if (FIRST) {
         const float2 dn = d[n], gn = g[n];
-        v[0] += (double)gn.x * dn.x + (double)gn.y * dn.y;
-        v[1] += (double)dn.x * dn.x + (double)dn.y * dn.y;
+        v[0] += (R)gn.x * dn.x + (R)gn.y * dn.y;
+        v[1] += (R)dn.x * dn.x + (R)dn.y * dn.y;
       }
       for (int k = 0; k < tv.n; ++k) {
         const TVTermDev& tm = tv.t[k];
@@ -627,12 +645,19 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
               }
             }
           }
-          v[V0 + j - J0] += (double)(tm.lam * e);
+          v[V0 + j - J0] += (R)(tm.lam * e);
         }
       }
     }
   }
-  block_reduce_store<NV>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
+  block_reduce_store<NV, R>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
+}
+
+// f(t) of the Armijo candidates: 0.5 (a + 2 b t + c t^2) + TV(t), evaluated in R.
+template <typename R>
+__device__ __forceinline__ double ls_value(const ProbState& s, double t, double tvj) {
+  const R tt = (R)t;
+  return (double)((R)0.5 * ((R)s.a + (R)2 * tt * (R)s.b + tt * tt * (R)s.c) + (R)tvj);
 }
 
 __device__ __forceinline__ void ls_record(ProbState& s, mrh_cg_hist* hist, int max_hist, int P, int p) {
@@ -644,14 +669,15 @@ __device__ __forceinline__ void ls_record(ProbState& s, mrh_cg_hist* hist, int m
 
 // Finish the first pass's reductions and try its steps (one block per problem):
 // Armijo f(t) <= f(0) + c1 t <g, d> for t = 1, beta, ... in that order.
+template <typename R>
 __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* part_i, const double* part_k,
                                                   int nb_i, int nb_k, double c1, double ls_beta, int ls_max,
                                                   mrh_cg_hist* hist, int max_hist, int P, int* active,
                                                   const double* part_q, int nb_q, double gl_lam) {
   const int p = blockIdx.x;
   double vals[3 + 2 + NT_A];
-  block_sum_partials_n<3>(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, 0, vals);
-  block_sum_partials_n<2 + NT_A>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 3);
+  block_sum_partials_n<3, R>(part_k + (int64_t)p * nb_k * PK_STRIDE, nb_k, PK_STRIDE, 0, vals);
+  block_sum_partials_n<2 + NT_A, R>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals + 3);
   gl_collect(st[p], part_q, nb_q, p);
   if (threadIdx.x != 0) return;
   ProbState& s = st[p];
@@ -659,7 +685,7 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
   if (s.status != MRH_CG_ACTIVE) { s.t = 0.0; return; }
   s.a = vals[0]; s.b = vals[1]; s.c = vals[2]; s.g0d = vals[3]; s.dd = vals[4];
   for (int j = 0; j < NT_A; ++j) s.tv[j] = vals[5 + j];
-  const double f0 = 0.5 * s.a + s.tv[0] + gl_value(s, gl_lam, 0.0);
+  const double f0 = (double)((R)0.5 * (R)s.a + (R)s.tv[0]) + gl_value(s, gl_lam, 0.0);
   s.f0 = f0;
   s.t = 0.0;
   s.ft = f0;
@@ -670,8 +696,8 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
   }
   double t = 1.0;
   for (int j = 1; j <= ls_max && j < NT_A; ++j) {
-    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j] + gl_value(s, gl_lam, t);
-    if (ft <= f0 + c1 * t * s.g0d) {
+    const double ft = ls_value<R>(s, t, s.tv[j]) + gl_value(s, gl_lam, t);
+    if ((R)ft <= (R)f0 + (R)c1 * (R)t * (R)s.g0d) {
       s.t = t; s.ft = ft;
       ls_record(s, hist, max_hist, P, p);
       return;
@@ -683,6 +709,7 @@ __global__ __launch_bounds__(RT) void k_ls_select(ProbState* st, const double* p
 }
 
 // Second pass: the remaining steps t = beta^(NT_A - 1), ..., for problems still pending.
+template <typename R>
 __global__ __launch_bounds__(RT) void k_ls_select_rest(ProbState* st, const double* part_i, int nb_i, double c1,
                                                        double ls_beta, int ls_max, mrh_cg_hist* hist,
                                                        int max_hist, int P, int* active, double gl_lam) {
@@ -690,15 +717,15 @@ __global__ __launch_bounds__(RT) void k_ls_select_rest(ProbState* st, const doub
   ProbState& s = st[p];
   if (!(s.status == MRH_CG_ACTIVE && s.ls_pending)) return;   // uniform over the block
   double vals[NT - NT_A];
-  block_sum_partials_n<NT - NT_A>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals);
+  block_sum_partials_n<NT - NT_A, R>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals);
   if (threadIdx.x != 0) return;
   s.ls_pending = 0;
   for (int j = NT_A; j < NT; ++j) s.tv[j] = vals[j - NT_A];
   double t = 1.0;
   for (int j = 1; j < NT_A; ++j) t *= ls_beta;
   for (int j = NT_A; j <= ls_max && j < NT; ++j) {
-    const double ft = 0.5 * (s.a + 2.0 * t * s.b + t * t * s.c) + s.tv[j] + gl_value(s, gl_lam, t);
-    if (ft <= s.f0 + c1 * t * s.g0d) {
+    const double ft = ls_value<R>(s, t, s.tv[j]) + gl_value(s, gl_lam, t);
+    if ((R)ft <= (R)s.f0 + (R)c1 * (R)t * (R)s.g0d) {
       s.t = t; s.ft = ft;
       ls_record(s, hist, max_hist, P, p);
       return;
@@ -979,12 +1006,13 @@ __global__ __launch_bounds__(RT) void k_accept_k(float2* __restrict__ Ax, const 
 // g += TV gradient at x; partials: TV energy, |g|^2, Re<g_prev, g>, Re<g, d>.  With a diagonal
 // preconditioner D (null: identity) the first two are weighted by 1 / D (graspcg/ops/dot.py:5-71
 // with diag, as the direction rules use them, graspcg/numerics/directions.py:75-162).
+template <typename R>
 __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, float2* __restrict__ g,
                                                 const float2* __restrict__ d, const float2* __restrict__ gp,
                                                 const float* __restrict__ D, TVDev tv, int64_t npix_p,
                                                 const ProbState* __restrict__ st, int prime, double* part) {
   const int p = blockIdx.y;
-  double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  R v[5] = {R(0), R(0), R(0), R(0), R(0)};
   if (prime || st[p].status == MRH_CG_ACTIVE) {
     const int plane = tv.S0 * tv.S1;
     for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
@@ -996,18 +1024,19 @@ __global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, fl
       if (tv.n > 0) gv = cadd(gv, tv_grad_at(x, n, f, ix, iy, tv, e));
       g[n] = gv;
       const float2 xv = x[n];
-      v[4] += (double)xv.x * xv.x + (double)xv.y * xv.y;
-      v[0] += (double)e;
-      const double wd = D ? 1.0 / (double)D[n] : 1.0;
-      v[1] += ((double)gv.x * gv.x + (double)gv.y * gv.y) * wd;
-      if (gp) { const float2 q = gp[n]; v[2] += ((double)q.x * gv.x + (double)q.y * gv.y) * wd; }
-      if (d) { const float2 q = d[n]; v[3] += (double)q.x * gv.x + (double)q.y * gv.y; }
+      v[4] += (R)xv.x * xv.x + (R)xv.y * xv.y;
+      v[0] += (R)e;
+      const R wd = D ? (R)1 / (R)D[n] : (R)1;
+      v[1] += ((R)gv.x * gv.x + (R)gv.y * gv.y) * wd;
+      if (gp) { const float2 q = gp[n]; v[2] += ((R)q.x * gv.x + (R)q.y * gv.y) * wd; }
+      if (d) { const float2 q = d[n]; v[3] += (R)q.x * gv.x + (R)q.y * gv.y; }
     }
   }
-  block_reduce_store<5>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
+  block_reduce_store<5, R>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
 }
 
 // Finish gradient reductions: beta, stop rule, history, iteration count.
+template <typename R>
 __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* part_i, int nb_i, int prime,
                                                 int direction, double rel_tol, double restart_c,
                                                 const double* __restrict__ target, mrh_cg_hist* hist,
@@ -1015,7 +1044,7 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
                                                 int nb_q) {
   const int p = blockIdx.x;
   double vals[5], eq[1] = {0.0};
-  block_sum_partials_n<5>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals);
+  block_sum_partials_n<5, R>(part_i + (int64_t)p * nb_i * PI_STRIDE, nb_i, PI_STRIDE, 0, vals);
   if (part_q) block_sum_partials_n<1>(part_q + (int64_t)p * nb_q * PQ_STRIDE, nb_q, PQ_STRIDE, 0, eq);
   if (threadIdx.x != 0) return;
   ProbState& s = st[p];
@@ -1029,19 +1058,20 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
     if (direction == MRH_CG_DIR_PRPLUS) { s.rho_prev = s.rho; s.have_rho_prev = 1; }
     return;
   }
-  double beta;
+  R beta;
+  const R rho = (R)s.rho, tiny = (R)1e-20;
   if (direction == MRH_CG_DIR_PRPLUS) {
-    beta = (s.rho - s.gpg) / fmax(s.rho_prev, 1e-20);
-    beta = fmax(beta, 0.0);
+    beta = (rho - (R)s.gpg) / fmax((R)s.rho_prev, tiny);
+    beta = fmax(beta, (R)0);
   } else if (direction == MRH_CG_DIR_DY) {
     // directions.py:111-139: Re<d, g> is this pass's gdot_step, Re<d, g_prev> the line search's g0d
-    beta = s.rho / fmax(s.gdot_step - s.g0d, 1e-20);
+    beta = rho / fmax((R)s.gdot_step - (R)s.g0d, tiny);
   } else {
     if (!s.have_rho_prev) { s.rho_prev = s.rho; s.have_rho_prev = 1; }  // first beta = 1 (directions.py:145-149)
-    beta = s.rho / fmax(s.rho_prev, 1e-20);
+    beta = rho / fmax((R)s.rho_prev, tiny);
   }
   // descent safeguard: Re<g, -g + beta d> = -rho + beta Re<g, d>
-  if (restart_c >= 0.0 && -s.rho + beta * s.gdot_step > -restart_c * s.rho) beta = 0.0;
+  if (restart_c >= 0.0 && -rho + beta * (R)s.gdot_step > -(R)restart_c * rho) beta = (R)0;
   s.rho_prev = s.rho;
   s.beta = beta;
   if (s.iter < max_hist) {
@@ -1050,7 +1080,7 @@ __global__ __launch_bounds__(RT) void k_cg_beta(ProbState* st, const double* par
     h.xnorm2 = vals[4]; h.dnorm2 = s.dd;
   }
   s.iter += 1;
-  if (rel_tol > 0.0 && s.have_f_prev && fabs(s.ft - s.f_prev) <= rel_tol * fabs(s.f_prev))
+  if (rel_tol > 0.0 && s.have_f_prev && fabs((R)s.ft - (R)s.f_prev) <= (R)rel_tol * fabs((R)s.f_prev))
     s.status = MRH_CG_REL_TOL;
   if (s.ft <= target[p]) s.status = MRH_CG_TARGET;
   if (s.status != MRH_CG_ACTIVE) active[p] = 0;
@@ -1150,6 +1180,12 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   e->direction = d->direction; e->ls_max_iter = d->ls_max_iter;
   e->c1 = d->c1; e->ls_beta = d->ls_beta; e->rel_tol = d->rel_tol; e->restart_c = d->restart_c;
   e->ls_kind = d->ls_kind; e->c2 = d->c2;
+  e->fp32 = d->fp32_reductions ? 1 : 0;
+  if (e->fp32 && (d->ls_kind != MRH_CG_LS_ARMIJO || d->gl_weight != 0.0)) {
+    delete e;
+    mrh_set_error("FP32 reductions are built for the Armijo search without a graph-Laplacian term");
+    return MRH_EUNSUP;
+  }
   e->d_diag = d->diag;
   if (d->gl_weight != 0.0) {
     if (d->gl_F != e->F || d->gl_F < 1 || d->gl_F > GL_MAXF || !d->gl_L || !d->gl_prof) {
@@ -1248,7 +1284,7 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
                                  const void* Ad, const void* y, void* stream) {
   MRH_REQUIRE(e && x && d && g && Ax && Ad && y, MRH_EINVAL, "cg_linesearch: null argument");
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_ls_kspace, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
+  LAUNCH_R(e, k_ls_kspace, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
                      (const float2*)y, e->d_kprob, e->d_st, e->d_part_k);
   MRH_CHECK_LAUNCH();
   const double gl_lam = e->gl.lam;
@@ -1278,17 +1314,21 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
     }
     return MRH_OK;
   }
-  hipLaunchKernelGGL((k_ls_image<0, NT_A>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+  if (e->fp32) hipLaunchKernelGGL((k_ls_image<0, NT_A, float>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
+  else hipLaunchKernelGGL((k_ls_image<0, NT_A, double>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
                      (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
   MRH_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
+  LAUNCH_R(e, k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
                      e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active,
                      part_q, e->nb_q, gl_lam);
   MRH_CHECK_LAUNCH();
-  hipLaunchKernelGGL((k_ls_image<NT_A, NT>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+  if (e->fp32) hipLaunchKernelGGL((k_ls_image<NT_A, NT, float>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
+  else hipLaunchKernelGGL((k_ls_image<NT_A, NT, double>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
                      (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
   MRH_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_ls_select_rest, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, e->c1,
+  LAUNCH_R(e, k_ls_select_rest, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, e->c1,
                      e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active, gl_lam);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
@@ -1317,11 +1357,11 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
                        e->gl, e->S0 * e->S1, e->npix_p, e->d_st, prime, e->d_part_q);
     MRH_CHECK_LAUNCH();
   }
-  hipLaunchKernelGGL(k_cg_grad, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
+  LAUNCH_R(e, k_cg_grad, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
                      (const float2*)d, (const float2*)g_prev, (const float*)e->d_diag, e->tv, e->npix_p, e->d_st,
                      prime, e->d_part_i);
   MRH_CHECK_LAUNCH();
-  hipLaunchKernelGGL(k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
+  LAUNCH_R(e, k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
                      e->direction, e->rel_tol, e->restart_c, e->d_target, e->d_hist, e->max_hist, e->P,
                      e->d_active, part_q, e->nb_q);
   MRH_CHECK_LAUNCH();

@@ -54,7 +54,7 @@ class NLCGEngine:
                  direction: str = "fr", c1: float = 1e-4, ls_beta: float = 0.5, ls_max_iter: int = 20,
                  rel_tol: float = 0.0, max_hist: int = 256, restart_c: Optional[float] = None,
                  skip_stopped: bool = True, ls_name: str = "armijo", c2: float = 0.9,
-                 precond: bool = False):
+                 precond: bool = False, iters_per_replay: int = 4, fp32_reductions: bool = False):
         if direction not in _DIRECTIONS:
             raise ValueError(f"direction must be one of {sorted(_DIRECTIONS)}")
         if ls_name not in _SEARCHES:
@@ -89,6 +89,9 @@ class NLCGEngine:
         desc.rel_tol, desc.max_hist = float(rel_tol or 0.0), int(max_hist)
         desc.restart_c = -1.0 if restart_c is None else float(restart_c)
         desc.ls_kind, desc.c2 = _SEARCHES[ls_name], float(c2)
+        # all-FP32 inner products / beta / line-search values, as the reference computes them
+        # (cg.py:89-90, ops/dot.py:54-71): a precision-study variant of the FP64 default
+        desc.fp32_reductions = 1 if fp32_reductions else 0
         self.ls_name = ls_name
         # diagonal preconditioner D (the reference workspace's "diag"): the regularisers'
         # majoriser at x0, built by prime() (RegManager.add_diag); None: D = 1
@@ -116,7 +119,15 @@ class NLCGEngine:
         self._active = flags.value
         # the iteration's NUFFTs skip the frames of problems that have stopped
         self.skip_stopped = bool(skip_stopped)
+        if int(iters_per_replay) < 1:
+            raise ValueError("iters_per_replay must be >= 1")
+        # iterations captured in one graph replay; the host reads the problems' status once per
+        # replay.  A problem that stops inside a replay is skipped by every later kernel of it
+        # (status check in the solver kernels, masked NUFFTs), so the results equal one replay
+        # per iteration; without skip_stopped the NUFFTs still run for stopped problems.
+        self.iters_per_replay = int(iters_per_replay)
         self._graph = None
+        self._graph_k = None
         self.n_steps = 0
 
     # ------------------------------------------------------------------ steps
@@ -166,25 +177,43 @@ class NLCGEngine:
             self.plan.forward(self.d, self.Ad)
 
     def capture(self) -> None:
-        """Capture :meth:`step` in a HIP graph (replayed by :meth:`run`)."""
+        """Capture :meth:`step` in a HIP graph, and ``iters_per_replay`` steps in a second one
+        (replayed by :meth:`run`)."""
         g = torch.cuda.CUDAGraph()
         with torch.cuda.graph(g):
             self.step()
         self._graph = g
+        self._graph_k = None
+        if self.iters_per_replay > 1:
+            gk = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(gk):
+                for _ in range(self.iters_per_replay):
+                    self.step()
+            self._graph_k = gk
 
     def run(self, max_iter: int, *, use_graph: bool = True, check_every: int = 1) -> int:
-        """Iterate until every problem stopped or ``max_iter`` steps; returns the steps run."""
+        """Iterate until every problem stopped or ``max_iter`` steps; returns the steps launched.
+        With the graph, ``iters_per_replay`` steps go per replay while at least that many remain.
+        The status is read once ``check_every`` steps have passed since the last read (so at most
+        once per replay; 0 = never)."""
         if use_graph and self._graph is None:
             self.capture()
         k = 0
+        last_check = 0
         while k < max_iter:
-            if use_graph:
+            if use_graph and self._graph_k is not None and max_iter - k >= self.iters_per_replay:
+                self._graph_k.replay()
+                n = self.iters_per_replay
+            elif use_graph:
                 self._graph.replay()
+                n = 1
             else:
                 self.step()
-            k += 1
-            self.n_steps += 1
-            if check_every and (k % check_every == 0 or k == max_iter):
+                n = 1
+            k += n
+            self.n_steps += n
+            if check_every and (k - last_check >= check_every or k == max_iter):
+                last_check = k
                 if not self.active().any():
                     break
         return k

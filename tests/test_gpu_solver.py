@@ -511,19 +511,23 @@ def test_skipping_stopped_problems_changes_nothing():
     be.plan.adjoint(y, x0)
     regm = RegManager([TVND("t", TVParams(weight=0.02, axes="temporal"))])
     out = []
-    for skip in (False, True):
+    # (skip stopped slices, iterations per graph replay): one replay per iteration with and without
+    # the masked NUFFTs, and 4 or 7 iterations per replay (slices stop inside a replay)
+    for skip, ipr in ((False, 1), (True, 1), (True, 4), (True, 7), (False, 4)):
         eng = NLCGEngine(be.plan, y, problems=3, frames=F, regm=regm, direction="prplus", rel_tol=1e-3,
-                         restart_c=0.0, skip_stopped=skip, max_hist=80)
+                         restart_c=0.0, skip_stopped=skip, max_hist=80, iters_per_replay=ipr)
         eng.prime(x0)
         eng.run(60)
         torch.cuda.synchronize()
         out.append((eng.history(), eng.x.cpu().numpy(), eng.status()[1]))
-    (h0, x0n, it0), (h1, x1n, it1) = out
-    assert list(it0) == list(it1) and len(set(it0.tolist())) > 1, it0     # slices stop at different iterations
-    for p in range(3):
-        assert [r["step"] for r in h0[p]] == [r["step"] for r in h1[p]]
-        assert [r["f_step"] for r in h0[p]] == [r["f_step"] for r in h1[p]]
-    assert np.array_equal(x0n, x1n)
+    h0, x0n, it0 = out[0]
+    assert len(set(it0.tolist())) > 1, it0                                 # slices stop at different iterations
+    for h1, x1n, it1 in out[1:]:
+        assert list(it0) == list(it1)
+        for p in range(3):
+            assert [r["step"] for r in h0[p]] == [r["step"] for r in h1[p]]
+            assert [r["f_step"] for r in h0[p]] == [r["f_step"] for r in h1[p]]
+        assert np.array_equal(x0n, x1n)
 
 
 def test_solver_option_limits_are_reported():
@@ -568,3 +572,30 @@ def test_latest_record_read_equals_the_history_tail():
         h = hist[p][-1]
         for k in ("gnorm", "xnorm", "stepnorm", "f", "f_step", "step"):
             assert last[p][k] == h[k], (p, k)
+
+
+def test_fp32_reduction_variant():
+    """The all-FP32 reduction variant (the reference's precision) on c1: it runs the same loop
+    (20 FR iterations), its objective stays within 1e-3 of the FP64 engine's, and its reductions
+    are really FP32 (the FP64 and FP32 histories differ)."""
+    wl = synth.config_c1()
+    be, ora = prepare(wl)
+    regm, oregs = regs_for(wl)
+    x0 = c64(ora.adjoint(wl.y))
+    fs = []
+    for fp32 in (False, True):
+        eng = NLCGEngine(be.plan, c64(wl.y), problems=1, frames=1, regm=regm, fp32_reductions=fp32)
+        eng.prime(x0)
+        eng.run(20)
+        torch.cuda.synchronize()
+        h = eng.history()[0]
+        assert len(h) == 20
+        f = objective_value(ora.forward, wl.y, eng.x.cpu().numpy().astype(np.complex128), oregs)
+        fs.append((f, [r["f"] for r in h]))
+        eng.close()
+    (f64, h64), (f32, h32) = fs
+    print(f"[c1] f after 20 iterations: FP64 {f64:.8e}, FP32 {f32:.8e}, rel gap {(f32 - f64) / f64:.2e}")
+    assert abs(f32 - f64) <= 1e-3 * abs(f64)
+    assert h64 != h32
+    with pytest.raises(NotImplementedError):
+        NLCGEngine(be.plan, c64(wl.y), problems=1, frames=1, regm=regm, fp32_reductions=True, ls_name="wolfe")
