@@ -59,6 +59,20 @@ __device__ __forceinline__ float kb_weight(const KBPoly& kb, float d) {
 #define STRIP_TC 8                     // default grid columns per strip workgroup (4 and 16 also built)
 #define STRIP_EPT 1                    // sample footprints cached in registers per thread
 
+// The run of footprint columns my ... my + W - 1 (mod G1) that falls in the strip
+// [c0, c0 + TC): first footprint column jlo, first strip column col, length ncol
+// (ncol <= 0: the footprint misses the strip).
+__host__ __device__ __forceinline__ void strip_run(int my, int W, int c0, int TC, int G1, int& jlo, int& col,
+                                                   int& ncol) {
+  int rel = my - c0;
+  if (rel < 0) rel += G1;
+  if (rel < TC) {
+    jlo = 0; col = rel; ncol = W < TC - rel ? W : TC - rel;
+  } else {
+    jlo = G1 - rel; col = 0; ncol = W - jlo < TC ? W - jlo : TC;
+  }
+}
+
 // ---------------------------------------------------------------- plan
 struct mrh_nufft_plan {
   int S0, S1, G0, G1, W, C, B, like_max, Tc, nstrips, np;
@@ -410,10 +424,14 @@ __global__ __launch_bounds__(256) void k_fwd_rows(
 // STRIP_EPT of the strip's samples once and keeps it in registers, so the
 // weights are shared by all coils; samples beyond that spill to an
 // evaluate-on-the-fly path.
+//
+// A footprint's columns inside the strip are a contiguous run: ncol columns
+// starting at strip column col, footprint columns jlo ... jlo + ncol - 1; only
+// their weights are kept.
 template <int WM>
 struct Foot {
-  float wx[WM], wy[WM];  // wy[j] = 0 when column (my + j) mod G1 lies outside the strip
-  int r0, my, k;         // first footprint row / column (mod G), sample index in the frame (-1: empty)
+  float wx[WM], wy[WM];  // wy[jj]: weight of footprint column jlo + jj (jj < ncol), else 0
+  int r0, col, ncol, k;  // first footprint row (mod G), first strip column, columns inside, sample (-1: empty)
   float wA, wAH;
 };
 
@@ -426,27 +444,23 @@ template <int WM, int TC>
 __device__ __forceinline__ void load_foot(Foot<WM>& F, int2 m, float4 f, int W, int c0, int G1,
                                           const KBPoly& kb) {
   F.r0 = m.x & 0xffff;
-  F.my = (int)((unsigned)m.x >> 16);
+  const int my = (int)((unsigned)m.x >> 16);
   F.k = m.y;
   F.wA = f.z;
   F.wAH = f.w;
+  int jlo;
+  strip_run(my, W, c0, TC, G1, jlo, F.col, F.ncol);
 #pragma unroll
   for (int i = 0; i < WM; ++i) F.wx[i] = (i < W) ? kb_weight<kb_terms(WM)>(kb, f.x - (float)i) : 0.f;
 #pragma unroll
-  for (int j = 0; j < WM; ++j) {
-    int cc = F.my + j;
-    if (cc >= G1) cc -= G1;
-    F.wy[j] = (j < W && (unsigned)(cc - c0) < (unsigned)TC) ? kb_weight<kb_terms(WM)>(kb, f.y - (float)j) : 0.f;
-  }
+  for (int jj = 0; jj < WM; ++jj)
+    F.wy[jj] = (jj < F.ncol) ? kb_weight<kb_terms(WM)>(kb, f.y - (float)(jlo + jj)) : 0.f;
 }
 
-// Forward: sum of the footprint over the LDS strip (column-major, padded rows).
-// Columns outside the strip (weight 0) are skipped, which is nearly uniform
-// over a wavefront (its lanes hold consecutive samples of one spoke); the W
-// reads of a column are issued together.
+// Forward: sum of the footprint over the LDS strip (column-major, padded rows),
+// column by column in footprint order; the W reads of a column are issued together.
 template <int WM, int CS>
-__device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* smem, int W, int c0, int G0,
-                                             int G1) {
+__device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* smem, int W, int G0) {
   int ro[WM];
 #pragma unroll
   for (int i = 0; i < WM; ++i) {
@@ -456,11 +470,9 @@ __device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* sm
   }
   float2 acc = make_float2(0.f, 0.f);
 #pragma unroll
-  for (int j = 0; j < WM; ++j) {
-    if (F.wy[j] != 0.f) {
-      int cc = F.my + j;
-      if (cc >= G1) cc -= G1;
-      const float2* colp = smem + (cc - c0) * CS;
+  for (int jj = 0; jj < WM; ++jj) {
+    if (jj < F.ncol) {
+      const float2* colp = smem + (F.col + jj) * CS;
       float2 g[WM];
 #pragma unroll
       for (int i = 0; i < WM; ++i) g[i] = colp[ro[i]];
@@ -472,8 +484,8 @@ __device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* sm
           rs.y = fmaf(g[i].y, F.wx[i], rs.y);
         }
       }
-      acc.x = fmaf(rs.x, F.wy[j], acc.x);
-      acc.y = fmaf(rs.y, F.wy[j], acc.y);
+      acc.x = fmaf(rs.x, F.wy[jj], acc.x);
+      acc.y = fmaf(rs.y, F.wy[jj], acc.y);
     }
   }
   return acc;
@@ -549,7 +561,7 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
 #pragma unroll
     for (int q = 0; q < STRIP_EPT; ++q) {
       if (F[q].k >= 0) {
-        const float2 a = interp_one<WM, CS>(F[q], smem, W, c0, G0, G1);
+        const float2 a = interp_one<WM, CS>(F[q], smem, W, G0);
         unsafeAtomicAdd(&yc[F[q].k].x, a.x * F[q].wA);
         unsafeAtomicAdd(&yc[F[q].k].y, a.y * F[q].wA);
       }
@@ -557,7 +569,7 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
     for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
       Foot<WM> Fo;
       load_foot<WM, TC>(Fo, em[e], ef[e], W, c0, G1, kb);
-      const float2 a = interp_one<WM, CS>(Fo, smem, W, c0, G0, G1);
+      const float2 a = interp_one<WM, CS>(Fo, smem, W, G0);
       unsafeAtomicAdd(&yc[Fo.k].x, a.x * Fo.wA);
       unsafeAtomicAdd(&yc[Fo.k].y, a.y * Fo.wA);
     }
@@ -623,10 +635,39 @@ __device__ __forceinline__ void fft_wave_from_fixed(float2* buf, const unsigned 
   FFTStages<N, R, true, 64, true>::run(buf, tw, lane);
 }
 
+// Adjoint footprint: the weights of all W columns, zero for the columns
+// outside the strip (the spread runs column j only where its weight is
+// non-zero).  Measured on the GRASP workload, this form spreads 3 % faster than
+// the forward's compact run of columns, which in turn interpolates 16 % faster.
+template <int WM>
+struct FootA {
+  float wx[WM], wy[WM];  // wy[j] = 0 when column (my + j) mod G1 lies outside the strip
+  int r0, my, k;         // first footprint row / column (mod G), sample index in the frame (-1: empty)
+  float wA, wAH;
+};
+
+template <int WM, int TC>
+__device__ __forceinline__ void load_foot_adj(FootA<WM>& F, int2 m, float4 f, int W, int c0, int G1,
+                                              const KBPoly& kb) {
+  F.r0 = m.x & 0xffff;
+  F.my = (int)((unsigned)m.x >> 16);
+  F.k = m.y;
+  F.wA = f.z;
+  F.wAH = f.w;
+#pragma unroll
+  for (int i = 0; i < WM; ++i) F.wx[i] = (i < W) ? kb_weight<kb_terms(WM)>(kb, f.x - (float)i) : 0.f;
+#pragma unroll
+  for (int j = 0; j < WM; ++j) {
+    int cc = F.my + j;
+    if (cc >= G1) cc -= G1;
+    F.wy[j] = (j < W && (unsigned)(cc - c0) < (unsigned)TC) ? kb_weight<kb_terms(WM)>(kb, f.y - (float)j) : 0.f;
+  }
+}
+
 // vs: the weighted sample already scaled by 2^sc.  Column j of the strip owns
 // words [2 j CS, 2 j CS + 2 CS) of acc, row rr's (re, im) at 2 PIDX(rr) + {0, 1}.
 template <int WM, int CS>
-__device__ __forceinline__ void spread_one(const Foot<WM>& F, float2 vs, unsigned long long* acc, int W, int c0,
+__device__ __forceinline__ void spread_one(const FootA<WM>& F, float2 vs, unsigned long long* acc, int W, int c0,
                                            int G0, int G1) {
 #pragma unroll
   for (int j = 0; j < WM; ++j) {
@@ -674,11 +715,11 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
   const int64_t Kb = koff[b + 1] - koff[b];
   const float2* ybase = y + (int64_t)C * (koff[b] * like + (int64_t)l * Kb);
   const int c0 = s * TC;
-  Foot<WM> F[STRIP_EPT];
+  FootA<WM> F[STRIP_EPT];
 #pragma unroll
   for (int q = 0; q < STRIP_EPT; ++q) {
     const int e = e0 + tid + q * NTH;
-    if (e < e1) load_foot<WM, TC>(F[q], em[e], ef[e], W, c0, G1, kb);
+    if (e < e1) load_foot_adj<WM, TC>(F[q], em[e], ef[e], W, c0, G1, kb);
     else F[q].k = -1;
   }
   const int h0 = S0 / 2;
@@ -728,8 +769,8 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
         if (F[q].k >= 0)
           spread_one<WM, CS>(F[q], make_float2(ldexpf(v[q].x, sc), ldexpf(v[q].y, sc)), acc, W, c0, G0, G1);
       for (int e = e0 + STRIP_EPT * NTH + tid; e < e1; e += NTH) {
-        Foot<WM> Fo;
-        load_foot<WM, TC>(Fo, em[e], ef[e], W, c0, G1, kb);
+        FootA<WM> Fo;
+        load_foot_adj<WM, TC>(Fo, em[e], ef[e], W, c0, G1, kb);
         const float2 vo = cscale(yc[Fo.k], Fo.wAH);
         spread_one<WM, CS>(Fo, make_float2(ldexpf(vo.x, sc), ldexpf(vo.y, sc)), acc, W, c0, G0, G1);
       }
