@@ -121,12 +121,18 @@ __device__ __forceinline__ void fft_sync() {
   }
 }
 
-// One Stockham pass.  `tw` holds W_N^k = exp(-2 pi i k / N), k < N (LDS).
+// Twiddle table in pass order (N - 1 entries): the pass with accumulated
+// length NS and radix R finds W_N^{m (N / (NS R)) r}, r in [1, R), m in [0, NS),
+// at (NS - 1) + (r - 1) NS + m.  Consecutive butterflies (m = j mod NS) read
+// consecutive entries, so a wavefront's twiddle loads are free of LDS bank
+// conflicts (a table indexed by the exponent is read at a stride of r).
+__host__ __device__ constexpr int fft_tw_index(int ns, int r, int m) { return (ns - 1) + (r - 1) * ns + m; }
+
+// One Stockham pass.  `tw` holds the pass-ordered twiddle table (LDS).
 template <int N, int R, int NS, bool INV, int NTHR, bool WAVE = false>
 __device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int t) {
   constexpr int NBF = N / R;
   constexpr int NB = (NBF + NTHR - 1) / NTHR;
-  constexpr int TSTEP = N / (NS * R);
   float2 v[NB][R];
 #pragma unroll
   for (int q = 0; q < NB; ++q) {
@@ -135,10 +141,10 @@ __device__ __forceinline__ void fft_pass(float2* buf, const float2* tw, int t) {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q][r] = buf[PIDX(j + r * NBF)];
       if (NS > 1) {
-        const int step = (j % NS) * TSTEP;
+        const int m = j % NS;
 #pragma unroll
         for (int r = 1; r < R; ++r) {
-          float2 w = tw[step * r];
+          float2 w = tw[fft_tw_index(NS, r, m)];
           if (INV) w.y = -w.y;
           v[q][r] = cmul(v[q][r], w);
         }

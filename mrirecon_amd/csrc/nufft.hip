@@ -141,11 +141,18 @@ static int upload(T** dst, const T* src, size_t n) {
   return MRH_OK;
 }
 
+// Pass-ordered twiddles of a length-n transform (the layout fft_pass reads,
+// fft_tw_index); n entries, the last one unused.
 static void twiddles(std::vector<float2>& tw, int n) {
-  tw.resize(n);
-  for (int k = 0; k < n; ++k) {
-    double a = -2.0 * M_PI * (double)k / (double)n;
-    tw[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+  tw.assign(n, make_float2(1.f, 0.f));
+  for (int ns = 1; ns < n;) {
+    const int R = fft_next_radix(n / ns), tstep = n / (ns * R);
+    for (int r = 1; r < R; ++r)
+      for (int m = 0; m < ns; ++m) {
+        const double a = -2.0 * M_PI * (double)(m * tstep * r) / (double)n;
+        tw[fft_tw_index(ns, r, m)] = make_float2((float)std::cos(a), (float)std::sin(a));
+      }
+    ns *= R;
   }
 }
 
