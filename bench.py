@@ -393,8 +393,10 @@ def nufft_rate(wl, dev, reps=5):
 
     from mrirecon_amd.nufft.backends.hip import HipNUFFTAdapter
     be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=wl.W, osf=wl.osf)
+    t0 = time.perf_counter()
     be.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(wl.traj), torch.from_numpy(wl.dcf),
                torch.from_numpy(wl.maps).to(dev), device=dev, k_offsets=wl.k_offsets)
+    setup = plan_setup(be, time.perf_counter() - t0)
     plan = be.plan
     x = torch.from_numpy(wl.x_true).to(dev)
     y = torch.empty(plan.n_k, dtype=torch.complex64, device=dev)
@@ -419,7 +421,20 @@ def nufft_rate(wl, dev, reps=5):
                                           for w, t in plan.tuning["ms_per_pair_by_width"].items()}
                                          if plan.tuning and "ms_per_pair_by_width" in plan.tuning else
                                          {str(k): round(v, 4) for k, v in plan.tuning["ms_per_pair"].items()}
-                                         if plan.tuning else None)}
+                                         if plan.tuning else None),
+                "plan_setup": setup}
+
+
+def plan_setup(be, seconds):
+    """Host-side plan cost of an adapter: prepare() wall time, and per strip width the C-ABI plan
+    creation (strip binning on host threads + uploads) and the coil-chunk autotune."""
+    plan = be.plan
+    t = plan.tuning or {}
+    info = plan.info()
+    return {"prepare_ms": round(seconds * 1e3, 1), "samples": int(info["n_samples"]),
+            "footprint_entries": int(info["n_entries"]),
+            "plan_create_ms_by_width": t.get("plan_create_ms_by_width", {int(info["strip_cols"]): round(plan.create_ms, 2)}),
+            "autotune_ms_by_width": t.get("autotune_ms_by_width")}
 
 
 def nufft3_rate(dev, S=128, C=8, spokes=1024, R=256, reps=5):
@@ -461,8 +476,19 @@ def nufft3_rate(dev, S=128, C=8, spokes=1024, R=256, reps=5):
         ts.append(e0.elapsed_time(e1))
     ms = float(np.median(ts))
     be.close()
-    return {"volume": f"{S}^3", "coils": C, "samples": int(traj.shape[1]), "ms_per_pair": round(ms, 3),
-            "coil_volume_pairs_per_s": round(C / (ms * 1e-3), 1)}
+    # algorithmic bytes of this grid-in-HBM pipeline, every kernel's inputs and outputs once:
+    # forward = volume + maps + pad write + 3 line-FFT passes (read + write) + interpolation read
+    # + trajectory + y; the adjoint mirrors it (y, spread write, 3 passes, crop read, maps, volume)
+    G = 2 * S
+    grid = 8 * C * G ** 3
+    K = int(traj.shape[1])
+    one_way = 8 * S ** 3 * (1 + C) + grid * (1 + 6 + 1) + 12 * K + 8 * C * K
+    bytes_pair = 2 * one_way
+    return {"volume": f"{S}^3", "coils": C, "samples": K, "ms_per_pair": round(ms, 3),
+            "coil_volume_pairs_per_s": round(C / (ms * 1e-3), 1),
+            "grid_model_bytes_per_pair": int(bytes_pair),
+            "grid_model_GBps": round(bytes_pair / (ms * 1e-3) / 1e9, 1),
+            "grid_model_frac": round(bytes_pair / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
 
 
 def tbasis_rate(dev, T=34, K=8, S=256, vols=1, reps=20):
@@ -702,8 +728,10 @@ def main():
     print("bench: generating the GRASP workload", file=sys.stderr, flush=True)
     wl = synth.config_c2()
     be = HipNUFFTAdapter(ndim=2, traj_units="rad", kb_width=wl.W, osf=wl.osf, strip_cols=args.strip_cols)
+    t0 = time.perf_counter()
     be.prepare((wl.n_coils, wl.S, wl.S), torch.from_numpy(wl.traj), torch.from_numpy(wl.dcf),
                torch.from_numpy(wl.maps).to(dev), device=dev, k_offsets=wl.k_offsets)
+    setup_c2 = plan_setup(be, time.perf_counter() - t0)
     plan = be.plan
     info = plan.info()
     x = torch.from_numpy(wl.x_true).to(dev)
@@ -784,7 +812,8 @@ def main():
         c1_gpu = gpu_c1_time_to_tol(dev)
         cpu = cpu_baseline(c1_gpu, ttt["plain_iterations_median"] if ttt else None)
 
-    detail = {"roofline": roofline, "kernels": per_kernel, "pipeline": pipeline, "time_to_tol": ttt, "unet": unet,
+    detail = {"roofline": roofline, "kernels": per_kernel, "pipeline": pipeline, "plan_setup": setup_c2,
+              "time_to_tol": ttt, "unet": unet,
               "nufft_configs": configs, "c3_precision": c3p, "batched_uneven": c4, "solver_variants": variants,
               "temporal_basis": tbasis, "cpu_baseline": cpu}
     try:
@@ -822,6 +851,8 @@ def main():
         "unet_mfma_frac": unet["mfma_frac"] if unet else None,
         "batched_uneven_speedup": c4["speedup"] if c4 else None,
         "c3_fp32_vs_fp64_objective_gap": c3p["nlcg"]["rel_gap_fp32_minus_fp64"] if c3p else None,
+        "plan_setup_ms": {"grasp": setup_c2["prepare_ms"],
+                          "batched48": c4["nufft"]["plan_setup"]["prepare_ms"] if c4 else None},
         "detail": args.detail,
     }
     print(json.dumps(line), flush=True)

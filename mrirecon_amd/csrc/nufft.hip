@@ -32,7 +32,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 // ---------------------------------------------------------------- KB window
@@ -156,6 +158,24 @@ static void twiddles(std::vector<float2>& tw, int n) {
   }
 }
 
+// Runs fn(b) for every frame b, on up to 16 host threads when the plan is large
+// (frames interleaved over the threads); fn must only touch frame b's data.
+template <typename Fn>
+static void for_each_frame(int B, int64_t nsamples, Fn fn) {
+  int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  if (nsamples < (1 << 18) || B < 2) nt = 1;
+  nt = std::min(nt, B);
+  if (nt == 1) {
+    for (int b = 0; b < B; ++b) fn(b);
+    return;
+  }
+  std::vector<std::thread> th;
+  th.reserve(nt);
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&, t] { for (int b = t; b < B; b += nt) fn(b); });
+  for (auto& x : th) x.join();
+}
+
 extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** out) {
   MRH_REQUIRE(d && out, MRH_EINVAL, "plan_create: null argument");
   *out = nullptr;
@@ -217,30 +237,35 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   p->Ktot = koff[B];
   if (p->Ktot >= (int64_t)1 << 31) { free_plan(p); mrh_set_error("too many samples for 32-bit entry indices"); return MRH_EUNSUP; }
 
-  // footprints and strip binning (stable counting sort by (frame, strip))
+  // footprints and strip binning: a stable counting sort by (frame, strip), two
+  // passes over the samples, frames spread over host threads (each frame's
+  // entries fill their own range, in sample order)
   const int W = d->W, G0 = d->G0, G1 = d->G1, ns = p->nstrips;
   const double sc0 = G0 / (2.0 * M_PI), sc1 = G1 / (2.0 * M_PI);
-  std::vector<int> cnt((size_t)B * ns + 1, 0);
-  struct Foot { int mx, my, s0, s1; float ux, uy; };
-  std::vector<Foot> foot((size_t)p->Ktot);
-  for (int b = 0; b < B; ++b) {
+  struct FootH { int mx, my, s0, s1; float ux, uy; };
+  auto footprint = [&](int b, int64_t k) {
     const int64_t Kb = koff[b + 1] - koff[b];
     const float* t0 = d->traj + 2 * koff[b];
-    const float* t1 = t0 + Kb;
-    for (int64_t k = 0; k < Kb; ++k) {
-      double u = (double)t0[k] * sc0, v = (double)t1[k] * sc1;
-      double fu = std::floor(u - 0.5 * W) + 1.0, fv = std::floor(v - 0.5 * W) + 1.0;
-      int64_t mx = ((int64_t)fu % G0 + G0) % G0, my = ((int64_t)fv % G1 + G1) % G1;
-      Foot f;
-      f.mx = (int)mx; f.my = (int)my;
-      f.ux = (float)(u - fu); f.uy = (float)(v - fv);
-      f.s0 = (int)(my / Tc);
-      f.s1 = (int)(((my + W - 1) % G1) / Tc);
-      foot[koff[b] + k] = f;
-      cnt[(size_t)b * ns + f.s0]++;
-      if (f.s1 != f.s0) cnt[(size_t)b * ns + f.s1]++;
+    const double u = (double)t0[k] * sc0, v = (double)t0[Kb + k] * sc1;
+    const double fu = std::floor(u - 0.5 * W) + 1.0, fv = std::floor(v - 0.5 * W) + 1.0;
+    FootH f;
+    f.mx = (int)((((int64_t)fu % G0) + G0) % G0);
+    f.my = (int)((((int64_t)fv % G1) + G1) % G1);
+    f.ux = (float)(u - fu);
+    f.uy = (float)(v - fv);
+    f.s0 = f.my / Tc;
+    f.s1 = ((f.my + W - 1) % G1) / Tc;
+    return f;
+  };
+  std::vector<int> cnt((size_t)B * ns, 0);
+  for_each_frame(B, p->Ktot, [&](int b) {
+    int* cb = cnt.data() + (size_t)b * ns;
+    for (int64_t k = 0, Kb = koff[b + 1] - koff[b]; k < Kb; ++k) {
+      const FootH f = footprint(b, k);
+      cb[f.s0]++;
+      if (f.s1 != f.s0) cb[f.s1]++;
     }
-  }
+  });
   std::vector<int> soff((size_t)B * ns + 1, 0);
   int64_t acc = 0;
   int maxe = 0;
@@ -252,13 +277,12 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   soff[(size_t)B * ns] = (int)acc;
   p->n_entries = acc;
   p->max_strip_entries = maxe;
-  std::vector<int2> em((size_t)acc);
-  std::vector<float4> ef((size_t)acc);
-  std::vector<int> cur(soff.begin(), soff.end() - 1);
-  for (int b = 0; b < B; ++b) {
-    const int64_t Kb = koff[b + 1] - koff[b];
-    for (int64_t k = 0; k < Kb; ++k) {
-      const Foot& f = foot[koff[b] + k];
+  std::unique_ptr<int2[]> em(new int2[(size_t)acc]);
+  std::unique_ptr<float4[]> ef(new float4[(size_t)acc]);
+  for_each_frame(B, p->Ktot, [&](int b) {
+    std::vector<int> cur(soff.begin() + (size_t)b * ns, soff.begin() + (size_t)(b + 1) * ns);
+    for (int64_t k = 0, Kb = koff[b + 1] - koff[b]; k < Kb; ++k) {
+      const FootH f = footprint(b, k);
       float dval = 1.f;
       if (d->dcf) dval = d->dcf[koff[b] + k];
       float wA = 1.f, wAH = 1.f;
@@ -268,16 +292,16 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
         wA = d->dcf_in_fwd ? dval : 1.f;
         wAH = d->dcf_in_adj ? dval : 1.f;
       }
-      int2 m = make_int2(f.mx | (f.my << 16), (int)k);
-      float4 fv = make_float4(f.ux, f.uy, wA, wAH);
-      int e = cur[(size_t)b * ns + f.s0]++;
+      const int2 m = make_int2(f.mx | (f.my << 16), (int)k);
+      const float4 fv = make_float4(f.ux, f.uy, wA, wAH);
+      int e = cur[f.s0]++;
       em[e] = m; ef[e] = fv;
       if (f.s1 != f.s0) {
-        e = cur[(size_t)b * ns + f.s1]++;
+        e = cur[f.s1]++;
         em[e] = m; ef[e] = fv;
       }
     }
-  }
+  });
 
   // deapodisation 1 / Phi((i - S/2) / G)
   std::vector<float> dx(d->S0), dy(d->S1);
@@ -290,8 +314,8 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   int rc;
 #define UP(dst, src, n) if ((rc = upload(&(dst), (src), (n))) != MRH_OK) { free_plan(p); return rc; }
   UP(p->d_koff, koff.data(), koff.size());
-  UP(p->d_em, em.data(), em.size());
-  UP(p->d_ef, ef.data(), ef.size());
+  UP(p->d_em, em.get(), (size_t)acc);
+  UP(p->d_ef, ef.get(), (size_t)acc);
   UP(p->d_soff, soff.data(), soff.size());
   UP(p->d_dx, dx.data(), dx.size());
   UP(p->d_dy, dy.data(), dy.size());

@@ -26,6 +26,7 @@ from __future__ import annotations
 
 import ctypes
 import math
+import time
 from dataclasses import dataclass, field
 from typing import Dict, Literal, Optional, Sequence, Tuple
 
@@ -68,8 +69,10 @@ class NufftPlan:
         d.maps = maps.data_ptr()
         d.strip_cols = int(strip_cols)
         h = ctypes.c_void_p()
+        t0 = time.perf_counter()
         with torch.cuda.device(maps.device):
             _lib.check(lib.mrh_nufft_plan_create(ctypes.byref(d), ctypes.byref(h)))
+        self.create_ms = (time.perf_counter() - t0) * 1e3     # host binning + uploads
         self.handle = h
         self.tuning = None
 
@@ -128,6 +131,7 @@ class NufftPlan:
         (one warm-up pair, then the median of ``reps`` pairs); the fastest is kept and
         ``self.tuning`` records every time (ms per pair).  Results do not depend on the
         choice."""
+        t_start = time.perf_counter()
         S0, S1 = self.S
         dev = self.maps.device
         x = torch.ones((self.B * like, S0, S1), dtype=torch.complex64, device=dev)
@@ -151,7 +155,8 @@ class NufftPlan:
             times[c] = float(sorted(ts)[len(ts) // 2])
         best = min(times, key=times.get)
         self.set_coil_chunk(best)
-        self.tuning = {"coil_chunk": best, "ms_per_pair": times}
+        self.tuning = {"coil_chunk": best, "ms_per_pair": times,
+                       "autotune_ms": (time.perf_counter() - t_start) * 1e3}
         return times
 
     def close(self) -> None:
@@ -362,7 +367,8 @@ class HipNUFFTAdapter:
         else:
             G0 = _grid_size(self._im_shape[1], self.osf)
             widths = [8] + ([16] if G0 <= 512 else []) + ([4] if self.kb_width == 4 else [])
-            best, times = None, {}
+            best, times, create_ms, tune_ms = None, {}, {}, {}
+            t_setup = time.perf_counter()
             for tc in widths:
                 try:
                     q = self._new_plan(like, tc)
@@ -372,6 +378,8 @@ class HipNUFFTAdapter:
                     continue                      # a width this grid cannot use (G1 % tc, KB terms)
                 t = q.autotune(like=like)
                 times[tc] = t
+                create_ms[tc] = round(q.create_ms, 2)
+                tune_ms[tc] = round(q.tuning["autotune_ms"], 2)
                 if best is None or min(t.values()) < min(best.tuning["ms_per_pair"].values()):
                     if best is not None:
                         best.close()
@@ -381,7 +389,9 @@ class HipNUFFTAdapter:
             p = best
             p.tuning = {"strip_cols": int(p.info()["strip_cols"]), "coil_chunk": p.tuning["coil_chunk"],
                         "ms_per_pair": p.tuning["ms_per_pair"],
-                        "ms_per_pair_by_width": {tc: dict(t) for tc, t in times.items()}}
+                        "ms_per_pair_by_width": {tc: dict(t) for tc, t in times.items()},
+                        "plan_create_ms_by_width": create_ms, "autotune_ms_by_width": tune_ms,
+                        "setup_ms": (time.perf_counter() - t_setup) * 1e3}
         self._plans[like] = p
         return p
 
