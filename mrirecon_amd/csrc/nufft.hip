@@ -630,12 +630,12 @@ __device__ __forceinline__ float from_fixed(unsigned long long u, int sc) {
   return (float)ldexp((double)q, -sc);
 }
 
-// Inverse length-N FFT of one column held as fixed-point (re, im) pairs at words
-// 2 PIDX(row), 2 PIDX(row) + 1 of acc, by one wavefront: the first Stockham pass
-// converts while it loads, writes its floats to buf[PIDX(.)] (buf aliases the
-// first half of acc) and the remaining passes run in place.  The result equals
-// converting every row first and calling fft_lds_wave.
-template <int N>
+// Inverse length-N FFT of one column held as fixed-point planes (re at word
+// PIDX(row) of acc, im at word CS + PIDX(row)), by one wavefront: the first
+// Stockham pass converts while it loads, writes its floats to buf[PIDX(.)] (buf
+// aliases the re plane) and the remaining passes run in place.  The result
+// equals converting every row first and calling fft_lds_wave.
+template <int N, int CS>
 __device__ __forceinline__ void fft_wave_from_fixed(float2* buf, const unsigned long long* acc, const float2* tw,
                                                     int lane, int sc) {
   constexpr int R = fft_next_radix(N), NBF = N / R, NB = (NBF + 63) / 64;
@@ -647,8 +647,8 @@ __device__ __forceinline__ void fft_wave_from_fixed(float2* buf, const unsigned 
     if (NBF % 64 == 0 || j < NBF) {
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const int w = 2 * PIDX(j + r * NBF);
-        v[q][r] = make_float2(from_fixed(acc[w], sc), from_fixed(acc[w + 1], sc));
+        const int w = PIDX(j + r * NBF);
+        v[q][r] = make_float2(from_fixed(acc[w], sc), from_fixed(acc[CS + w], sc));
       }
       DFT<R, true>::run(v[q]);
     }
@@ -696,7 +696,9 @@ __device__ __forceinline__ void load_foot_adj(FootA<WM>& F, int2 m, float4 f, in
 }
 
 // vs: the weighted sample already scaled by 2^sc.  Column j of the strip owns
-// words [2 j CS, 2 j CS + 2 CS) of acc, row rr's (re, im) at 2 PIDX(rr) + {0, 1}.
+// words [2 j CS, 2 j CS + 2 CS) of acc: row rr's re at PIDX(rr), its im at CS +
+// PIDX(rr) (two planes: lanes on consecutive rows hit consecutive 8-byte words,
+// half the bank conflicts of interleaved (re, im) pairs).
 template <int WM, int CS>
 __device__ __forceinline__ void spread_one(const FootA<WM>& F, float2 vs, unsigned long long* acc, int W, int c0,
                                            int G0, int G1) {
@@ -712,9 +714,9 @@ __device__ __forceinline__ void spread_one(const FootA<WM>& F, float2 vs, unsign
         if (i < W) {
           int rr = F.r0 + i;
           if (rr >= G0) rr -= G0;
-          unsigned long long* dst = colp + 2 * PIDX(rr);
+          unsigned long long* dst = colp + PIDX(rr);
           atomicAdd(dst, fixed_of(vj.x * F.wx[i]));
-          atomicAdd(dst + 1, fixed_of(vj.y * F.wx[i]));
+          atomicAdd(dst + CS, fixed_of(vj.y * F.wx[i]));
         }
       }
     }
@@ -814,8 +816,8 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
     {
       unsigned long long* ac = acc + 2 * col * CS;
       float2* fc = fbuf + 2 * col * CS;
-      fft_wave_from_fixed<G0>(fc, ac, stw, lane, sc);
-      for (int q = lane; q < CS; q += 64) ac[CS + q] = 0ull;
+      fft_wave_from_fixed<G0, CS>(fc, ac, stw, lane, sc);
+      for (int q = lane; q < CS; q += 64) ac[CS + q] = 0ull;   // the im plane
       for (int r = S0 - h0 + lane; r < G0 - h0; r += 64) fc[PIDX(r)] = make_float2(0.f, 0.f);
       for (int q = lane; q < CS; q += 64)   // slots that hold no row: the padding holes and the tail
         if (q % 33 == 32 || q > PIDX(G0 - 1)) fc[q] = make_float2(0.f, 0.f);
