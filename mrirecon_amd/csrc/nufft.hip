@@ -619,15 +619,22 @@ __device__ __forceinline__ float fmax_finite(float m, float2 v) {
 // 2^-30 of that largest component, far below FP32 rounding of the large
 // terms) and summed exactly as int64 with ds_add_u64, which runs at full LDS
 // rate on gfx950 where float LDS atomics do not.  The sum is independent of
-// the order of the adds; converting back costs one rounding.
+// the order of the adds.
 __device__ __forceinline__ unsigned long long fixed_of(float scaled) {
   const int q = (int)rintf(scaled);
   return (unsigned long long)(long long)q;
 }
 
+// Back to float, times 2^-sc, in 32-bit arithmetic: q = hi 2^32 + lo with lo the
+// low word read as signed (|lo| <= 2^31) and hi adjusted to match, so q < 2^31
+// converts exactly rounded and larger q within one unit in the last place of
+// the correctly rounded value (the fma's product is exact, so only the rounding
+// of lo and the fma's own rounding enter); six 32-bit instructions in place of
+// a double-precision path.
 __device__ __forceinline__ float from_fixed(unsigned long long u, int sc) {
-  const long long q = (long long)u;
-  return (float)ldexp((double)q, -sc);
+  const int lo = (int)(unsigned)u;
+  const int hi = (int)(unsigned)(u >> 32) - (lo >> 31);
+  return ldexpf(fmaf((float)hi, 4294967296.f, (float)lo), -sc);
 }
 
 // Inverse length-N FFT of one column held as fixed-point planes (re at word
@@ -812,15 +819,20 @@ __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
     // Inverse column FFT by the wavefront that owns the column; its first pass
     // reads the fixed-point accumulators and writes floats into the column's own
     // int64 region (float slot p is the region's word p).  The wavefront then
-    // zeroes every word that the read-out below does not visit.
+    // zeroes the rows that the read-out below does not visit (the read-out
+    // zeroes the ones it reads) and the im plane.  Slots that hold no row (the
+    // padding holes of PIDX and the tail) are never written after the initial
+    // clear, so they stay zero.
     {
       unsigned long long* ac = acc + 2 * col * CS;
       float2* fc = fbuf + 2 * col * CS;
       fft_wave_from_fixed<G0, CS>(fc, ac, stw, lane, sc);
-      for (int q = lane; q < CS; q += 64) ac[CS + q] = 0ull;   // the im plane
+#pragma unroll
+      for (int k = 0; k < (CS + 63) / 64; ++k) {   // the im plane
+        const int q = lane + 64 * k;
+        if (CS % 64 == 0 || q < CS) ac[CS + q] = 0ull;
+      }
       for (int r = S0 - h0 + lane; r < G0 - h0; r += 64) fc[PIDX(r)] = make_float2(0.f, 0.f);
-      for (int q = lane; q < CS; q += 64)   // slots that hold no row: the padding holes and the tail
-        if (q % 33 == 32 || q > PIDX(G0 - 1)) fc[q] = make_float2(0.f, 0.f);
     }
     __syncthreads();                       // all columns transformed
     float2* Tb = Tbb + c * cstride;
