@@ -395,6 +395,26 @@ int mrh_unet_pack_input(const void* img, void* out, int64_t npix, int C, float s
 /* xout = xin + (net[:, 0] + i net[:, 1]) * inv_scale (complex64; net is BF16 (npix, Cn)). */
 int mrh_unet_unpack_residual(const void* xin, const void* net, void* xout, int64_t npix, int Cn,
                              float inv_scale, void* stream);
+/* Inference with the full-resolution level fused (no reference counterpart; one volume of F
+ * frames per call, 32 channels at that level; MRH_EUNSUP unless
+ * mrh_unet_level0_fused_supported(H, W) returns 1, i.e. H % 8 == 0 and W % 32 == 0).
+ * mrh_unet_enc0_fused: e0 = relu(convT(relu(conv3x3(pack(x))))) and i1 = pool2(e0), where
+ *   pack(x) = the 8-channel input of mrh_unet_pack_input (x complex64 (F, H, W), scale),
+ *   ws/bs the 3x3 layer (BF16 [9][32][8], FP32 bias[32]), wt/bt the temporal layer
+ *   (BF16 [3][32][32], bias[32]); e0 BF16 (F, H, W, 32), i1 BF16 (F, H/2, W/2, 32).  The
+ *   3x3 layer's output never reaches memory.
+ * mrh_unet_dec0_tail: xout = xin + (out[:, 0] + i out[:, 1]) * inv_scale with
+ *   out = conv1x1(relu(convT(b0))): b0 BF16 (F, H, W, 32), wt/bt the temporal layer,
+ *   wo/bo the 1x1 layer (BF16 [1][32][32] with rows past Cout zero, bias >= 2 values); the
+ *   temporal layer's output and out never reach memory.
+ * Each equals the chain of mrh_unet_conv_fwd / pool2 / pack / unpack calls it replaces up
+ * to the FP32 summation order inside the MFMA tiles (activations are rounded to BF16 at the
+ * same points). */
+int mrh_unet_level0_fused_supported(int H, int W);
+int mrh_unet_enc0_fused(const void* x, float scale, const void* ws, const float* bs, const void* wt,
+                        const float* bt, void* e0, void* i1, int F, int H, int W, void* stream);
+int mrh_unet_dec0_tail(const void* b0, const void* wt, const float* bt, const void* wo, const float* bo,
+                       const void* xin, float inv_scale, void* xout, int F, int H, int W, void* stream);
 /* loss += weight * sum |(xout - xt) scale|^2 (FP64, device scalar) and
  * gnet[:, 0:2] = 2 weight (xout - xt) scale + gx_coef (Re gx, Im gx),
  * gnet[:, 2:] = 0 (BF16 (npix, Cn)).  gx (complex64, may be NULL) is the

@@ -15,7 +15,7 @@ import torch  # noqa: F401  (loads the HIP runtime first)
 
 # MRH_LIB selects another build of the same ABI (e.g. a debug build); default is the in-tree one.
 LIB_PATH = Path(os.environ.get("MRH_LIB") or Path(__file__).resolve().parent / "libmrirecon_hip.so")
-ABI_VERSION = 12
+ABI_VERSION = 13
 
 DCF_NONE, DCF_BALANCED, DCF_STANDARD = 0, 1, 2
 TV_AXIS_T, TV_AXIS_X, TV_AXIS_Y = 1, 2, 4
@@ -122,6 +122,9 @@ _SIGS = {
     "mrh_unet_up2_concat_bwd": (c_int, [c_void_p, c_void_p, c_void_p] + [c_int] * 5 + [c_void_p]),
     "mrh_unet_pack_input": (c_int, [c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),
     "mrh_unet_unpack_residual": (c_int, [c_void_p, c_void_p, c_void_p, c_int64, c_int, c_float, c_void_p]),
+    "mrh_unet_level0_fused_supported": (c_int, [c_int, c_int]),
+    "mrh_unet_enc0_fused": (c_int, [c_void_p, c_float] + [c_void_p] * 6 + [c_int] * 3 + [c_void_p]),
+    "mrh_unet_dec0_tail": (c_int, [c_void_p] * 6 + [c_float, c_void_p] + [c_int] * 3 + [c_void_p]),
     "mrh_unet_mse_grad": (c_int, [c_void_p, c_void_p, c_void_p, c_float, c_void_p, c_void_p, c_int64, c_int,
                                   c_float, c_float, c_void_p]),
     "mrh_unet_fill": (c_int, [c_void_p, c_int64, c_float, c_void_p]),

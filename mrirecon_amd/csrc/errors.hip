@@ -13,4 +13,4 @@ void mrh_set_error(const char* fmt, ...) {
 
 extern "C" const char* mrh_last_error(void) { return g_err; }
 
-extern "C" int mrh_abi_version(void) { return 12; }
+extern "C" int mrh_abi_version(void) { return 13; }

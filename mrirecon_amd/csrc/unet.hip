@@ -696,6 +696,267 @@ __global__ void k_unpack_residual(const float2* __restrict__ xin, const u16* __r
   }
 }
 
+// ------------------------------------------------------------------ fused full-resolution level (inference)
+// The 32-channel activations of the full-resolution level that only feed the next
+// layer (a_0, h_0, the network output and the packed input) never reach HBM: each
+// kernel fixes a pixel tile and visits the frames of its frame range in order, so a
+// temporal tap's three frames are on chip.  Same operations, roundings and epilogues
+// as the separate layers (BF16 storage of every activation, FP32 accumulation); only
+// the MFMA accumulation order of the sums differs.
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ unsigned bf_pair(float a, float b) { return (unsigned)f2bf(a) | ((unsigned)f2bf(b) << 16); }
+
+#define L0_C 32        // channels of the full-resolution level
+#define L0_RS 40       // LDS stride (halves) of one pixel's 32 channels: 16-lane fragment reads hit distinct banks
+#define E0_TH 8        // enc0 tile: 8 rows x 32 columns, wave w owns rows 2w and 2w + 1
+#define E0_TW 32
+#define E0_HX (E0_TW + 2)
+#define E0_NH ((E0_TH + 2) * E0_HX)   // halo pixels of the 3x3 taps
+
+// e_0 = relu(convT(relu(conv3x3(pack(x))))) and i_1 = avgpool2x2(e_0) for one volume's
+// frames [fg by, fg by + fg).  pack(x) = (re, im, 0, ..., 0) * scale as BF16 (8 channels).
+// The MFMA operands are those of k_conv_fwd for these layers (3x3: K steps of four taps x 8
+// channels; temporal: one K step per tap), so every sum and rounding is the same and the
+// result equals the separate layers bit for bit.  Grid (H/8 * W/32, ceil(F/fg)).
+__global__ __launch_bounds__(256) void k_enc0_fused(const float2* __restrict__ x, const u16* __restrict__ ws,
+                                                    const float* __restrict__ bs, const u16* __restrict__ wt,
+                                                    const float* __restrict__ bt, u16* __restrict__ e0,
+                                                    u16* __restrict__ i1, int F, int H, int W, int fg,
+                                                    float scale) {
+  __shared__ unsigned sX[2][E0_NH];
+  __shared__ __align__(16) u16 sR[4][3][64 * L0_RS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, kg = lane >> 4;
+  const int tx_n = W / E0_TW;
+  const int y0 = (blockIdx.x / tx_n) * E0_TH, x0 = (blockIdx.x % tx_n) * E0_TW;
+  const int fa = blockIdx.y * fg, fb = min(F, fa + fg);
+  const size_t HW = (size_t)H * W;
+  // A fragments (weights) for the block's lifetime; 3x3 K step s holds tap 4 s + kg in this lane
+  bf16x8 As[3][2], At[3][2];
+  float bsv[2][4], btv[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int co = 16 * a + r;
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+      const int t = 4 * st + kg;
+      As[st][a] = t < 9 ? load8(ws + ((size_t)t * L0_C + co) * 8) : zero8();
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) At[d][a] = load8(wt + ((size_t)d * L0_C + co) * L0_C + 8 * kg);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      bsv[a][j] = bs[16 * a + 4 * kg + j];
+      btv[a][j] = bt[16 * a + 4 * kg + j];
+    }
+  }
+  // halo offsets of this lane's taps 4 s + kg, s = 0, 1, 2
+  int toff[3];
+  bool tok[3];
+#pragma unroll
+  for (int st = 0; st < 3; ++st) {
+    const int t = 4 * st + kg;
+    tok[st] = t < 9;
+    toff[st] = tok[st] ? (t / 3) * E0_HX + (t % 3) : 0;
+  }
+  // halo pixels tid and tid + 256 of a frame, prefetched one frame ahead
+  float2 pre[2];
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int hp = tid + 256 * q;
+      pre[q] = make_float2(0.f, 0.f);
+      if (hp < E0_NH) {
+        const int gy = y0 + hp / E0_HX - 1, gx = x0 + hp % E0_HX - 1;
+        if ((unsigned)gy < (unsigned)H && (unsigned)gx < (unsigned)W) pre[q] = x[(size_t)f * HW + (size_t)gy * W + gx];
+      }
+    }
+  };
+  u16* ring = sR[wave][0];
+  auto temporal = [&](int fo) {   // e_0 and i_1 of frame fo from the ring
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int h = 0; h < 4; ++h) acc[a][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const int f = fo - 1 + d;
+      if (f < 0 || f >= F) continue;
+      const u16* sl = ring + (f % 3) * 64 * L0_RS;
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        const bf16x8 bv = load8(sl + (16 * h + r) * L0_RS + 8 * kg);
+#pragma unroll
+        for (int a = 0; a < 2; ++a) acc[a][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(At[d][a], bv, acc[a][h], 0, 0, 0);
+      }
+    }
+    // epilogue: bias, ReLU, BF16; the 2 x 2 pool of rows 2w, 2w+1 (columns h and h + 2 of this
+    // lane) with the horizontal neighbour in lane r ^ 1, summed in the order of k_pool_fwd
+    float ev[2][4][4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int ty = 2 * wave + (h >> 1), tx = 16 * (h & 1) + r;
+      u16* dst = e0 + (((size_t)fo * H + y0 + ty) * W + x0 + tx) * L0_C;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) ev[a][h][j] = bf2f(f2bf(fmaxf(acc[a][h][j] + btv[a][j], 0.f)));
+        *reinterpret_cast<uint2*>(dst + 16 * a + 4 * kg) =
+            make_uint2(bf_pair(ev[a][h][0], ev[a][h][1]), bf_pair(ev[a][h][2], ev[a][h][3]));
+      }
+    }
+    const int Ho = H / 2, Wo = W / 2;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int tx = 16 * h + r;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        float pv[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const float p00 = ev[a][h][j], p10 = ev[a][h + 2][j];
+          const float p01 = __shfl_xor(p00, 1, 64), p11 = __shfl_xor(p10, 1, 64);
+          float s = p00;
+          s += p01;
+          s += p10;
+          pv[j] = 0.25f * (s + p11);
+        }
+        if ((r & 1) == 0) {
+          u16* dst = i1 + (((size_t)fo * Ho + y0 / 2 + wave) * Wo + (x0 + tx) / 2) * L0_C + 16 * a + 4 * kg;
+          *reinterpret_cast<uint2*>(dst) = make_uint2(bf_pair(pv[0], pv[1]), bf_pair(pv[2], pv[3]));
+        }
+      }
+    }
+  };
+  const int s_lo = max(fa - 1, 0), s_hi = min(fb, F - 1);
+  fetch(s_lo);
+  for (int fs = s_lo; fs <= s_hi; ++fs) {
+    unsigned* xb = sX[fs & 1];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int hp = tid + 256 * q;
+      if (hp < E0_NH) xb[hp] = bf_pair(pre[q].x * scale, pre[q].y * scale);
+    }
+    if (fs < s_hi) fetch(fs + 1);
+    __syncthreads();   // the halo tile of frame fs is in; every wave is past the tile of frame fs - 2
+    // a_0 of frame fs for this wave's 64 pixels -> ring slot fs % 3
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const int ty = 2 * wave + (h >> 1), tx = 16 * (h & 1) + r;
+      const int base = ty * E0_HX + tx;
+#pragma unroll
+      for (int a = 0; a < 2; ++a) acc[a][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int st = 0; st < 3; ++st) {
+        // channels 0..7 of tap 4 st + kg: (re, im) then six zeros
+        const bf16x8 bv = __builtin_bit_cast(bf16x8, make_uint4(tok[st] ? xb[base + toff[st]] : 0u, 0u, 0u, 0u));
+#pragma unroll
+        for (int a = 0; a < 2; ++a) acc[a][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(As[st][a], bv, acc[a][h], 0, 0, 0);
+      }
+    }
+    wave_lds_sync();   // the ring slot being replaced (frame fs - 3) has been read
+    u16* sl = ring + (fs % 3) * 64 * L0_RS;
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = fmaxf(acc[a][h][j] + bsv[a][j], 0.f);
+        *reinterpret_cast<uint2*>(sl + (16 * h + r) * L0_RS + 16 * a + 4 * kg) =
+            make_uint2(bf_pair(v[0], v[1]), bf_pair(v[2], v[3]));
+      }
+    wave_lds_sync();
+    if (fs - 1 >= fa && fs - 1 < fb) temporal(fs - 1);
+  }
+  if (s_hi == F - 1 && F - 1 >= fa && F - 1 < fb) temporal(F - 1);
+}
+
+// x_out = x_in + (out_0 + i out_1) / scale with out = conv1x1(h_0), h_0 = relu(convT(b_0)), for one
+// volume's frames [fg by, fg by + fg): the temporal layer, the 1x1 output layer and the residual
+// of k_unpack_residual in one pass.  Each wave owns 64 consecutive pixels of a frame; its three
+// b_0 frames are registers (the MFMA B fragment is 16 bytes of one pixel's channels, read
+// straight from memory); h_0 goes through a wave-private LDS tile into the 1x1 layer's B
+// layout.  Grid (H W / 256, ceil(F/fg)); needs H W % 256 == 0, out channels 0, 1 = (re, im).
+__global__ __launch_bounds__(256) void k_dec0_tail(const u16* __restrict__ b0, const u16* __restrict__ wt,
+                                                   const float* __restrict__ bt, const u16* __restrict__ wo,
+                                                   const float* __restrict__ bo, const float2* __restrict__ xin,
+                                                   float2* __restrict__ xout, int F, int HW, int fg,
+                                                   float inv_scale) {
+  __shared__ __align__(16) u16 sH[4][64 * L0_RS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, r = lane & 15, kg = lane >> 4;
+  const int p0 = blockIdx.x * 256 + 64 * wave;
+  const int fa = blockIdx.y * fg, fb = min(F, fa + fg);
+  bf16x8 At[3][2], Ao;
+  float btv[2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) At[d][a] = load8(wt + ((size_t)d * L0_C + 16 * a + r) * L0_C + 8 * kg);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) btv[a][j] = bt[16 * a + 4 * kg + j];
+  }
+  Ao = load8(wo + (size_t)r * L0_C + 8 * kg);   // output channels 0..15 (rows past Cout are zero)
+  const float bo0 = bo[0], bo1 = bo[1];
+  auto frame = [&](int f, bf16x8* v) {
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+      v[h] = (f >= 0 && f < F) ? load8(b0 + ((size_t)f * HW + p0 + 16 * h + r) * L0_C + 8 * kg) : zero8();
+  };
+  bf16x8 cm[4], c0[4], cp[4], nx[4];
+  frame(fa - 1, cm);
+  frame(fa, c0);
+  frame(fa + 1, cp);
+  u16* sh = sH[wave];
+  for (int fo = fa; fo < fb; ++fo) {
+    frame(fo + 1 < fb ? fo + 2 : F, nx);   // frame F reads as zero: nothing to fetch past the group
+    f32x4 acc[2][4];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int h = 0; h < 4; ++h) {
+        f32x4 s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(At[0][a], cm[h], f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        s = __builtin_amdgcn_mfma_f32_16x16x32_bf16(At[1][a], c0[h], s, 0, 0, 0);
+        acc[a][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(At[2][a], cp[h], s, 0, 0, 0);
+      }
+    wave_lds_sync();   // the previous frame's h_0 tile has been read
+#pragma unroll
+    for (int h = 0; h < 4; ++h)
+#pragma unroll
+      for (int a = 0; a < 2; ++a) {
+        float v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = fmaxf(acc[a][h][j] + btv[a][j], 0.f);
+        *reinterpret_cast<uint2*>(sh + (16 * h + r) * L0_RS + 16 * a + 4 * kg) =
+            make_uint2(bf_pair(v[0], v[1]), bf_pair(v[2], v[3]));
+      }
+    wave_lds_sync();
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      const bf16x8 bv = load8(sh + (16 * h + r) * L0_RS + 8 * kg);
+      const f32x4 o = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Ao, bv, f32x4{0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+      if (kg == 0) {   // rows 0, 1 of the output tile: channels re, im of pixel 16 h + r
+        const size_t p = (size_t)fo * HW + p0 + 16 * h + r;
+        const float2 a = xin[p];
+        xout[p] = make_float2(a.x + bf2f(f2bf(o[0] + bo0)) * inv_scale, a.y + bf2f(f2bf(o[1] + bo1)) * inv_scale);
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 4; ++h) {
+      cm[h] = c0[h];
+      c0[h] = cp[h];
+      cp[h] = nx[h];
+    }
+  }
+}
+
 __global__ __launch_bounds__(256) void k_mse_grad(const float2* __restrict__ xout, const float2* __restrict__ xt,
                                                   const float2* __restrict__ gx, float gx_coef,
                                                   u16* __restrict__ gnet, int64_t npix, int Cn, float scale,
@@ -961,6 +1222,41 @@ extern "C" int mrh_unet_unpack_residual(const void* xin, const void* net, void* 
   MRH_REQUIRE(xin && net && xout && npix >= 1 && Cn >= 2, MRH_EINVAL, "unet_unpack_residual: bad arguments");
   hipLaunchKernelGGL(k_unpack_residual, dim3(grid_for(npix)), dim3(256), 0, (hipStream_t)stream,
                      (const float2*)xin, (const u16*)net, (float2*)xout, npix, Cn, inv_scale);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+// Frame groups per volume: enough workgroups for two per CU (a group re-reads its two edge frames).
+static int level0_groups(int tiles, int F) { return std::max(1, std::min(F, (512 + tiles - 1) / tiles)); }
+
+extern "C" int mrh_unet_level0_fused_supported(int H, int W) {
+  return (H >= 8 && W >= 32 && H % 8 == 0 && W % 32 == 0 && ((int64_t)H * W) % 256 == 0) ? 1 : 0;
+}
+
+extern "C" int mrh_unet_enc0_fused(const void* x, float scale, const void* ws, const float* bs, const void* wt,
+                                   const float* bt, void* e0, void* i1, int F, int H, int W, void* stream) {
+  MRH_REQUIRE(x && ws && bs && wt && bt && e0 && i1, MRH_EINVAL, "unet_enc0_fused: null buffer");
+  MRH_REQUIRE(F >= 1 && mrh_unet_level0_fused_supported(H, W), MRH_EUNSUP,
+              "unet_enc0_fused: %d frames of %dx%d (needs H %% 8 == 0, W %% 32 == 0)", F, H, W);
+  MRH_REQUIRE((int64_t)F * H * W < (1ll << 31), MRH_EINVAL, "unet_enc0_fused: volume too large");
+  const int tiles = (H / E0_TH) * (W / E0_TW), ng = level0_groups(tiles, F), fg = (F + ng - 1) / ng;
+  hipLaunchKernelGGL(k_enc0_fused, dim3(tiles, (F + fg - 1) / fg), dim3(256), 0, (hipStream_t)stream,
+                     (const float2*)x, (const u16*)ws, bs, (const u16*)wt, bt, (u16*)e0, (u16*)i1, F, H, W, fg,
+                     scale);
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+extern "C" int mrh_unet_dec0_tail(const void* b0, const void* wt, const float* bt, const void* wo, const float* bo,
+                                  const void* xin, float inv_scale, void* xout, int F, int H, int W, void* stream) {
+  MRH_REQUIRE(b0 && wt && bt && wo && bo && xin && xout, MRH_EINVAL, "unet_dec0_tail: null buffer");
+  MRH_REQUIRE(F >= 1 && mrh_unet_level0_fused_supported(H, W), MRH_EUNSUP,
+              "unet_dec0_tail: %d frames of %dx%d (needs H %% 8 == 0, W %% 32 == 0)", F, H, W);
+  MRH_REQUIRE((int64_t)F * H * W < (1ll << 31), MRH_EINVAL, "unet_dec0_tail: volume too large");
+  const int HW = H * W, tiles = HW / 256, ng = level0_groups(tiles, F), fg = (F + ng - 1) / ng;
+  hipLaunchKernelGGL(k_dec0_tail, dim3(tiles, (F + fg - 1) / fg), dim3(256), 0, (hipStream_t)stream,
+                     (const u16*)b0, (const u16*)wt, bt, (const u16*)wo, bo, (const float2*)xin, (float2*)xout, F,
+                     HW, fg, inv_scale);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }

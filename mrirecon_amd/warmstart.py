@@ -282,12 +282,32 @@ class UNet2DT:
         """Inference reads the decoder concatenation of level l in place (no stored copy)."""
         return bool(self.lib.mrh_unet_upcat_supported(H >> l, W >> l, self.ch[l], self.ch[l]))
 
-    def _forward(self, ws, nvol: int, F: int, H: int, W: int, train: bool = True) -> None:
+    def _level0_fused(self, H: int, W: int) -> bool:
+        """Inference runs the full-resolution level as two fused kernels (mrh_unet_enc0_fused,
+        mrh_unet_dec0_tail) when the shapes allow it."""
+        return (self.ch[0] == 32 and self.in_ch == 8 and self.layer["out"].cout_pad == 32
+                and bool(self.lib.mrh_unet_level0_fused_supported(H, W)))
+
+    def _forward(self, ws, nvol: int, F: int, H: int, W: int, train: bool = True,
+                 fused0: Optional[Tuple[torch.Tensor, torch.Tensor, Sequence[float]]] = None) -> None:
+        """The network on ws["x8"] into ws["out"]; with fused0 = (x_adj, x_out, scales) (inference)
+        the full-resolution level reads x_adj and writes x_out = x_adj + net / scale directly."""
         Lv, L = self.levels, self.layer
         geo = lambda l: (nvol, F, H >> l, W >> l)
         nf = nvol * F
+        per = F * H * W
         inp = ws["x8"]
         for l in range(Lv):
+            if l == 0 and fused0 is not None:
+                x_adj, _, scales = fused0
+                Ls, Lt = L["enc0_s"], L["enc0_t"]
+                for v, sc in enumerate(scales):
+                    _lib.check(self.lib.mrh_unet_enc0_fused(
+                        x_adj.reshape(-1)[v * per:].data_ptr(), float(sc), Ls.bufs["wf"].data_ptr(),
+                        Ls.bufs["b"].data_ptr(), Lt.bufs["wf"].data_ptr(), Lt.bufs["b"].data_ptr(),
+                        ws["e0"][v * per:].data_ptr(), ws["i1"][v * per // 4:].data_ptr(), F, H, W, self._st()))
+                inp = ws["i1"]
+                continue
             self._conv(L[f"enc{l}_s"], inp, ws[f"a{l}"], geo(l))
             self._conv(L[f"enc{l}_t"], ws[f"a{l}"], ws[f"e{l}"], geo(l))
             if l < Lv - 1:
@@ -307,6 +327,15 @@ class UNet2DT:
                                                         ws[f"u{l}"].data_ptr(), nf, H >> l, W >> l, self.ch[l],
                                                         self.ch[l], self._st()))
                 self._conv(L[f"dec{l}_s"], ws[f"u{l}"], ws[f"b{l}"], geo(l))
+            if l == 0 and fused0 is not None:
+                x_adj, x_out, scales = fused0
+                Lt, Lo = L["dec0_t"], L["out"]
+                for v, sc in enumerate(scales):
+                    _lib.check(self.lib.mrh_unet_dec0_tail(
+                        ws["b0"][v * per:].data_ptr(), Lt.bufs["wf"].data_ptr(), Lt.bufs["b"].data_ptr(),
+                        Lo.bufs["wf"].data_ptr(), Lo.bufs["b"].data_ptr(), x_adj.reshape(-1)[v * per:].data_ptr(),
+                        float(1.0 / sc), x_out.reshape(-1)[v * per:].data_ptr(), F, H, W, self._st()))
+                return
             self._conv(L[f"dec{l}_t"], ws[f"b{l}"], ws[f"h{l}"], geo(l))
         self._conv(L["out"], ws["h0"], ws["out"], geo(0))
 
@@ -370,13 +399,20 @@ class UNet2DT:
 
     @torch.no_grad()
     def infer(self, x_adj: torch.Tensor, nvol: int = 1, out: Optional[torch.Tensor] = None,
-              scales: Optional[Sequence[float]] = None) -> torch.Tensor:
-        """x_warm = x_adj + U-Net(x_adj) for (nvol * F, H, W) complex64 input."""
+              scales: Optional[Sequence[float]] = None, fused: Optional[bool] = None) -> torch.Tensor:
+        """x_warm = x_adj + U-Net(x_adj) for (nvol * F, H, W) complex64 input.  fused: run the
+        full-resolution level as fused kernels (default: whenever the shapes allow it); False
+        runs every layer separately, as the training forward does."""
         F, H, W = self._check_image(x_adj, nvol)
         ws = self._workspace(nvol, F, H, W, train=False)
         if scales is None:
             scales = self.volume_scales(x_adj, nvol, ws)
         out = ws["x_warm"] if out is None else out
+        if fused is None:
+            fused = self._level0_fused(H, W)
+        if fused:
+            self._forward(ws, nvol, F, H, W, train=False, fused0=(x_adj, out, list(scales)))
+            return out
         self._pack(ws, x_adj, scales, F)
         self._forward(ws, nvol, F, H, W, train=False)
         self._unpack(ws, x_adj, out, scales, F)

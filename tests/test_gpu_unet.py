@@ -319,6 +319,46 @@ def test_inference_reading_the_concatenation_in_place_equals_the_training_forwar
     assert torch.equal(a, b)
 
 
+@pytest.mark.parametrize("geom", [(2, 7, 32, 64), (1, 34, 64, 32), (1, 1, 8, 32), (1, 5, 256, 256)])
+def test_fused_full_resolution_level_equals_the_separate_layers(geom):
+    """Inference with the full-resolution level fused (enc0 3x3 + temporal + pool from the
+    complex input; dec0 temporal + 1x1 output + residual into the complex output) gives the
+    bits of the layer-by-layer forward: one frame per workgroup group and several frame
+    groups per volume (256^2: two groups), two volumes with their own scales, a single frame."""
+    nvol, F, H, W = geom
+    lib = _lib.load()
+    assert lib.mrh_unet_level0_fused_supported(H, W) == 1
+    net = UNet2DT(levels=3, base=32, device=DEV, seed=11)
+    rng = np.random.default_rng(F + H)
+    for L in net.layer.values():
+        L.bufs["wf"].copy_((torch.randn_like(L.bufs["wf"].float()) * 0.08).to(torch.bfloat16))
+        L.bufs["b"].copy_(torch.randn_like(L.bufs["b"]) * 0.05)
+    shape = (nvol * F, H, W)
+    x_adj = (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)).astype(np.complex64)
+    xa = torch.from_numpy(x_adj).to(DEV)
+    scales = [0.7, 1.3][:nvol]
+    a = net.infer(xa, nvol=nvol, scales=scales, fused=True).clone()
+    b = net.infer(xa, nvol=nvol, scales=scales, fused=False).clone()
+    assert not torch.equal(a, xa)
+    assert torch.equal(a, b)
+
+
+def test_fused_full_resolution_level_refuses_other_shapes():
+    lib = _lib.load()
+    assert lib.mrh_unet_level0_fused_supported(16, 48) == 0 and lib.mrh_unet_level0_fused_supported(12, 32) == 0
+    net = UNet2DT(levels=3, base=32, device=DEV)
+    assert not net._level0_fused(16, 48)
+    x = torch.zeros((2, 16, 48), dtype=torch.complex64, device=DEV)
+    e = torch.empty((2 * 16 * 48, 32), dtype=torch.bfloat16, device=DEV)
+    Ls, Lt = net.layer["enc0_s"], net.layer["enc0_t"]
+    rc = lib.mrh_unet_enc0_fused(x.data_ptr(), 1.0, Ls.bufs["wf"].data_ptr(), Ls.bufs["b"].data_ptr(),
+                                 Lt.bufs["wf"].data_ptr(), Lt.bufs["b"].data_ptr(), e.data_ptr(), e.data_ptr(),
+                                 2, 16, 48, st())
+    with pytest.raises(NotImplementedError):
+        _lib.check(rc)
+    assert torch.equal(net.infer(x), x)   # the separate layers still run (zero output layer: identity)
+
+
 def test_shape_errors():
     net = UNet2DT(levels=3, base=32, device=DEV)
     with pytest.raises(ValueError):
