@@ -88,6 +88,7 @@ struct mrh_nufft_plan {
   int2* d_em;        // (m0x | m0y << 16, k)
   float4* d_ef;      // (u - m0x, v - m0y, wA, wAH)
   int* d_soff;       // B * nstrips + 1
+  int* d_srank;      // B * nstrips: frame b's strips in decreasing order of entries (dispatch order)
   float* d_dx;
   float* d_dy;
   float2* d_tw0;
@@ -127,7 +128,7 @@ static bool fft_size_ok(int n) {
 
 static int free_plan(mrh_nufft_plan* p) {
   if (!p) return MRH_OK;
-  void* bufs[] = {p->d_koff, p->d_em, p->d_ef, p->d_soff, p->d_dx, p->d_dy,
+  void* bufs[] = {p->d_koff, p->d_em, p->d_ef, p->d_soff, p->d_srank, p->d_dx, p->d_dy,
                   p->d_tw0, p->d_tw1, p->d_maps, p->d_T};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete p;
@@ -275,6 +276,16 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
     maxe = std::max(maxe, cnt[i]);
   }
   soff[(size_t)B * ns] = (int)acc;
+  // Strip kernels run a frame's strips heaviest first (longest-processing-time order): the
+  // strips around the k-space centre hold up to twenty times the entries of the outer ones, and
+  // started last they would be the launch's tail.
+  std::vector<int> srank((size_t)B * ns);
+  for (int b = 0; b < B; ++b) {
+    int* rk = srank.data() + (size_t)b * ns;
+    for (int st = 0; st < ns; ++st) rk[st] = st;
+    const int* cb = cnt.data() + (size_t)b * ns;
+    std::stable_sort(rk, rk + ns, [cb](int u, int v) { return cb[u] > cb[v]; });
+  }
   p->n_entries = acc;
   p->max_strip_entries = maxe;
   std::unique_ptr<int2[]> em(new int2[(size_t)acc]);
@@ -317,6 +328,7 @@ extern "C" int mrh_nufft_plan_create(const mrh_nufft_desc* d, mrh_nufft_plan** o
   UP(p->d_em, em.get(), (size_t)acc);
   UP(p->d_ef, ef.get(), (size_t)acc);
   UP(p->d_soff, soff.data(), soff.size());
+  UP(p->d_srank, srank.data(), srank.size());
   UP(p->d_dx, dx.data(), dx.size());
   UP(p->d_dy, dy.data(), dy.size());
   UP(p->d_tw0, tw0.data(), tw0.size());
@@ -522,23 +534,25 @@ __device__ __forceinline__ float2 interp_one(const Foot<WM>& F, const float2* sm
   return acc;
 }
 
-// Forward strip pass.  Grid (nstrips, nbatch, coil chunks), 64 TC threads (one wavefront per column).
+// Forward strip pass.  Grid (coil chunks, nbatch, nstrips): blockIdx.z is the rank of the strip in
+// its frame's heaviest-first order (srank); 64 TC threads (one wavefront per column).
 template <int G0, int WT, int TC>
 __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_fwd_strips(
     const float2* __restrict__ T, float2* __restrict__ y, const int2* __restrict__ em,
-    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk,
-    FrameMask fm) {
+    const float4* __restrict__ ef, const int* __restrict__ soff, const int* __restrict__ srank,
+    const int64_t* __restrict__ koff, const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like,
+    int Wrt, int cchunk, FrameMask fm) {
   constexpr int NTH = 64 * TC, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
   if (frame_skipped(fm, blockIdx.y)) return;
-  const int cb = blockIdx.z * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
+  const int cb = blockIdx.x * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
   __shared__ float2 smem2[2 * TC * CS];   // double buffer: coil c fills one half while c-1 is read
   __shared__ float2 stw[G0];
   const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
-  const int s = blockIdx.x, bb = blockIdx.y, nstr = gridDim.x;
+  const int bb = blockIdx.y, nstr = gridDim.z;
   const int b = bb / like, l = bb - b * like;
+  const int s = srank[b * nstr + blockIdx.z];
   for (int i = tid; i < G0; i += NTH) stw[i] = tw0[i];
   const int e0 = soff[b * nstr + s], e1 = soff[b * nstr + s + 1];
   const int64_t Kb = koff[b + 1] - koff[b];
@@ -731,25 +745,26 @@ __device__ __forceinline__ void spread_one(const FootA<WM>& F, float2 vs, unsign
 }
 
 // Adjoint strip pass: fixed-point spreading into the LDS strip, inverse column
-// FFT, crop to the S0 image rows.  Grid (nstrips, nbatch, coil chunks), 64 TC threads.
+// FFT, crop to the S0 image rows.  Grid (coil chunks, nbatch, nstrips) as in the forward, 64 TC threads.
 template <int G0, int WT, int TC>
 __global__ __launch_bounds__(64 * TC, (G0 <= 512 ? 4 : 2)) void k_adj_strips(
     const float2* __restrict__ y, float2* __restrict__ T, const int2* __restrict__ em,
-    const float4* __restrict__ ef, const int* __restrict__ soff, const int64_t* __restrict__ koff,
-    const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like, int Wrt, int cchunk,
-    FrameMask fm) {
+    const float4* __restrict__ ef, const int* __restrict__ soff, const int* __restrict__ srank,
+    const int64_t* __restrict__ koff, const float2* __restrict__ tw0, KBPoly kb, int S0, int G1, int C, int like,
+    int Wrt, int cchunk, FrameMask fm) {
   constexpr int NTH = 64 * TC, CS = fft_padded(G0) + 1;
   constexpr int WM = WT ? WT : MRH_WMAX;
   const int W = WT ? WT : Wrt;
   if (frame_skipped(fm, blockIdx.y)) return;
-  const int cb = blockIdx.z * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
+  const int cb = blockIdx.x * cchunk, ce = min(C, cb + cchunk);   // this workgroup's coils
   __shared__ unsigned long long acc[2 * TC * CS];
   __shared__ float2 stw[G0];
   __shared__ float wmax[NTH / 64];
   float2* fbuf = reinterpret_cast<float2*>(acc);
   const int tid = threadIdx.x, lane = tid & 63, col = tid >> 6;
-  const int s = blockIdx.x, bb = blockIdx.y, nstr = gridDim.x;
+  const int bb = blockIdx.y, nstr = gridDim.z;
   const int b = bb / like, l = bb - b * like;
+  const int s = srank[b * nstr + blockIdx.z];
   for (int i = tid; i < G0; i += NTH) stw[i] = tw0[i];
   const int e0 = soff[b * nstr + s], e1 = soff[b * nstr + s + 1];
   const int64_t Kb = koff[b + 1] - koff[b];
@@ -940,13 +955,16 @@ static int launch_rows_adj(const mrh_nufft_plan* p, float2* x, int nb, FrameMask
 template <int G, int WT, int TC>
 static int launch_strips_tc(const mrh_nufft_plan* p, bool adjoint, const float2* yin, float2* yout,
                             int nb, int like, FrameMask fm, hipStream_t st) {
-  dim3 grid(p->nstrips, nb, (p->C + p->cchunk - 1) / p->cchunk), block(64 * TC);
+  // grid (coil chunk, frame, strip rank): the dispatch order runs every frame's heaviest strips first
+  dim3 grid((p->C + p->cchunk - 1) / p->cchunk, nb, p->nstrips), block(64 * TC);
   if (adjoint)
     hipLaunchKernelGGL((k_adj_strips<G, WT, TC>), grid, block, 0, st, yin, p->d_T, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk, fm);
+                       p->d_soff, p->d_srank, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W,
+                       p->cchunk, fm);
   else
     hipLaunchKernelGGL((k_fwd_strips<G, WT, TC>), grid, block, 0, st, p->d_T, yout, p->d_em, p->d_ef,
-                       p->d_soff, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W, p->cchunk, fm);
+                       p->d_soff, p->d_srank, p->d_koff, p->d_tw0, p->kb, p->S0, p->G1, p->C, like, p->W,
+                       p->cchunk, fm);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
 }
