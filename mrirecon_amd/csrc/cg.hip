@@ -204,39 +204,52 @@ __device__ __forceinline__ float2 fdiff(const float2* __restrict__ v, int64_t n,
   return make_float2(w * (p.x - q.x), w * (p.y - q.y));
 }
 
-// Flux components p_a and energy density of one term at (f, ix, iy).
-__device__ __forceinline__ float term_flux(const float2* __restrict__ x, int64_t n, int f, int ix, int iy,
-                                           const TVTermDev& tm, const TVDev& tv, float2 (&p)[3]) {
-  float2 dv[3];
+// Flux components p_a and energy density of one term from its weighted forward differences dv.
+// Contraction is off and the fused multiply-adds are written out: a flux enters the divergence of
+// its own pixel and of its forward neighbours, and every caller (the standalone TV kernel, which
+// evaluates it again at the neighbour, and the engine's tiled gradient, which shares it through
+// LDS) then gets the same rounded values.
+__device__ __forceinline__ float abs2_plus(float2 v, float s) { return fmaf(v.x, v.x, fmaf(v.y, v.y, s)); }
+
+__device__ __forceinline__ float flux_of(const float2 (&dv)[3], const TVTermDev& tm, float2 (&p)[3]) {
+#pragma clang fp contract(off)
   float ss = 0.f;
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    dv[a] = make_float2(0.f, 0.f);
-    if (tm.axes & (1 << a)) dv[a] = fdiff(x, n, f, ix, iy, a, tv, tm.w[a]);
-  }
   const float e2 = tm.eps * tm.eps;
   float e = 0.f;
   if (tm.iso) {
 #pragma unroll
-    for (int a = 0; a < 3; ++a) ss += dv[a].x * dv[a].x + dv[a].y * dv[a].y;
+    for (int a = 0; a < 3; ++a) ss = abs2_plus(dv[a], ss);
     const float den = sqrtf(ss + e2);
     e = den - tm.eps;
     const float inv = 1.f / fmaxf(den, 1e-12f);
 #pragma unroll
-    for (int a = 0; a < 3; ++a) p[a] = cscale(dv[a], inv);
+    for (int a = 0; a < 3; ++a) p[a] = make_float2(dv[a].x * inv, dv[a].y * inv);
   } else {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (tm.axes & (1 << a)) {
-        const float den = sqrtf(dv[a].x * dv[a].x + dv[a].y * dv[a].y + e2);
+        const float den = sqrtf(abs2_plus(dv[a], e2));
         e += den - tm.eps;
-        p[a] = cscale(dv[a], 1.f / fmaxf(den, 1e-12f));
+        const float inv = 1.f / fmaxf(den, 1e-12f);
+        p[a] = make_float2(dv[a].x * inv, dv[a].y * inv);
       } else {
         p[a] = make_float2(0.f, 0.f);
       }
     }
   }
   return e;
+}
+
+// Flux components p_a and energy density of one term at (f, ix, iy).
+__device__ __forceinline__ float term_flux(const float2* __restrict__ x, int64_t n, int f, int ix, int iy,
+                                           const TVTermDev& tm, const TVDev& tv, float2 (&p)[3]) {
+  float2 dv[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    dv[a] = make_float2(0.f, 0.f);
+    if (tm.axes & (1 << a)) dv[a] = fdiff(x, n, f, ix, iy, a, tv, tm.w[a]);
+  }
+  return flux_of(dv, tm, p);
 }
 
 // TV gradient contribution (-lambda div flux) and lambda * energy density of all terms at pixel n.
@@ -266,9 +279,9 @@ __device__ __forceinline__ float2 tv_grad_at(const float2* __restrict__ x, int64
         div = csub(div, pb[a]);
       }
     }
-    gsum.x -= tm.lam * div.x;
-    gsum.y -= tm.lam * div.y;
-    energy += tm.lam * e;
+    gsum.x = fmaf(-tm.lam, div.x, gsum.x);
+    gsum.y = fmaf(-tm.lam, div.y, gsum.y);
+    energy = fmaf(tm.lam, e, energy);
   }
   return gsum;
 }
@@ -403,6 +416,7 @@ struct mrh_cg {
   double c1, ls_beta, rel_tol, restart_c, c2;
   int64_t npix_p, kmax_p;
   int nb_i, nb_k;
+  int tg_tiles_x, tg_ntiles, tg_fg, nb_g;   // k_cg_grad_tiled: tiles per row, per frame, frames per group, blocks
   TVDev tv;
   int64_t* d_kprob;
   double* d_part_i;   // [P][nb_i][PI_STRIDE]
@@ -1006,26 +1020,191 @@ __global__ __launch_bounds__(RT) void k_accept_k(float2* __restrict__ Ax, const 
 // g += TV gradient at x; partials: TV energy, |g|^2, Re<g_prev, g>, Re<g, d>.  With a diagonal
 // preconditioner D (null: identity) the first two are weighted by 1 / D (graspcg/ops/dot.py:5-71
 // with diag, as the direction rules use them, graspcg/numerics/directions.py:75-162).
+// 16 x 16-pixel tiles walk their frames in order: each tile stages x of the frames f and f + 1 over
+// its pixels, one row / column before them (the backward neighbours) and one after (the forward
+// differences) in LDS; every flux is evaluated once (the tile's pixels, plus the 32 neighbours above
+// and to the left, which hold the spatial backward fluxes), and a term's temporal flux at f - 1 stays
+// in registers.  The fluxes are those of term_flux on the same values (same arithmetic, flux_of),
+// summed into the divergence in tv_grad_at's order, so g equals k_tv_energy_grad's to the bit; only
+// the per-thread order of the block partials differs.  Frame f + 2's stage and frame f + 1's g, g_prev,
+// d and D are loaded while frame f is computed, so a frame waits for loads issued one frame earlier.
+// Grid (ntiles * ngroups, P): block b runs tile b % ntiles over frames [fg (b / ntiles), + fg) (the
+// group re-reads x at its first frame - 1 for that frame's temporal flux); the groups are as many as
+// fit on the device at once.
+#define TG_T 16                  // tile side
+#define TG_H (TG_T + 2)          // staged side: one before, one after
+#define TG_P (TG_T + 1)          // flux side: one before
 template <typename R>
-__global__ __launch_bounds__(RT) void k_cg_grad(const float2* __restrict__ x, float2* __restrict__ g,
-                                                const float2* __restrict__ d, const float2* __restrict__ gp,
-                                                const float* __restrict__ D, TVDev tv, int64_t npix_p,
-                                                const ProbState* __restrict__ st, int prime, double* part) {
-  const int p = blockIdx.y;
+__global__ __launch_bounds__(RT) void k_cg_grad_tiled(const float2* __restrict__ x, float2* __restrict__ g,
+                                                      const float2* __restrict__ d, const float2* __restrict__ gp,
+                                                      const float* __restrict__ D, TVDev tv, int64_t npix_p,
+                                                      const ProbState* __restrict__ st, int prime, int tiles_x,
+                                                      int ntiles, int fg, double* part) {
+  __shared__ float2 sx[2][TG_H * TG_H];
+  __shared__ float2 sp[MRH_TV_MAX_TERMS][2][TG_P * TG_P];
+  const int p = blockIdx.y, tid = threadIdx.x;
   R v[5] = {R(0), R(0), R(0), R(0), R(0)};
-  if (prime || st[p].status == MRH_CG_ACTIVE) {
-    const int plane = tv.S0 * tv.S1;
-    for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
-      const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
-      const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
-      const int64_t n = (int64_t)p * npix_p + i;
-      float e = 0.f;
-      float2 gv = g[n];
-      if (tv.n > 0) gv = cadd(gv, tv_grad_at(x, n, f, ix, iy, tv, e));
-      g[n] = gv;
-      const float2 xv = x[n];
+  const bool on = prime || st[p].status == MRH_CG_ACTIVE;
+  const int tile = blockIdx.x % ntiles, grp = blockIdx.x / ntiles;
+  const int X0 = (tile / tiles_x) * TG_T, Y0 = (tile % tiles_x) * TG_T;   // first row (axis 1), column (axis 2)
+  const int F = tv.F, S0 = tv.S0, S1 = tv.S1;
+  const int fa = grp * fg, fb = min(F, fa + fg);
+  const int lx = tid / TG_T, ly = tid % TG_T;          // this thread's pixel in the tile
+  const int ix = X0 + lx, iy = Y0 + ly;
+  const bool mine = ix < S0 && iy < S1;
+  // halo flux position of threads 0..31: above the tile (row X0 - 1) or left of it (column Y0 - 1)
+  const bool hal = tid < 2 * TG_T;
+  const int hx = tid < TG_T ? -1 : tid - TG_T, hy = tid < TG_T ? tid : -1;
+  const int plane = S0 * S1;
+  const float2* xp = x + (int64_t)p * npix_p;
+  // x of frame f over rows X0-1 .. X0+16, columns Y0-1 .. Y0+16: element tid and tid + RT of the
+  // staged square, fetched into registers and put into LDS later (zero outside the volume)
+  auto fetch = [&](int f, float2 (&r)[2]) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int i = tid + j * RT, row = i / TG_H, c = i - row * TG_H;
+      const int gx = X0 - 1 + row, gy = Y0 - 1 + c;
+      r[j] = (i < TG_H * TG_H && f < F && (unsigned)gx < (unsigned)S0 && (unsigned)gy < (unsigned)S1)
+                 ? xp[(int64_t)f * plane + (int64_t)gx * S1 + gy] : make_float2(0.f, 0.f);
+    }
+  };
+  auto put = [&](float2* s, const float2 (&r)[2]) {
+    s[tid] = r[0];
+    if (tid + RT < TG_H * TG_H) s[tid + RT] = r[1];
+  };
+  // flux of term tm at tile position (rx, ry) (staged coordinates minus one) of frame f
+  auto flux_at = [&](const float2* s0, const float2* s1, int rx, int ry, int f, const TVTermDev& tm,
+                     float2 (&pf)[3]) {
+    const int gx = X0 + rx, gy = Y0 + ry, i = (rx + 1) * TG_H + (ry + 1);
+    const float2 q = s0[i];
+    float2 dv[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) dv[a] = make_float2(0.f, 0.f);
+    if ((tm.axes & 1) && f + 1 < F) { const float2 u = s1[i]; dv[0] = make_float2(tm.w[0] * (u.x - q.x), tm.w[0] * (u.y - q.y)); }
+    if ((tm.axes & 2) && gx + 1 < S0) { const float2 u = s0[i + TG_H]; dv[1] = make_float2(tm.w[1] * (u.x - q.x), tm.w[1] * (u.y - q.y)); }
+    if ((tm.axes & 4) && gy + 1 < S1) { const float2 u = s0[i + 1]; dv[2] = make_float2(tm.w[2] * (u.x - q.x), tm.w[2] * (u.y - q.y)); }
+    return flux_of(dv, tm, pf);
+  };
+  float2 pt[MRH_TV_MAX_TERMS];          // each term's temporal flux at (f - 1, this pixel)
+#pragma unroll
+  for (int k = 0; k < MRH_TV_MAX_TERMS; ++k) pt[k] = make_float2(0.f, 0.f);
+  if (on && tv.n > 0) {
+    int cur = 0;
+    float2 nx[2];                        // the next frame to stage, in flight during the current one
+    if (fa > 0) {                        // temporal fluxes of frame fa - 1
+      fetch(fa - 1, nx);
+      put(sx[0], nx);
+      fetch(fa, nx);
+      put(sx[1], nx);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < MRH_TV_MAX_TERMS; ++k) {
+        if (k >= tv.n) break;
+        const TVTermDev& tm = tv.t[k];
+        if (tm.lam == 0.f || tm.axes == 0 || !(tm.axes & 1)) continue;
+        float2 pf[3];
+        (void)flux_at(sx[0], sx[1], lx, ly, fa - 1, tm, pf);
+        pt[k] = pf[0];
+      }
+      cur = 1;
+    } else {
+      fetch(fa, nx);
+      put(sx[0], nx);
+    }
+    fetch(fa + 1, nx);
+    // g, g_prev, d and D of the next frame at this pixel, in flight during the current frame (each
+    // pixel's g is read and written by its own thread only)
+    const int64_t n0 = (int64_t)p * npix_p + (int64_t)ix * S1 + iy;
+    float2 ng = make_float2(0.f, 0.f), np = ng, nd = ng;
+    float nD = 1.f;
+    auto fetch_g = [&](int f) {
+      const int64_t n = n0 + (int64_t)f * plane;
+      if (mine) {
+        ng = g[n];
+        if (gp) np = gp[n];
+        if (d) nd = d[n];
+        if (D) nD = D[n];
+      }
+    };
+    fetch_g(fa);
+    for (int f = fa; f < fb; ++f) {
+      const int64_t n = n0 + (int64_t)f * plane;
+      float2 gv = ng;
+      const float2 qp = np, qd = nd;
+      const float dn = nD;
+      __syncthreads();                   // frame f - 1's stage and fluxes have been read
+      put(sx[cur ^ 1], nx);              // frame f + 1
+      if (f + 2 <= fb) fetch(f + 2, nx);
+      if (f + 1 < fb) fetch_g(f + 1);
+      __syncthreads();
+      const float2* s0 = sx[cur];
+      const float2* s1 = sx[cur ^ 1];
+      float2 pq[MRH_TV_MAX_TERMS][3];
+      float eq[MRH_TV_MAX_TERMS];
+#pragma unroll
+      for (int k = 0; k < MRH_TV_MAX_TERMS; ++k) {
+        eq[k] = 0.f;
+        if (k >= tv.n) break;
+        const TVTermDev& tm = tv.t[k];
+        if (tm.lam == 0.f || tm.axes == 0) continue;
+        eq[k] = flux_at(s0, s1, lx, ly, f, tm, pq[k]);
+        const int o = (lx + 1) * TG_P + (ly + 1);
+        sp[k][0][o] = pq[k][1];
+        sp[k][1][o] = pq[k][2];
+        if (hal) {
+          float2 ph[3];
+          (void)flux_at(s0, s1, hx, hy, f, tm, ph);
+          const int oh = (hx + 1) * TG_P + (hy + 1);
+          sp[k][0][oh] = ph[1];
+          sp[k][1][oh] = ph[2];
+        }
+      }
+      __syncthreads();
+      if (mine) {
+        // tv_grad_at's sums, in its order
+        float2 gsum = make_float2(0.f, 0.f);
+        float energy = 0.f;
+#pragma unroll
+        for (int k = 0; k < MRH_TV_MAX_TERMS; ++k) {
+          if (k >= tv.n) break;
+          const TVTermDev& tm = tv.t[k];
+          if (tm.lam == 0.f || tm.axes == 0) continue;
+          float2 div = make_float2(0.f, 0.f);
+          if (tm.axes & 1) {
+            div = cadd(div, pq[k][0]);
+            if (f > 0) div = csub(div, pt[k]);
+            pt[k] = pq[k][0];
+          }
+          if (tm.axes & 2) {
+            div = cadd(div, pq[k][1]);
+            if (ix > 0) div = csub(div, sp[k][0][lx * TG_P + (ly + 1)]);
+          }
+          if (tm.axes & 4) {
+            div = cadd(div, pq[k][2]);
+            if (iy > 0) div = csub(div, sp[k][1][(lx + 1) * TG_P + ly]);
+          }
+          gsum.x = fmaf(-tm.lam, div.x, gsum.x);
+          gsum.y = fmaf(-tm.lam, div.y, gsum.y);
+          energy = fmaf(tm.lam, eq[k], energy);
+        }
+        gv = cadd(gv, gsum);
+        g[n] = gv;
+        const float2 xv = s0[(lx + 1) * TG_H + (ly + 1)];
+        v[4] += (R)xv.x * xv.x + (R)xv.y * xv.y;
+        v[0] += (R)energy;
+        const R wd = D ? (R)1 / (R)dn : (R)1;
+        v[1] += ((R)gv.x * gv.x + (R)gv.y * gv.y) * wd;
+        if (gp) v[2] += ((R)qp.x * gv.x + (R)qp.y * gv.y) * wd;
+        if (d) v[3] += (R)qd.x * gv.x + (R)qd.y * gv.y;
+      }
+      cur ^= 1;
+    }
+  } else if (on) {                       // no TV term: the norms and dots of g alone
+    for (int f = fa; f < fb; ++f) {
+      if (!mine) continue;
+      const int64_t n = (int64_t)p * npix_p + (int64_t)f * plane + (int64_t)ix * S1 + iy;
+      const float2 gv = g[n], xv = x[n];
       v[4] += (R)xv.x * xv.x + (R)xv.y * xv.y;
-      v[0] += (R)e;
       const R wd = D ? (R)1 / (R)D[n] : (R)1;
       v[1] += ((R)gv.x * gv.x + (R)gv.y * gv.y) * wd;
       if (gp) { const float2 q = gp[n]; v[2] += ((R)q.x * gv.x + (R)q.y * gv.y) * wd; }
@@ -1208,12 +1387,29 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   e->kmax_p = kmax;
   e->nb_i = blocks_for(e->npix_p);
   e->nb_k = blocks_for(kmax);
+  {  // gradient tiles: as many frame groups as fit on the device at once (one round of workgroups;
+     // a partly filled second round costs a whole group's frames again)
+    e->tg_tiles_x = (e->S1 + TG_T - 1) / TG_T;
+    e->tg_ntiles = ((e->S0 + TG_T - 1) / TG_T) * e->tg_tiles_x;
+    int dev = 0, ncu = 0, per_cu = 0;
+    int64_t resident = 2048;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        (e->fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_grad_tiled<float>, RT, 0)
+                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_grad_tiled<double>, RT, 0)) == hipSuccess &&
+        ncu > 0 && per_cu > 0)
+      resident = (int64_t)ncu * per_cu;
+    const int64_t want = std::max<int64_t>(1, resident / ((int64_t)e->tg_ntiles * e->P));
+    const int ng = (int)std::min<int64_t>(e->F, want);
+    e->tg_fg = (e->F + ng - 1) / ng;
+    e->nb_g = e->tg_ntiles * ((e->F + e->tg_fg - 1) / e->tg_fg);
+  }
   hipError_t err;
 #define ALLOC(ptr, bytes)                                                         \
   err = hipMalloc((void**)&(ptr), (bytes));                                      \
   if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: hipMalloc failed: %s", hipGetErrorString(err)); return MRH_ENOMEM; }
   ALLOC(e->d_kprob, sizeof(int64_t) * (e->P + 1));
-  ALLOC(e->d_part_i, sizeof(double) * (size_t)e->P * e->nb_i * PI_STRIDE);
+  ALLOC(e->d_part_i, sizeof(double) * (size_t)e->P * std::max(e->nb_i, e->nb_g) * PI_STRIDE);
   ALLOC(e->d_part_k, sizeof(double) * (size_t)e->P * e->nb_k * PK_STRIDE);
   ALLOC(e->d_st, sizeof(ProbState) * e->P);
   ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
@@ -1357,11 +1553,11 @@ extern "C" int mrh_cg_gradient(mrh_cg* e, const void* x, void* g, const void* d,
                        e->gl, e->S0 * e->S1, e->npix_p, e->d_st, prime, e->d_part_q);
     MRH_CHECK_LAUNCH();
   }
-  LAUNCH_R(e, k_cg_grad, dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
+  LAUNCH_R(e, k_cg_grad_tiled, dim3(e->nb_g, e->P), dim3(RT), 0, st, (const float2*)x, (float2*)g,
                      (const float2*)d, (const float2*)g_prev, (const float*)e->d_diag, e->tv, e->npix_p, e->d_st,
-                     prime, e->d_part_i);
+                     prime, e->tg_tiles_x, e->tg_ntiles, e->tg_fg, e->d_part_i);
   MRH_CHECK_LAUNCH();
-  LAUNCH_R(e, k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, prime,
+  LAUNCH_R(e, k_cg_beta, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_g, prime,
                      e->direction, e->rel_tol, e->restart_c, e->d_target, e->d_hist, e->max_hist, e->P,
                      e->d_active, part_q, e->nb_q);
   MRH_CHECK_LAUNCH();

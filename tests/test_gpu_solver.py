@@ -70,6 +70,44 @@ def test_tv_energy_grad_matches_oracle(iso):
         assert rel_l2(g.cpu().numpy()[p * F:(p + 1) * F], gp) <= 2e-6
 
 
+@pytest.mark.parametrize("P,F,S0,S1", [(1, 34, 64, 48), (2, 3, 16, 20), (3, 1, 37, 23), (1, 7, 128, 96)])
+def test_engine_gradient_equals_standalone_tv(P, F, S0, S1):
+    """The solver's gradient pass (16 x 16 tiles walking their frames, each flux evaluated once) adds to g
+    the bits of the standalone TV kernel (every backward flux re-evaluated from memory) for temporal,
+    spatial iso / aniso and a weighted mixed-axes term, on tiles cut by the image edge, one frame, several
+    problems and several frame groups; its energy partials sum to the same TV energy."""
+    lib = _lib.load()
+    regs = [TVND("t", TVParams(weight=0.02, axes="temporal")),
+            TVND("s", TVParams(weight=0.01, axes="spatial", isotropic=True)),
+            TVND("w", TVParams(weight=0.005, axes=(0, 2, 3), isotropic=False, axis_weights=[0.5, 1.0, 2.0]))]
+    regm = RegManager(regs)
+    x = c64(crand((P * F, S0, S1), 7 + F))
+    g0 = crand((P * F, S0, S1), 8 + F)
+    g_ref = c64(g0)
+    e_ref = regm.energy_and_grad_tensors(x, g_ref, problems=P).cpu().numpy()
+    desc = _lib.CGDesc()
+    desc.tv = regm.tv_desc(P, F, S0, S1)
+    k_prob = np.zeros(P + 1, dtype=np.int64)
+    desc.k_prob = k_prob.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+    desc.direction, desc.c1, desc.ls_beta, desc.ls_max_iter, desc.max_hist = 0, 1e-4, 0.5, 20, 4
+    desc.restart_c, desc.ls_kind = -1.0, 0
+    h = ctypes.c_void_p()
+    _lib.check(lib.mrh_cg_create(ctypes.byref(desc), ctypes.byref(h)))
+    try:
+        g = c64(g0)           # the first gradient of a solve (prime)
+        _lib.check(lib.mrh_cg_gradient(h, x.data_ptr(), g.data_ptr(), None, None, 1, _lib.stream_of(g)))
+        g2 = c64(g0)          # an iteration's gradient: records the TV energy in the history
+        _lib.check(lib.mrh_cg_gradient(h, x.data_ptr(), g2.data_ptr(), None, None, 0, _lib.stream_of(g)))
+        buf = (_lib.CGHist * P)()
+        _lib.check(lib.mrh_cg_read_history(h, buf, 1, _lib.stream_of(g)))
+        torch.cuda.synchronize()
+    finally:
+        lib.mrh_cg_destroy(h)
+    assert torch.equal(g, g_ref) and torch.equal(g2, g_ref)
+    freg = np.array([buf[p].freg for p in range(P)])
+    assert np.all(np.abs(freg - e_ref) <= 1e-9 * np.abs(e_ref)), (freg, e_ref)
+
+
 def small_dynamic(seed=21, F=5, S=64, C=4, spokes=13, R=128):
     traj, dcf, off = synth.radial_frames([spokes] * F, R)
     x = synth.dynamic_phantom(S, F, seed=seed, n_ellipses=6)
