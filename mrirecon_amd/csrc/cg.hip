@@ -23,6 +23,7 @@
 #define RED_BLOCKS 2048   // blocks of the standalone reductions (8 per CU)
 #define NT 21             // line-search candidates: t = 0, 1, beta, ..., beta^19
 #define NT_A 6            // candidates of the first image pass: t = 0, 1, ..., beta^4
+#define TG_T 16           // side of the pixel tiles of the gradient and line-search image passes
 #define PI_STRIDE 32      // doubles per block partial (image pass)
 #define PK_STRIDE 4       // doubles per block partial (k-space pass)
 #define NW 7              // strong-Wolfe candidates per image pass
@@ -209,7 +210,13 @@ __device__ __forceinline__ float2 fdiff(const float2* __restrict__ v, int64_t n,
 // its own pixel and of its forward neighbours, and every caller (the standalone TV kernel, which
 // evaluates it again at the neighbour, and the engine's tiled gradient, which shares it through
 // LDS) then gets the same rounded values.
+// The denominators sqrt(|z|^2 + eps^2) and their reciprocals come from the hardware square root and
+// reciprocal square root (v_sqrt_f32, v_rsq_f32: 1 ulp each) instead of the correctly rounded
+// sqrtf and division, about 25 instructions fewer per flux; 1 / max(den, 1e-12) as
+// rsq(max(den^2, 1e-24)).
 __device__ __forceinline__ float abs2_plus(float2 v, float s) { return fmaf(v.x, v.x, fmaf(v.y, v.y, s)); }
+__device__ __forceinline__ float hw_sqrt(float t) { return __builtin_amdgcn_sqrtf(t); }
+__device__ __forceinline__ float hw_rsq_guarded(float t) { return __builtin_amdgcn_rsqf(fmaxf(t, 1e-24f)); }
 
 __device__ __forceinline__ float flux_of(const float2 (&dv)[3], const TVTermDev& tm, float2 (&p)[3]) {
 #pragma clang fp contract(off)
@@ -219,18 +226,18 @@ __device__ __forceinline__ float flux_of(const float2 (&dv)[3], const TVTermDev&
   if (tm.iso) {
 #pragma unroll
     for (int a = 0; a < 3; ++a) ss = abs2_plus(dv[a], ss);
-    const float den = sqrtf(ss + e2);
-    e = den - tm.eps;
-    const float inv = 1.f / fmaxf(den, 1e-12f);
+    const float t = ss + e2;
+    e = hw_sqrt(t) - tm.eps;
+    const float inv = hw_rsq_guarded(t);
 #pragma unroll
     for (int a = 0; a < 3; ++a) p[a] = make_float2(dv[a].x * inv, dv[a].y * inv);
   } else {
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
       if (tm.axes & (1 << a)) {
-        const float den = sqrtf(abs2_plus(dv[a], e2));
-        e += den - tm.eps;
-        const float inv = 1.f / fmaxf(den, 1e-12f);
+        const float t = abs2_plus(dv[a], e2);
+        e += hw_sqrt(t) - tm.eps;
+        const float inv = hw_rsq_guarded(t);
         p[a] = make_float2(dv[a].x * inv, dv[a].y * inv);
       } else {
         p[a] = make_float2(0.f, 0.f);
@@ -416,7 +423,9 @@ struct mrh_cg {
   double c1, ls_beta, rel_tol, restart_c, c2;
   int64_t npix_p, kmax_p;
   int nb_i, nb_k;
-  int tg_tiles_x, tg_ntiles, tg_fg, nb_g;   // k_cg_grad_tiled: tiles per row, per frame, frames per group, blocks
+  int tg_tiles_x, tg_ntiles;   // tiled passes: tiles per row, per frame
+  int tg_fg, nb_g;             // k_cg_grad_tiled: frames per group, blocks per problem
+  int tl_fg, nb_l;             // k_ls_image: frames per group, blocks per problem
   TVDev tv;
   int64_t* d_kprob;
   double* d_part_i;   // [P][nb_i][PI_STRIDE]
@@ -594,15 +603,24 @@ __global__ __launch_bounds__(RT) void k_ls_kspace(const float2* __restrict__ Ax,
 // Pass 2 of the line search: Re<g, d>, |d|^2 (first pass) and TV(x + t_j d) for the
 // candidates j in [J0, J1).  The first pass covers t = 0 and the NT_A - 1 largest
 // steps, which is where Armijo almost always stops; the second pass covers the rest
-// and does work only for problems whose first pass found no step.  Each candidate's
-// sum is accumulated exactly as in a single pass over all candidates.
+// and does work only for problems whose first pass found no step.  Both passes cut the
+// frames the same way, so each candidate's sum is accumulated exactly as in a single
+// pass over all candidates.
+// 16 x 16-pixel tiles walk their frames in order: x and d of frames f and f + 1 over the
+// tile and one row / column after it (the forward differences) are staged in LDS, the
+// next frame's stage (and, first pass, g of the next frame) in flight while the current
+// one is evaluated.  Grid (ntiles * ngroups, P): block b runs tile b % ntiles over
+// frames [fg (b / ntiles), + fg).
+#define TL_H (TG_T + 1)          // staged side: the tile and one after
 template <int J0, int J1, typename R>
 __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, const float2* __restrict__ d,
                                                  const float2* __restrict__ g, TVDev tv, int64_t npix_p,
-                                                 double ls_beta, const ProbState* __restrict__ st, double* part) {
+                                                 double ls_beta, const ProbState* __restrict__ st, int tiles_x,
+                                                 int ntiles, int fg, double* part) {
   constexpr bool FIRST = J0 == 0;
   constexpr int NV = (FIRST ? 2 : 0) + (J1 - J0), V0 = FIRST ? 2 : 0;
-  const int p = blockIdx.y;
+  __shared__ float2 sv[2][2][TL_H * TL_H];     // [x, d][frame parity][staged square]
+  const int p = blockIdx.y, tid = threadIdx.x;
   if (!FIRST && !(st[p].status == MRH_CG_ACTIVE && st[p].ls_pending)) return;   // partials unread: skip
   R v[NV];
 #pragma unroll
@@ -615,53 +633,109 @@ __global__ __launch_bounds__(RT) void k_ls_image(const float2* __restrict__ x, c
     for (int j = 1; j < NT; ++j) { tj[j] = (float)t; t *= ls_beta; }
   }
   if (st[p].status == MRH_CG_ACTIVE && (FIRST || st[p].ls_pending)) {
-    const int plane = tv.S0 * tv.S1;
-    for (int64_t i = blockIdx.x * (int64_t)RT + threadIdx.x; i < npix_p; i += (int64_t)gridDim.x * RT) {
-      const int f = (int)(i / plane), rem = (int)(i - (int64_t)f * plane);
-      const int ix = rem / tv.S1, iy = rem - ix * tv.S1;
-      const int64_t n = (int64_t)p * npix_p + i;
-      if (FIRST) {
-        const float2 dn = d[n], gn = g[n];
-        v[0] += (R)gn.x * dn.x + (R)gn.y * dn.y;
-        v[1] += (R)dn.x * dn.x + (R)dn.y * dn.y;
+    const int tile = blockIdx.x % ntiles, grp = blockIdx.x / ntiles;
+    const int X0 = (tile / tiles_x) * TG_T, Y0 = (tile % tiles_x) * TG_T;
+    const int F = tv.F, S0 = tv.S0, S1 = tv.S1, plane = S0 * S1;
+    const int fa = grp * fg, fb = min(F, fa + fg);
+    const int lx = tid / TG_T, ly = tid % TG_T, ix = X0 + lx, iy = Y0 + ly;
+    const bool mine = ix < S0 && iy < S1;
+    const int c = lx * TL_H + ly;                // this pixel in the staged square
+    const int64_t base = (int64_t)p * npix_p;
+    // x and d of frame f at elements tid and tid + RT of the staged square (zero outside the volume)
+    auto fetch = [&](int f, float2 (&r)[2][2]) {
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int i = tid + j * RT, row = i / TL_H, col = i - row * TL_H;
+        const int gx = X0 + row, gy = Y0 + col;
+        const bool in = i < TL_H * TL_H && f < F && gx < S0 && gy < S1;
+        const int64_t n = base + (int64_t)f * plane + (int64_t)gx * S1 + gy;
+        r[0][j] = in ? x[n] : make_float2(0.f, 0.f);
+        r[1][j] = in ? d[n] : make_float2(0.f, 0.f);
       }
-      for (int k = 0; k < tv.n; ++k) {
-        const TVTermDev& tm = tv.t[k];
-        if (tm.lam == 0.f || tm.axes == 0) continue;
-        float2 dx[3], dd[3];
+    };
+    auto put = [&](int par, const float2 (&r)[2][2]) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a) {
-          dx[a] = make_float2(0.f, 0.f); dd[a] = make_float2(0.f, 0.f);
-          if (tm.axes & (1 << a)) {
-            dx[a] = fdiff(x, n, f, ix, iy, a, tv, tm.w[a]);
-            dd[a] = fdiff(d, n, f, ix, iy, a, tv, tm.w[a]);
-          }
+      for (int q = 0; q < 2; ++q) {
+        sv[q][par][tid] = r[q][0];
+        if (tid + RT < TL_H * TL_H) sv[q][par][tid + RT] = r[q][1];
+      }
+    };
+    float2 nx[2][2];
+    fetch(fa, nx);
+    put(0, nx);
+    fetch(fa + 1, nx);
+    float2 ng = make_float2(0.f, 0.f);
+    const int64_t n0 = base + (int64_t)ix * S1 + iy;
+    if (FIRST && mine && fa < fb) ng = g[n0 + (int64_t)fa * plane];
+    int cur = 0;
+    for (int f = fa; f < fb; ++f) {
+      const float2 gn = ng;
+      __syncthreads();                   // frame f - 1's stage has been read
+      put(cur ^ 1, nx);                  // frame f + 1
+      if (f + 2 <= fb) fetch(f + 2, nx);
+      if (FIRST && mine && f + 1 < fb) ng = g[n0 + (int64_t)(f + 1) * plane];
+      __syncthreads();
+      if (mine) {
+        const float2 *x0 = sv[0][cur], *x1 = sv[0][cur ^ 1], *d0 = sv[1][cur], *d1 = sv[1][cur ^ 1];
+        if (FIRST) {
+          const float2 dn = d0[c];
+          v[0] += (R)gn.x * dn.x + (R)gn.y * dn.y;
+          v[1] += (R)dn.x * dn.x + (R)dn.y * dn.y;
         }
-        const float e2 = tm.eps * tm.eps;
+        const float2 xq = x0[c], dq = d0[c];
+        float ej[J1 - J0];               // sum over terms of lambda * energy density, per candidate
 #pragma unroll
-        for (int j = J0; j < J1; ++j) {
-          const float t = tj[j];
-          float e = 0.f;
-          if (tm.iso) {
-            float ss = 0.f;
+        for (int j = 0; j < J1 - J0; ++j) ej[j] = 0.f;
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              const float zx = fmaf(t, dd[a].x, dx[a].x), zy = fmaf(t, dd[a].y, dx[a].y);
-              ss = fmaf(zx, zx, fmaf(zy, zy, ss));
-            }
-            e = sqrtf(ss + e2) - tm.eps;
-          } else {
+        for (int k = 0; k < MRH_TV_MAX_TERMS; ++k) {
+          if (k >= tv.n) break;
+          const TVTermDev& tm = tv.t[k];
+          if (tm.lam == 0.f || tm.axes == 0) continue;
+          // weighted forward differences of x and d (fdiff's arithmetic)
+          float2 dx[3], dd[3];
 #pragma unroll
-            for (int a = 0; a < 3; ++a) {
-              if (tm.axes & (1 << a)) {
+          for (int a = 0; a < 3; ++a) {
+            dx[a] = make_float2(0.f, 0.f); dd[a] = make_float2(0.f, 0.f);
+            if (!(tm.axes & (1 << a))) continue;
+            int o;
+            const float2 *xs, *ds;
+            if (a == 0) { if (f + 1 >= F) continue; o = c; xs = x1; ds = d1; }
+            else if (a == 1) { if (ix + 1 >= S0) continue; o = c + TL_H; xs = x0; ds = d0; }
+            else { if (iy + 1 >= S1) continue; o = c + 1; xs = x0; ds = d0; }
+            const float w = tm.w[a];
+            const float2 xu = xs[o], du = ds[o];
+            dx[a] = make_float2(w * (xu.x - xq.x), w * (xu.y - xq.y));
+            dd[a] = make_float2(w * (du.x - dq.x), w * (du.y - dq.y));
+          }
+          const float e2 = tm.eps * tm.eps;
+#pragma unroll
+          for (int j = J0; j < J1; ++j) {
+            const float t = tj[j];
+            float e = 0.f;
+            if (tm.iso) {
+              float ss = 0.f;
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
                 const float zx = fmaf(t, dd[a].x, dx[a].x), zy = fmaf(t, dd[a].y, dx[a].y);
-                e += sqrtf(fmaf(zx, zx, zy * zy) + e2) - tm.eps;
+                ss = fmaf(zx, zx, fmaf(zy, zy, ss));
+              }
+              e = hw_sqrt(ss + e2) - tm.eps;
+            } else {
+#pragma unroll
+              for (int a = 0; a < 3; ++a) {
+                if (tm.axes & (1 << a)) {
+                  const float zx = fmaf(t, dd[a].x, dx[a].x), zy = fmaf(t, dd[a].y, dx[a].y);
+                  e += hw_sqrt(fmaf(zx, zx, zy * zy) + e2) - tm.eps;
+                }
               }
             }
+            ej[j - J0] = fmaf(tm.lam, e, ej[j - J0]);
           }
-          v[V0 + j - J0] += (R)(tm.lam * e);
         }
+#pragma unroll
+        for (int j = 0; j < J1 - J0; ++j) v[V0 + j] += (R)ej[j];
       }
+      cur ^= 1;
     }
   }
   block_reduce_store<NV, R>(v, part + ((int64_t)p * gridDim.x + blockIdx.x) * PI_STRIDE);
@@ -816,17 +890,17 @@ __global__ __launch_bounds__(RT) void k_wolfe_image(const float2* __restrict__ x
               ss = fmaf(zx, zx, fmaf(zy, zy, ss));
               zd = fmaf(zx, du[a].x, fmaf(zy, du[a].y, zd));
             }
-            const float den = sqrtf(ss + e2);
-            e = den - tm.eps;
-            sl = zd / fmaxf(den, 1e-12f);
+            const float q = ss + e2;
+            e = hw_sqrt(q) - tm.eps;
+            sl = zd * hw_rsq_guarded(q);
           } else {
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
               if (tm.axes & (1 << a)) {
                 const float zx = fmaf(t, dw[a].x, dx[a].x), zy = fmaf(t, dw[a].y, dx[a].y);
-                const float den = sqrtf(fmaf(zx, zx, zy * zy) + e2);
-                e += den - tm.eps;
-                sl += fmaf(zx, du[a].x, zy * du[a].y) / fmaxf(den, 1e-12f);
+                const float q = fmaf(zx, zx, zy * zy) + e2;
+                e += hw_sqrt(q) - tm.eps;
+                sl += fmaf(zx, du[a].x, zy * du[a].y) * hw_rsq_guarded(q);
               }
             }
           }
@@ -1031,7 +1105,6 @@ __global__ __launch_bounds__(RT) void k_accept_k(float2* __restrict__ Ax, const 
 // Grid (ntiles * ngroups, P): block b runs tile b % ntiles over frames [fg (b / ntiles), + fg) (the
 // group re-reads x at its first frame - 1 for that frame's temporal flux); the groups are as many as
 // fit on the device at once.
-#define TG_T 16                  // tile side
 #define TG_H (TG_T + 2)          // staged side: one before, one after
 #define TG_P (TG_T + 1)          // flux side: one before
 template <typename R>
@@ -1339,6 +1412,23 @@ static int blocks_for(int64_t n) {
   return (int)std::max<int64_t>(1, std::min<int64_t>(2048, b));
 }
 
+// Frame groups of a tiled pass: as many as fit on the device at once (one round of workgroups; a
+// partly filled second round costs a whole group's frames again).  fg = frames per group,
+// nb = workgroups per problem.
+template <typename K>
+static void tile_frame_groups(K kernel, int ntiles, int P, int F, int* fg, int* nb) {
+  int dev = 0, ncu = 0, per_cu = 0;
+  int64_t resident = 2048;
+  if (hipGetDevice(&dev) == hipSuccess &&
+      hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, RT, 0) == hipSuccess && ncu > 0 && per_cu > 0)
+    resident = (int64_t)ncu * per_cu;
+  const int64_t want = std::max<int64_t>(1, resident / ((int64_t)ntiles * P));
+  const int ng = (int)std::min<int64_t>(F, want);
+  *fg = (F + ng - 1) / ng;
+  *nb = ntiles * ((F + *fg - 1) / *fg);
+}
+
 extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   MRH_REQUIRE(d && out && d->k_prob, MRH_EINVAL, "cg_create: null argument");
   *out = nullptr;
@@ -1387,29 +1477,23 @@ extern "C" int mrh_cg_create(const mrh_cg_desc* d, mrh_cg** out) {
   e->kmax_p = kmax;
   e->nb_i = blocks_for(e->npix_p);
   e->nb_k = blocks_for(kmax);
-  {  // gradient tiles: as many frame groups as fit on the device at once (one round of workgroups;
-     // a partly filled second round costs a whole group's frames again)
-    e->tg_tiles_x = (e->S1 + TG_T - 1) / TG_T;
-    e->tg_ntiles = ((e->S0 + TG_T - 1) / TG_T) * e->tg_tiles_x;
-    int dev = 0, ncu = 0, per_cu = 0;
-    int64_t resident = 2048;
-    if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        (e->fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_grad_tiled<float>, RT, 0)
-                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_cg_grad_tiled<double>, RT, 0)) == hipSuccess &&
-        ncu > 0 && per_cu > 0)
-      resident = (int64_t)ncu * per_cu;
-    const int64_t want = std::max<int64_t>(1, resident / ((int64_t)e->tg_ntiles * e->P));
-    const int ng = (int)std::min<int64_t>(e->F, want);
-    e->tg_fg = (e->F + ng - 1) / ng;
-    e->nb_g = e->tg_ntiles * ((e->F + e->tg_fg - 1) / e->tg_fg);
+  // tiled passes (gradient, line-search image): 16 x 16 tiles, frames split into as many groups as
+  // fit on the device at once
+  e->tg_tiles_x = (e->S1 + TG_T - 1) / TG_T;
+  e->tg_ntiles = ((e->S0 + TG_T - 1) / TG_T) * e->tg_tiles_x;
+  if (e->fp32) {
+    tile_frame_groups(k_cg_grad_tiled<float>, e->tg_ntiles, e->P, e->F, &e->tg_fg, &e->nb_g);
+    tile_frame_groups(k_ls_image<0, NT_A, float>, e->tg_ntiles, e->P, e->F, &e->tl_fg, &e->nb_l);
+  } else {
+    tile_frame_groups(k_cg_grad_tiled<double>, e->tg_ntiles, e->P, e->F, &e->tg_fg, &e->nb_g);
+    tile_frame_groups(k_ls_image<0, NT_A, double>, e->tg_ntiles, e->P, e->F, &e->tl_fg, &e->nb_l);
   }
   hipError_t err;
 #define ALLOC(ptr, bytes)                                                         \
   err = hipMalloc((void**)&(ptr), (bytes));                                      \
   if (err != hipSuccess) { mrh_cg_destroy(e); mrh_set_error("cg_create: hipMalloc failed: %s", hipGetErrorString(err)); return MRH_ENOMEM; }
   ALLOC(e->d_kprob, sizeof(int64_t) * (e->P + 1));
-  ALLOC(e->d_part_i, sizeof(double) * (size_t)e->P * std::max(e->nb_i, e->nb_g) * PI_STRIDE);
+  ALLOC(e->d_part_i, sizeof(double) * (size_t)e->P * std::max({e->nb_i, e->nb_g, e->nb_l}) * PI_STRIDE);
   ALLOC(e->d_part_k, sizeof(double) * (size_t)e->P * e->nb_k * PK_STRIDE);
   ALLOC(e->d_st, sizeof(ProbState) * e->P);
   ALLOC(e->d_hist, sizeof(mrh_cg_hist) * (size_t)e->P * e->max_hist);
@@ -1510,21 +1594,25 @@ extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const 
     }
     return MRH_OK;
   }
-  if (e->fp32) hipLaunchKernelGGL((k_ls_image<0, NT_A, float>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
-                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
-  else hipLaunchKernelGGL((k_ls_image<0, NT_A, double>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
-                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
+  if (e->fp32) hipLaunchKernelGGL((k_ls_image<0, NT_A, float>), dim3(e->nb_l, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->tg_tiles_x,
+                     e->tg_ntiles, e->tl_fg, e->d_part_i);
+  else hipLaunchKernelGGL((k_ls_image<0, NT_A, double>), dim3(e->nb_l, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->tg_tiles_x,
+                     e->tg_ntiles, e->tl_fg, e->d_part_i);
   MRH_CHECK_LAUNCH();
-  LAUNCH_R(e, k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_i,
+  LAUNCH_R(e, k_ls_select, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->d_part_k, e->nb_l,
                      e->nb_k, e->c1, e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active,
                      part_q, e->nb_q, gl_lam);
   MRH_CHECK_LAUNCH();
-  if (e->fp32) hipLaunchKernelGGL((k_ls_image<NT_A, NT, float>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
-                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
-  else hipLaunchKernelGGL((k_ls_image<NT_A, NT, double>), dim3(e->nb_i, e->P), dim3(RT), 0, st, (const float2*)x,
-                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->d_part_i);
+  if (e->fp32) hipLaunchKernelGGL((k_ls_image<NT_A, NT, float>), dim3(e->nb_l, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->tg_tiles_x,
+                     e->tg_ntiles, e->tl_fg, e->d_part_i);
+  else hipLaunchKernelGGL((k_ls_image<NT_A, NT, double>), dim3(e->nb_l, e->P), dim3(RT), 0, st, (const float2*)x,
+                     (const float2*)d, (const float2*)g, e->tv, e->npix_p, e->ls_beta, e->d_st, e->tg_tiles_x,
+                     e->tg_ntiles, e->tl_fg, e->d_part_i);
   MRH_CHECK_LAUNCH();
-  LAUNCH_R(e, k_ls_select_rest, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_i, e->c1,
+  LAUNCH_R(e, k_ls_select_rest, dim3(e->P), dim3(RT), 0, st, e->d_st, e->d_part_i, e->nb_l, e->c1,
                      e->ls_beta, e->ls_max_iter, e->d_hist, e->max_hist, e->P, e->d_active, gl_lam);
   MRH_CHECK_LAUNCH();
   return MRH_OK;
