@@ -444,6 +444,106 @@ __global__ __launch_bounds__(256, 2) void k_conv_tile(const u16* __restrict__ x,
   }
 }
 
+// ------------------------------------------------------------------ temporal layers walking frames
+// 3-tap temporal convolution with Cin = 32 NCH and 16 MT-cout blocks.  A temporal tap reads the
+// same pixel in the previous, current and next frame, so there is no spatial halo: every MFMA B
+// fragment (16 bytes of one pixel's channels per lane) is loaded straight from the channels-last
+// input into a ring of registers that holds frames f - 1 .. f + PD, and each wave walks a group of
+// frames of its 16 NT pixels: an input frame is loaded once per wave instead of once per tap, and
+// the next PD frames are in flight while frame f is multiplied.  The block's [tap][16 MT couts][Cin]
+// weights are staged in LDS once.  Per output the MFMAs run chunk by chunk, taps inside a chunk,
+// with zero fragments beyond the volume's first / last frame and the same epilogue (+ bias,
+// optional ReLU, round to BF16), as k_conv_tile<.., TEMP = true>: the results are the same bits.
+// Grid (nvol * ngroups * npb * ncb), WPB waves: block b runs cout block b % ncb of pixel block
+// (b / ncb) % npb (16 NT WPB pixels, 16 NT per wave) over frames [fg grp, + fg) of its volume.
+template <int MT, int NT, int NCH, int PD, int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_conv_t3_walk(const u16* __restrict__ x, const u16* __restrict__ w,
+                                                      const float* __restrict__ bias, u16* __restrict__ out, int F,
+                                                      int HW, int Cout, int Cout_pad, int relu, int fg, int npb,
+                                                      int ncb) {
+  constexpr int BC = 16 * MT, CIN = 32 * NCH, CP = CIN + 8, RS = 2 + PD;
+  __shared__ __align__(16) u16 sW[3 * BC * CP];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int r = lane & 15, kg = lane >> 4;
+  const int b = blockIdx.x;
+  const int cb = b % ncb, pb = (b / ncb) % npb, vg = b / (ncb * npb);
+  const int ng = (F + fg - 1) / fg, vol = vg / ng, grp = vg - vol * ng;
+  const int co0 = cb * BC, fa = grp * fg, fb = min(F, fa + fg);
+  for (int i = tid; i < 3 * BC * (CIN / 8); i += 64 * WPB) {
+    const int row = i / (CIN / 8), q = i - row * (CIN / 8);   // row = tap * BC + cout
+    const int t = row / BC, c = row - t * BC;
+    *reinterpret_cast<u32x4*>(sW + row * CP + 8 * q) =
+        *reinterpret_cast<const u32x4*>(w + ((size_t)t * Cout_pad + co0 + c) * CIN + 8 * q);
+  }
+  const int px = (pb * WPB + wave) * 16 * NT + r;            // this lane's pixel in MFMA column 0
+  const u16* xv = x + ((size_t)vol * F * HW + px) * CIN + 8 * kg;
+  bf16x8 ring[RS][NT][NCH];
+  auto fetch = [&](int f, bf16x8 (&v)[NT][NCH]) {
+    const bool in = (unsigned)f < (unsigned)F;
+#pragma unroll
+    for (int h = 0; h < NT; ++h)
+#pragma unroll
+      for (int c = 0; c < NCH; ++c)
+        v[h][c] = in ? load8(xv + ((size_t)f * HW + 16 * h) * CIN + 32 * c) : zero8();
+  };
+#pragma unroll
+  for (int s = 0; s < RS - 1; ++s) fetch(fa - 1 + s, ring[s]);
+  __syncthreads();
+  for (int f = fa; f < fb; ++f) {
+    if (f + PD <= fb) fetch(f + PD, ring[RS - 1]);
+    f32x4 acc[MT][NT];
+#pragma unroll
+    for (int m = 0; m < MT; ++m)
+#pragma unroll
+      for (int h = 0; h < NT; ++h) acc[m][h] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < NCH; ++c)
+#pragma unroll
+      for (int t = 0; t < 3; ++t) {
+        bf16x8 a[MT];
+#pragma unroll
+        for (int m = 0; m < MT; ++m) a[m] = load8(sW + (t * BC + 16 * m + r) * CP + 32 * c + 8 * kg);
+#pragma unroll
+        for (int m = 0; m < MT; ++m)
+#pragma unroll
+          for (int h = 0; h < NT; ++h)
+            acc[m][h] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[m], ring[t][h][c], acc[m][h], 0, 0, 0);
+        __builtin_amdgcn_sched_barrier(0);   // keep each step's weight loads beside its MFMAs (registers)
+      }
+    // lane holds couts co0 + 16 m + 4 kg + j of pixel px + 16 h
+#pragma unroll
+    for (int h = 0; h < NT; ++h) {
+      const size_t p = ((size_t)vol * F + f) * HW + px + 16 * h;
+#pragma unroll
+      for (int m = 0; m < MT; ++m) {
+        const int cbase = co0 + 16 * m + 4 * kg;
+        if (cbase >= Cout) continue;
+        u16 o[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          float v = acc[m][h][j] + ((bias && cbase + j < Cout) ? bias[cbase + j] : 0.f);
+          if (relu) v = fmaxf(v, 0.f);
+          o[j] = f2bf(v);
+        }
+        if (cbase + 3 < Cout) {
+          *reinterpret_cast<uint2*>(out + p * Cout + cbase) =
+              make_uint2((unsigned)o[0] | ((unsigned)o[1] << 16), (unsigned)o[2] | ((unsigned)o[3] << 16));
+        } else {
+#pragma unroll
+          for (int j = 0; j < 4; ++j)
+            if (cbase + j < Cout) out[p * Cout + cbase + j] = o[j];
+        }
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < RS - 1; ++s)
+#pragma unroll
+      for (int h = 0; h < NT; ++h)
+#pragma unroll
+        for (int c = 0; c < NCH; ++c) ring[s][h][c] = ring[s + 1][h][c];
+  }
+}
+
 // ------------------------------------------------------------------ backward-weight
 // Grid (nchunks, Cout_pad / 32, ntap * ceil(Cin / 32)), 256 threads.  Block
 // computes dW[tap][32 couts][32 cins] over pixels [chunk * bx, +chunk) in
@@ -1039,6 +1139,38 @@ static int make_geom(int kind, int F, int H, int W, TapGeom* g) {
 
 #define MRH_TRY(call) do { int _rc = (call); if (_rc) return _rc; } while (0)
 
+// Temporal layers with Cin = 64, 128 or 256 on k_conv_t3_walk: 8 waves per workgroup (64-cout blocks,
+// 32 for Cin = 256 so that the register ring fits), frame groups of at least 5 frames (a group loads
+// its first frame - 1 and last frame + 1 again) and about 512 workgroups, two per CU.  Measured on
+// the 34 x 256^2 volume against 4 waves and 3-frame groups, 16 waves, and 256 - 2048 workgroups.
+// Returns false for shapes it does not cover.
+static bool launch_t3_walk(const u16* x, const u16* w, const float* bias, u16* out, int nvol, int F, int HW,
+                           int Cin, int Cout, int Cout_pad, int relu, hipStream_t st) {
+  constexpr int WPB = 8, TARGET_BLOCKS = 512, FG_MIN = 5;
+  int mt, nt;
+  if (Cin == 64) { mt = 4; nt = 2; }
+  else if (Cin == 128) { mt = 4; nt = 1; }
+  else if (Cin == 256) { mt = 2; nt = 1; }
+  else return false;
+  if (Cout_pad % (16 * mt) != 0 || HW % (16 * nt * WPB) != 0) return false;
+  const int npb = HW / (16 * nt * WPB), ncb = Cout_pad / (16 * mt);
+  const int64_t per_group = (int64_t)nvol * npb * ncb;
+  const int ng_max = std::max(1, (F + FG_MIN - 1) / FG_MIN);
+  const int ng = (int)std::max<int64_t>(1, std::min<int64_t>(ng_max, (TARGET_BLOCKS + per_group - 1) / per_group));
+  const int fg = (F + ng - 1) / ng, ngroups = (F + fg - 1) / fg;
+  const int64_t nblocks = per_group * ngroups;
+  if (nblocks > INT32_MAX) return false;
+  const dim3 grid((unsigned)nblocks);
+#define MRH_T3W(MT, NT, NCH, PD)                                                                               \
+  hipLaunchKernelGGL((k_conv_t3_walk<MT, NT, NCH, PD, WPB>), grid, dim3(64 * WPB), 0, st, x, w, bias, out, F, HW, \
+                     Cout, Cout_pad, relu, fg, npb, ncb)
+  if (Cin == 64) MRH_T3W(4, 2, 2, 2);
+  else if (Cin == 128) MRH_T3W(4, 1, 4, 2);
+  else MRH_T3W(2, 1, 8, 1);
+#undef MRH_T3W
+  return true;
+}
+
 extern "C" int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias, void* out, int kind, int nvol,
                                  int F, int H, int W, int Cin, int Cout, int Cout_pad, int relu, void* stream) {
   MRH_REQUIRE(x && w && out, MRH_EINVAL, "unet_conv_fwd: null buffer");
@@ -1061,6 +1193,10 @@ extern "C" int mrh_unet_conv_fwd(const void* x, const void* w, const float* bias
   hipLaunchKernelGGL((k_conv_fwd_lds<MT, NT, KC>),                                                        \
                      dim3((unsigned)((npix + 64 * NT - 1) / (64 * NT)), Cout_pad / (16 * MT)), dim3(256), 0, st, \
                      xs, ws, bias, os, g, (int)npix, Cin, Cout, Cout_pad, relu)
+  if (kind == MRH_UNET_TAP_T3 && launch_t3_walk(xs, ws, bias, os, nvol, F, H * W, Cin, Cout, Cout_pad, relu, st)) {
+    MRH_CHECK_LAUNCH();
+    return MRH_OK;
+  }
   const int tw = W >= 64 && W % 64 == 0 ? 64 : (W == 32 ? 32 : 0);
   const bool tile3 = kind == MRH_UNET_TAP_3X3 && tw && H % (256 / tw) == 0;
   // temporal tiles pay off with 64-cout blocks; 32-cout temporal layers stay on the register tiles

@@ -108,6 +108,33 @@ def test_conv_tiles(kind, cin, cout, relu, geom):
     check_bf16_close(out.float().cpu().numpy().astype(np.float64), ref)
 
 
+@pytest.mark.parametrize("geom", [(1, 34, 16, 16), (2, 7, 32, 32), (1, 1, 16, 16), (2, 2, 64, 64)])
+@pytest.mark.parametrize("cin,cout,relu,bias", [(64, 64, True, True), (64, 40, False, True), (128, 128, True, False),
+                                                (128, 64, False, True), (256, 256, True, True), (256, 72, False, False)])
+def test_temporal_layers_walking_frames(cin, cout, relu, bias, geom):
+    """Temporal layers with Cin = 64, 128 or 256 (the network's levels 1-3): every wave walks a group
+    of frames holding the frames before, at and after the current one in registers.  Groups of 5
+    frames (34 frames: the last group shorter, and every group boundary inside the volume), one
+    frame, two volumes, couts that end inside a cout block, with and without bias."""
+    lib = _lib.load()
+    nvol, F, H, W = geom
+    npix = nvol * F * H * W
+    rng = np.random.default_rng(cin * 11 + cout + F + H)
+    x = bf16_np(rng.standard_normal((nvol * F, H, W, cin)))
+    Wt = bf16_np(rng.standard_normal((3, cout, cin)) * np.sqrt(2.0 / (3 * cin)))
+    b = (rng.standard_normal(cout) * 0.1).astype(np.float32) if bias else None
+    cpad = (cout + 31) // 32 * 32
+    Wp = np.zeros((3, cpad, cin))
+    Wp[:, :cout] = Wt
+    out = torch.full((npix, cout), 7.0, dtype=torch.bfloat16, device=DEV)
+    xd, wd = to_dev_bf16(x), to_dev_bf16(Wp)
+    bd = torch.from_numpy(b).to(DEV) if bias else None
+    _lib.check(lib.mrh_unet_conv_fwd(xd.data_ptr(), wd.data_ptr(), bd.data_ptr() if bias else None,
+                                     out.data_ptr(), ou.TAP_T3, nvol, F, H, W, cin, cout, cpad, int(relu), st()))
+    ref = oracle_conv(x, Wt, None if b is None else b.astype(np.float64), ou.TAP_T3, nvol, F, relu).reshape(npix, cout)
+    check_bf16_close(out.float().cpu().numpy().astype(np.float64), ref)
+
+
 @pytest.mark.parametrize("kind,cin,cout", [(ou.TAP_3X3, 64, 32), (ou.TAP_T3, 32, 32), (ou.TAP_1X1, 256, 128),
                                            (ou.TAP_1X1, 32, 8), (ou.TAP_3X3, 32, 64)])
 def test_conv_backward_data(kind, cin, cout):
