@@ -1256,6 +1256,12 @@ __global__ __launch_bounds__(256) void k3_interp(const float2* __restrict__ grid
   }
   const float2* gc = grid + (int64_t)c * G0 * G1 * G2;
   float2 acc = make_float2(0.f, 0.f);
+  // A footprint row's W points are contiguous along axis 2 unless the row wraps round the grid.
+  // With G2 even (rows start 16-byte aligned) such a run is read as 16-byte pairs from the even
+  // index at or below its first point: ceil((W + sh) / 2) loads instead of W.  The last pair ends
+  // at most at index m2 + W, which is inside the row (m2 + W is then odd and at most G2 - 1).
+  const int m2 = s.m[2], a2 = m2 & ~1, sh = m2 - a2;
+  const bool run = (G2 & 1) == 0 && m2 + W <= G2;
   for (int i = 0; i < W; ++i) {
     int i0 = s.m[0] + i; if (i0 >= G0) i0 -= G0;
     for (int j = 0; j < W; ++j) {
@@ -1263,11 +1269,32 @@ __global__ __launch_bounds__(256) void k3_interp(const float2* __restrict__ grid
       const float2* row = gc + ((int64_t)i0 * G1 + i1) * G2;
       const float wij = w0[i] * w1[j];
       float2 r = make_float2(0.f, 0.f);
-      for (int l = 0; l < W; ++l) {
-        int i2 = s.m[2] + l; if (i2 >= G2) i2 -= G2;
-        const float2 gv = row[i2];
-        r.x = fmaf(gv.x, w2[l], r.x);
-        r.y = fmaf(gv.y, w2[l], r.y);
+      if (run) {
+        float2 gv[MRH_WMAX + 2];
+        const float4* rp = reinterpret_cast<const float4*>(row + a2);
+#pragma unroll
+        for (int q = 0; q < (MRH_WMAX + 2) / 2; ++q) {
+          if (2 * q < W + sh) {
+            const float4 v = rp[q];
+            gv[2 * q] = make_float2(v.x, v.y);
+            gv[2 * q + 1] = make_float2(v.z, v.w);
+          }
+        }
+#pragma unroll
+        for (int l = 0; l < MRH_WMAX; ++l) {
+          if (l < W) {
+            const float2 g = sh ? gv[l + 1] : gv[l];
+            r.x = fmaf(g.x, w2[l], r.x);
+            r.y = fmaf(g.y, w2[l], r.y);
+          }
+        }
+      } else {
+        for (int l = 0; l < W; ++l) {
+          int i2 = m2 + l; if (i2 >= G2) i2 -= G2;
+          const float2 gv = row[i2];
+          r.x = fmaf(gv.x, w2[l], r.x);
+          r.y = fmaf(gv.y, w2[l], r.y);
+        }
       }
       acc.x = fmaf(r.x, wij, acc.x);
       acc.y = fmaf(r.y, wij, acc.y);
