@@ -1323,15 +1323,28 @@ __device__ __forceinline__ void gather_range(const float2* __restrict__ yt, cons
   float2 acc[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
+  // a full group of 8 coils with C even starts 16 bytes aligned: read as four 16-byte pairs
+  const bool pairs = (C & 1) == 0 && c0 + 8 <= C;
   for (int e = e0; e < e1; ++e) {
     const float2* ys = yt + (int64_t)gk[e] * C + c0;
     const float w = gw[e];
+    float2 v[8];
+    if (pairs) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const float4 u = reinterpret_cast<const float4*>(ys)[q];
+        v[2 * q] = make_float2(u.x, u.y);
+        v[2 * q + 1] = make_float2(u.z, u.w);
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = c0 + q < C ? ys[q] : make_float2(0.f, 0.f);
+    }
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       if (c0 + q < C) {
-        const float2 v = ys[q];
-        acc[q].x = fmaf(w, v.x, acc[q].x);
-        acc[q].y = fmaf(w, v.y, acc[q].y);
+        acc[q].x = fmaf(w, v[q].x, acc[q].x);
+        acc[q].y = fmaf(w, v[q].y, acc[q].y);
       }
     }
   }
