@@ -1162,48 +1162,69 @@ static int free_plan3(mrh_nufft3_plan* p) {
   return MRH_OK;
 }
 
-// grid[c][g] = x[r] map_c[r] deap(r) where r = (g + S/2) mod G lies inside the volume, else 0.
+// The grid points that hold image samples along an axis: image index r sits at grid index
+// (r - S/2) mod G, i.e. [0, S - S/2) and [G - S/2, G).  Rows outside are the zero padding.
+struct AxisRows {
+  int e, b;   // grid index g is a row iff g < e || g >= b (e = G: every row)
+  __host__ __device__ bool has(int g) const { return g < e || g >= b; }
+  __host__ __device__ int nth(int i) const { return i < e ? i : b + (i - e); }   // i-th row in order
+};
+static AxisRows image_rows(int S, int G) { return AxisRows{S - S / 2, G - S / 2}; }
+static AxisRows all_rows(int G) { return AxisRows{G, G}; }
+
+// grid[c][g] = x[r] map_c[r] deap(r) at the grid points g = (r - S/2) mod G of the image samples r;
+// the zero padding is not written (the forward FFT passes never read it).
 __global__ __launch_bounds__(256) void k3_pad(const float2* __restrict__ x, const float2* __restrict__ maps,
                                               const float* __restrict__ d0, const float* __restrict__ d1,
                                               const float* __restrict__ d2, float2* __restrict__ grid, int S0,
                                               int S1, int S2, int G0, int G1, int G2, int C) {
-  const int64_t gvol = (int64_t)G0 * G1 * G2, n = (int64_t)C * gvol;
+  const int64_t svol = (int64_t)S0 * S1 * S2, n = (int64_t)C * svol;
   for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < n; t += (int64_t)gridDim.x * 256) {
-    const int c = (int)(t / gvol);
-    int64_t g = t - (int64_t)c * gvol;
-    const int g2 = (int)(g % G2); g /= G2;
-    const int g1 = (int)(g % G1);
-    const int g0 = (int)(g / G1);
-    const int r0 = (g0 + S0 / 2) % G0, r1 = (g1 + S1 / 2) % G1, r2 = (g2 + S2 / 2) % G2;
-    float2 v = make_float2(0.f, 0.f);
-    if (r0 < S0 && r1 < S1 && r2 < S2) {
-      const int64_t r = ((int64_t)r0 * S1 + r1) * S2 + r2;
-      const float2 xv = x[r], m = maps[(int64_t)c * S0 * S1 * S2 + r];
-      v = cscale(cmul(xv, m), d0[r0] * d1[r1] * d2[r2]);
-    }
-    grid[t] = v;
+    const int c = (int)(t / svol);
+    const int64_t r = t - (int64_t)c * svol;
+    int64_t q = r;
+    const int r2 = (int)(q % S2); q /= S2;
+    const int r1 = (int)(q % S1);
+    const int r0 = (int)(q / S1);
+    const int g0 = (r0 - S0 / 2 + G0) % G0, g1 = (r1 - S1 / 2 + G1) % G1, g2 = (r2 - S2 / 2 + G2) % G2;
+    const float2 xv = x[r], m = maps[t];
+    grid[(int64_t)c * G0 * G1 * G2 + ((int64_t)g0 * G1 + g1) * G2 + g2] = cscale(cmul(xv, m), d0[r0] * d1[r1] * d2[r2]);
   }
 }
 
-// In-place length-N DFT of every line of a (n_outer, N, n_inner) volume along its middle axis:
-// one wavefront per line (no block barriers), elements staged in a padded LDS buffer.
+// Which lines / tiles of a pass are transformed: the volume is (n_outer, N, n_inner); an outer index
+// o = c Ga + ga (tiles) or a line index o = (c Gb + gb) Ga + ga (lines along the last axis) is taken
+// for the rows ra x rb of (ga, gb) only (nA = ra's count; nB = rb's count, nB = Gb = 1: no second
+// filter), enumerated compactly.  in / out: the rows along the transformed axis that are read (the
+// others are zero) and written.
+struct Pass3 {
+  AxisRows ra, rb, in, out;
+  int Ga, Gb, nA, nB;
+  __device__ int64_t index(int64_t k) const {   // k-th taken outer (or line) index
+    const int64_t ia = k % nA, q = k / nA, ib = q % nB, c = q / nB;
+    return (c * Gb + rb.nth((int)ib)) * Ga + ra.nth((int)ia);
+  }
+};
+
+// In-place length-N DFT of lines along the last axis (n_inner = 1): one wavefront per line (no
+// block barriers), elements staged in a padded LDS buffer.
 template <int N, bool INV>
 __global__ __launch_bounds__(256) void k3_fft_lines(float2* __restrict__ v, const float2* __restrict__ tw0,
-                                                    int64_t n_inner, int64_t n_lines) {
+                                                    Pass3 ps, int64_t n_lines) {
   __shared__ float2 buf[4][fft_padded(N)];
   __shared__ float2 tw[N];
   for (int i = threadIdx.x; i < N; i += 256) tw[i] = tw0[i];
   __syncthreads();
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   float2* b = buf[wv];
-  for (int64_t line = (int64_t)blockIdx.x * 4 + wv; line < n_lines; line += (int64_t)gridDim.x * 4) {
-    const int64_t outer = line / n_inner, inner = line - outer * n_inner;
-    float2* base = v + outer * N * n_inner + inner;
-    for (int i = lane; i < N; i += 64) b[PIDX(i)] = base[(int64_t)i * n_inner];
+  for (int64_t k = (int64_t)blockIdx.x * 4 + wv; k < n_lines; k += (int64_t)gridDim.x * 4) {
+    float2* base = v + ps.index(k) * N;
+    for (int i = lane; i < N; i += 64) b[PIDX(i)] = ps.in.has(i) ? base[i] : make_float2(0.f, 0.f);
     fft_sync<true>();
     fft_lds_wave<N, INV>(b, tw, lane);
     fft_sync<true>();
-    for (int i = lane; i < N; i += 64) base[(int64_t)i * n_inner] = b[PIDX(i)];
+    for (int i = lane; i < N; i += 64)
+      if (ps.out.has(i)) base[i] = b[PIDX(i)];
     fft_sync<true>();
   }
 }
@@ -1214,7 +1235,7 @@ __global__ __launch_bounds__(256) void k3_fft_lines(float2* __restrict__ v, cons
 // staging), and its wavefronts transform the tile's lines in place.
 template <int N, bool INV, int TI>
 __global__ __launch_bounds__(256) void k3_fft_tiles(float2* __restrict__ v, const float2* __restrict__ tw0,
-                                                    int64_t n_inner, int64_t n_tiles) {
+                                                    Pass3 ps, int64_t n_inner, int64_t n_tiles) {
   constexpr int LS = fft_padded(N) + 1;
   __shared__ float2 tile[TI * LS];
   __shared__ float2 tw[N];
@@ -1222,19 +1243,19 @@ __global__ __launch_bounds__(256) void k3_fft_tiles(float2* __restrict__ v, cons
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t per_outer = n_inner / TI;
   for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
-    const int64_t outer = t / per_outer, inner0 = (t - outer * per_outer) * TI;
-    float2* base = v + outer * N * n_inner + inner0;
+    const int64_t ko = t / per_outer, inner0 = (t - ko * per_outer) * TI;
+    float2* base = v + ps.index(ko) * N * n_inner + inner0;
     __syncthreads();                       // twiddles staged / the previous tile written back
     for (int e = threadIdx.x; e < N * TI; e += 256) {
       const int i = e / TI, j = e - i * TI;
-      tile[j * LS + PIDX(i)] = base[(int64_t)i * n_inner + j];
+      tile[j * LS + PIDX(i)] = ps.in.has(i) ? base[(int64_t)i * n_inner + j] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     for (int j = wv; j < TI; j += 4) fft_lds_wave<N, INV>(tile + j * LS, tw, lane);
     __syncthreads();
     for (int e = threadIdx.x; e < N * TI; e += 256) {
       const int i = e / TI, j = e - i * TI;
-      base[(int64_t)i * n_inner + j] = tile[j * LS + PIDX(i)];
+      if (ps.out.has(i)) base[(int64_t)i * n_inner + j] = tile[j * LS + PIDX(i)];
     }
   }
 }
@@ -1410,18 +1431,32 @@ __global__ __launch_bounds__(256) void k3_crop(const float2* __restrict__ grid, 
 }
 
 template <bool INV>
-static int launch_fft_axis(int N, float2* v, const float2* tw, int64_t n_inner, int64_t n_lines, hipStream_t st) {
-  constexpr int TI = 16;
-  const bool tiled = n_inner % TI == 0;
-  const int64_t n_tiles = n_lines / TI;
-  const int blocks = tiled ? (int)std::max<int64_t>(1, std::min<int64_t>(65535, n_tiles))
-                           : (int)std::max<int64_t>(1, std::min<int64_t>(65535, (n_lines + 3) / 4));
+static int launch_fft_lines3(int N, float2* v, const float2* tw, const Pass3& ps, int64_t n_lines, hipStream_t st) {
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(65535, (n_lines + 3) / 4));
 #define MRH_FFT3_CASE(n)                                                                                   \
-  case n:                                                                                                 \
-    if (tiled)                                                                                            \
-      hipLaunchKernelGGL((k3_fft_tiles<n, INV, TI>), dim3(blocks), dim3(256), 0, st, v, tw, n_inner, n_tiles); \
-    else                                                                                                  \
-      hipLaunchKernelGGL((k3_fft_lines<n, INV>), dim3(blocks), dim3(256), 0, st, v, tw, n_inner, n_lines);  \
+  case n: hipLaunchKernelGGL((k3_fft_lines<n, INV>), dim3(blocks), dim3(256), 0, st, v, tw, ps, n_lines); break;
+  switch (N) {
+    MRH_FFT3_CASE(32) MRH_FFT3_CASE(64) MRH_FFT3_CASE(96) MRH_FFT3_CASE(128) MRH_FFT3_CASE(192)
+    MRH_FFT3_CASE(256) MRH_FFT3_CASE(384) MRH_FFT3_CASE(512) MRH_FFT3_CASE(768) MRH_FFT3_CASE(1024)
+    default: mrh_set_error("3-D grid size %d not built", N); return MRH_EUNSUP;
+  }
+#undef MRH_FFT3_CASE
+  MRH_CHECK_LAUNCH();
+  return MRH_OK;
+}
+
+// Passes along axes 0 and 1 run on 16-line tiles (every built grid size is a multiple of 16).
+template <bool INV>
+static int launch_fft_tiles3(int N, float2* v, const float2* tw, const Pass3& ps, int64_t n_inner, int64_t n_outer,
+                             hipStream_t st) {
+  constexpr int TI = 16;
+  MRH_REQUIRE(n_inner % TI == 0, MRH_EUNSUP, "3-D FFT pass: inner extent %lld is not a multiple of %d",
+              (long long)n_inner, TI);
+  const int64_t n_tiles = n_outer * (n_inner / TI);
+  const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(65535, n_tiles));
+#define MRH_FFT3_CASE(n)                                                                                      \
+  case n:                                                                                                    \
+    hipLaunchKernelGGL((k3_fft_tiles<n, INV, TI>), dim3(blocks), dim3(256), 0, st, v, tw, ps, n_inner, n_tiles); \
     break;
   switch (N) {
     MRH_FFT3_CASE(32) MRH_FFT3_CASE(64) MRH_FFT3_CASE(96) MRH_FFT3_CASE(128) MRH_FFT3_CASE(192)
@@ -1433,14 +1468,37 @@ static int launch_fft_axis(int N, float2* v, const float2* tw, int64_t n_inner, 
   return MRH_OK;
 }
 
-// FFT of every coil's grid along all three axes (grid layout (C, G0, G1, G2), G2 fastest).
+static int rows_count(const AxisRows& r, int G) { return r.e + (G - r.b); }
+
+// FFT of every coil's grid along all three axes (grid layout (C, G0, G1, G2), G2 fastest), pruned to
+// the rows that hold data.  Forward (k3_pad wrote only the image points): axis 2 over the lines whose
+// (g0, g1) are image rows, reading their image rows; axis 1 over the planes whose g0 is an image row,
+// reading image rows; axis 0 over everything, reading image rows; the zero padding is never read.
+// Inverse (the gather wrote every point), axes in the opposite order: axis 0 writes only the image
+// rows, axis 1 transforms those planes and writes image rows, axis 2 transforms the lines (g0, g1)
+// of image rows and writes the image rows that the crop reads.
 template <bool INV>
 static int fft3(const mrh_nufft3_plan* p, hipStream_t st) {
-  const int G0 = p->G[0], G1 = p->G[1], G2 = p->G[2];
-  const int64_t tot = (int64_t)p->C * G0 * G1 * G2;
-  int rc = launch_fft_axis<INV>(G2, p->d_grid, p->d_tw[2], 1, tot / G2, st);
-  if (!rc) rc = launch_fft_axis<INV>(G1, p->d_grid, p->d_tw[1], G2, tot / G1, st);
-  if (!rc) rc = launch_fft_axis<INV>(G0, p->d_grid, p->d_tw[0], (int64_t)G1 * G2, tot / G0, st);
+  const int G0 = p->G[0], G1 = p->G[1], G2 = p->G[2], C = p->C;
+  const AxisRows r0 = image_rows(p->S[0], G0), r1 = image_rows(p->S[1], G1), r2 = image_rows(p->S[2], G2);
+  const AxisRows one = all_rows(1);
+  const int n0 = rows_count(r0, G0), n1 = rows_count(r1, G1);
+  // axis 2: lines (c, g0 in r0, g1 in r1)
+  const Pass3 pa{r1, r0, INV ? all_rows(G2) : r2, INV ? r2 : all_rows(G2), G1, G0, n1, n0};
+  // axis 1: planes (c, g0 in r0), inner extent G2
+  const Pass3 pb{r0, one, INV ? all_rows(G1) : r1, INV ? r1 : all_rows(G1), G0, 1, n0, 1};
+  // axis 0: every coil, inner extent G1 G2
+  const Pass3 pc{one, one, INV ? all_rows(G0) : r0, INV ? r0 : all_rows(G0), 1, 1, 1, 1};
+  int rc;
+  if (!INV) {
+    rc = launch_fft_lines3<INV>(G2, p->d_grid, p->d_tw[2], pa, (int64_t)C * n0 * n1, st);
+    if (!rc) rc = launch_fft_tiles3<INV>(G1, p->d_grid, p->d_tw[1], pb, G2, (int64_t)C * n0, st);
+    if (!rc) rc = launch_fft_tiles3<INV>(G0, p->d_grid, p->d_tw[0], pc, (int64_t)G1 * G2, C, st);
+  } else {
+    rc = launch_fft_tiles3<INV>(G0, p->d_grid, p->d_tw[0], pc, (int64_t)G1 * G2, C, st);
+    if (!rc) rc = launch_fft_tiles3<INV>(G1, p->d_grid, p->d_tw[1], pb, G2, (int64_t)C * n0, st);
+    if (!rc) rc = launch_fft_lines3<INV>(G2, p->d_grid, p->d_tw[2], pa, (int64_t)C * n0 * n1, st);
+  }
   return rc;
 }
 
@@ -1622,10 +1680,10 @@ extern "C" int mrh_nufft3_plan_destroy(mrh_nufft3_plan* p) { return free_plan3(p
 extern "C" int mrh_nufft3_forward(mrh_nufft3_plan* p, const void* x, void* y, void* stream) {
   MRH_REQUIRE(p && x && y, MRH_EINVAL, "nufft3_forward: null argument");
   hipStream_t st = (hipStream_t)stream;
-  const int64_t svol = (int64_t)p->S[0] * p->S[1] * p->S[2], gtot = (int64_t)p->C * p->G[0] * p->G[1] * p->G[2];
+  const int64_t svol = (int64_t)p->S[0] * p->S[1] * p->S[2];
   for (int b = 0; b < p->B; ++b) {
     const int64_t k0 = p->h_koff[b], Kb = p->h_koff[b + 1] - k0;
-    hipLaunchKernelGGL(k3_pad, dim3(grid_blocks(gtot)), dim3(256), 0, st, (const float2*)x + b * svol, p->d_maps,
+    hipLaunchKernelGGL(k3_pad, dim3(grid_blocks((int64_t)p->C * svol)), dim3(256), 0, st, (const float2*)x + b * svol, p->d_maps,
                        p->d_deap[0], p->d_deap[1], p->d_deap[2], p->d_grid, p->S[0], p->S[1], p->S[2], p->G[0],
                        p->G[1], p->G[2], p->C);
     MRH_CHECK_LAUNCH();
