@@ -1140,8 +1140,7 @@ struct mrh_nufft3_plan {
   float2* d_grid;      // C G0 G1 G2
   // adjoint spreading as a gather: per frame, the footprint entries of every grid point
   int* d_goff;         // B (G0 G1 G2 + 1): entry range of grid point g of frame b (absolute)
-  int* d_gk;           // sample index within the frame
-  float* d_gw;         // KB weight x wAH
+  int2* d_gkw;         // (sample index within the frame, KB weight x wAH as float bits)
   float2* d_yt;        // one frame's k-space sample-major (K_b, C), read by the gather
   // grid points with more than GATHER_PART entries: their lists in parts of at most that many
   // (summed into d_part, then added in part order), so no thread walks a long list
@@ -1154,7 +1153,7 @@ struct mrh_nufft3_plan {
 static int free_plan3(mrh_nufft3_plan* p) {
   if (!p) return MRH_OK;
   void* bufs[] = {p->d_samp, p->d_deap[0], p->d_deap[1], p->d_deap[2], p->d_tw[0], p->d_tw[1], p->d_tw[2],
-                  p->d_maps, p->d_grid, p->d_goff, p->d_gk, p->d_gw, p->d_yt, p->d_heavy, p->d_parts,
+                  p->d_maps, p->d_grid, p->d_goff, p->d_gkw, p->d_yt, p->d_heavy, p->d_parts,
                   p->d_part};
   for (void* b : bufs) if (b) (void)hipFree(b);
   delete[] p->h_koff;
@@ -1337,18 +1336,19 @@ __global__ __launch_bounds__(256) void k3_transpose_y(const float2* __restrict__
 
 #define GATHER_PART 64   // longest entry list one gather thread walks
 
-// out[q * ostride] = sum over entries [e0, e1) of gw[e] yt[gk[e]][c0 + q] for the coils of one group.
-__device__ __forceinline__ void gather_range(const float2* __restrict__ yt, const int* __restrict__ gk,
-                                             const float* __restrict__ gw, int e0, int e1, int c0, int C,
-                                             float2* __restrict__ out, int64_t ostride) {
+// out[q * ostride] = sum over entries [e0, e1) of w_e yt[k_e][c0 + q] for the coils of one group,
+// (k_e, w_e) = gkw[e] read as one 8-byte pair.
+__device__ __forceinline__ void gather_range(const float2* __restrict__ yt, const int2* __restrict__ gkw, int e0,
+                                             int e1, int c0, int C, float2* __restrict__ out, int64_t ostride) {
   float2 acc[8];
 #pragma unroll
   for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
   // a full group of 8 coils with C even starts 16 bytes aligned: read as four 16-byte pairs
   const bool pairs = (C & 1) == 0 && c0 + 8 <= C;
   for (int e = e0; e < e1; ++e) {
-    const float2* ys = yt + (int64_t)gk[e] * C + c0;
-    const float w = gw[e];
+    const int2 kw = gkw[e];
+    const float2* ys = yt + (int64_t)kw.x * C + c0;
+    const float w = __int_as_float(kw.y);
     float2 v[8];
     if (pairs) {
 #pragma unroll
@@ -1374,27 +1374,27 @@ __device__ __forceinline__ void gather_range(const float2* __restrict__ yt, cons
     if (c0 + q < C) out[q * ostride] = acc[q];
 }
 
-// grid[c][g] = sum over the entries e of grid point g of gw[e] yt[gk[e]][c]; coils in groups of 8
+// grid[c][g] = sum over the entries e of grid point g of w_e yt[k_e][c]; coils in groups of 8
 // held in registers, so each entry is read once per group.  Points with more than GATHER_PART
 // entries are left to k3_gather_parts / k3_gather_heavy.
 __global__ __launch_bounds__(256) void k3_gather(const float2* __restrict__ yt, const int* __restrict__ goff,
-                                                 const int* __restrict__ gk, const float* __restrict__ gw,
-                                                 float2* __restrict__ grid, int64_t gvol, int C) {
+                                                 const int2* __restrict__ gkw, float2* __restrict__ grid,
+                                                 int64_t gvol, int C) {
   for (int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x; g < gvol; g += (int64_t)gridDim.x * 256) {
     const int e0 = goff[g], e1 = goff[g + 1];
     if (e1 - e0 > GATHER_PART) continue;
-    for (int c0 = 0; c0 < C; c0 += 8) gather_range(yt, gk, gw, e0, e1, c0, C, grid + (int64_t)c0 * gvol + g, gvol);
+    for (int c0 = 0; c0 < C; c0 += 8) gather_range(yt, gkw, e0, e1, c0, C, grid + (int64_t)c0 * gvol + g, gvol);
   }
 }
 
 // One part of a long list: part[i][c] = its entries' sum (parts of a frame, in list order).
 __global__ __launch_bounds__(256) void k3_gather_parts(const float2* __restrict__ yt, const int2* __restrict__ parts,
-                                                       const int* __restrict__ gk, const float* __restrict__ gw,
-                                                       float2* __restrict__ part, int n_parts, int C) {
+                                                       const int2* __restrict__ gkw, float2* __restrict__ part,
+                                                       int n_parts, int C) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n_parts) return;
   const int2 r = parts[i];
-  for (int c0 = 0; c0 < C; c0 += 8) gather_range(yt, gk, gw, r.x, r.y, c0, C, part + (int64_t)i * C + c0, 1);
+  for (int c0 = 0; c0 < C; c0 += 8) gather_range(yt, gkw, r.x, r.y, c0, C, part + (int64_t)i * C + c0, 1);
 }
 
 // A long list's grid point: the sum of its parts in order (deterministic).
@@ -1590,8 +1590,7 @@ extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan*
       return v;
     };
     std::vector<int> goff((size_t)(B * (gvol + 1)), 0);
-    std::vector<int> gk((size_t)std::max<int64_t>(1, n_ent));
-    std::vector<float> gw((size_t)std::max<int64_t>(1, n_ent));
+    std::vector<int2> gkw((size_t)std::max<int64_t>(1, n_ent));
     int64_t base = 0;
     std::vector<int> fill;
     for (int b = 0; b < B && !rc; ++b) {
@@ -1619,8 +1618,10 @@ extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan*
             for (int l = 0; l < W; ++l) {
               const int64_t g = gidx(s, i, j, l);
               const int e = fill[(size_t)g]++;
-              gk[(size_t)e] = (int)k;
-              gw[(size_t)e] = (float)(wij * kbw(s.u[2] - l) * s.wAH);
+              const float w = (float)(wij * kbw(s.u[2] - l) * s.wAH);
+              int wb;
+              std::memcpy(&wb, &w, sizeof(wb));
+              gkw[(size_t)e] = make_int2((int)k, wb);
             }
           }
         }
@@ -1654,8 +1655,7 @@ extern "C" int mrh_nufft3_plan_create(const mrh_nufft3_desc* d, mrh_nufft3_plan*
     for (int b = 0; b < B; ++b) kmax = std::max<int64_t>(kmax, p->h_koff[b + 1] - p->h_koff[b]);
     if (!rc) rc = upload<float2>(&p->d_yt, nullptr, (size_t)(kmax * d->C));
     if (!rc) rc = upload(&p->d_goff, goff.data(), goff.size());
-    if (!rc) rc = upload(&p->d_gk, gk.data(), gk.size());
-    if (!rc) rc = upload(&p->d_gw, gw.data(), gw.size());
+    if (!rc) rc = upload(&p->d_gkw, gkw.data(), gkw.size());
   }
   for (int a = 0; a < 3 && !rc; ++a) {
     std::vector<float> dv(p->S[a]);
@@ -1711,12 +1711,12 @@ extern "C" int mrh_nufft3_adjoint(mrh_nufft3_plan* p, const void* y, void* x, vo
       MRH_CHECK_LAUNCH();
     }
     hipLaunchKernelGGL(k3_gather, dim3(grid_blocks(gvol)), dim3(256), 0, st, (const float2*)p->d_yt,
-                       p->d_goff + (int64_t)b * (gvol + 1), p->d_gk, p->d_gw, p->d_grid, gvol, p->C);
+                       p->d_goff + (int64_t)b * (gvol + 1), p->d_gkw, p->d_grid, gvol, p->C);
     MRH_CHECK_LAUNCH();
     const int n_parts = (int)(p->h_poff[b + 1] - p->h_poff[b]), n_heavy = (int)(p->h_hoff[b + 1] - p->h_hoff[b]);
     if (n_heavy > 0) {
       hipLaunchKernelGGL(k3_gather_parts, dim3((n_parts + 255) / 256), dim3(256), 0, st, (const float2*)p->d_yt,
-                         p->d_parts + p->h_poff[b], p->d_gk, p->d_gw, p->d_part, n_parts, p->C);
+                         p->d_parts + p->h_poff[b], p->d_gkw, p->d_part, n_parts, p->C);
       MRH_CHECK_LAUNCH();
       hipLaunchKernelGGL(k3_gather_heavy, dim3((n_heavy + 255) / 256), dim3(256), 0, st, p->d_heavy + p->h_hoff[b],
                          (const float2*)p->d_part, p->d_grid, gvol, n_heavy, (int)p->h_poff[b], p->C);
