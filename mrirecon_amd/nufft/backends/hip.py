@@ -176,7 +176,7 @@ class Nufft3Plan:
 
     def __init__(self, *, S: Tuple[int, int, int], G: Tuple[int, int, int], W: int, C: int,
                  k_offsets: np.ndarray, traj_flat: np.ndarray, dcf_flat: Optional[np.ndarray],
-                 dcf_mode: str, dcf_in_fwd: bool, dcf_in_adj: bool, maps: torch.Tensor):
+                 dcf_mode: str, dcf_in_fwd: bool, dcf_in_adj: bool, maps: torch.Tensor, beta: float = 0.0):
         lib = _lib.load()
         self._lib = lib
         self.k_offsets = np.ascontiguousarray(k_offsets, dtype=np.int64)
@@ -188,7 +188,7 @@ class Nufft3Plan:
         d = _lib.Nufft3Desc()
         d.S0, d.S1, d.S2 = self.S
         d.G0, d.G1, d.G2 = self.G
-        d.W, d.beta, d.C, d.B = self.W, 0.0, self.C, self.B
+        d.W, d.beta, d.C, d.B = self.W, float(beta), self.C, self.B
         d.k_offsets = self.k_offsets.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
         d.traj = self.traj_flat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         d.dcf = (self.dcf_flat.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
@@ -225,6 +225,13 @@ class Nufft3Plan:
 
 def _grid_size(S: int, osf: float) -> int:
     return int(math.ceil(osf * S))
+
+
+def _kb_beta(W: int, osf: float) -> float:
+    """Kaiser-Bessel beta from the configured oversampling (Beatty et al. 2005), as
+    graspcg/nufft/backends/ptnufft.py:125 computes it: with a grid of ceil(osf S) points the
+    ratio G / S can differ from osf, and the window follows osf."""
+    return math.pi * math.sqrt(((W / float(osf)) * (float(osf) - 0.5)) ** 2 - 0.8)
 
 
 @dataclass
@@ -340,13 +347,15 @@ class HipNUFFTAdapter:
             return Nufft3Plan(S=(S0, S1, S2), G=tuple(_grid_size(v, self.osf) for v in (S0, S1, S2)),
                               W=self.kb_width, C=C, k_offsets=self._k_offsets, traj_flat=self._traj_flat,
                               dcf_flat=self._dcf_flat, dcf_mode=self.dcf_mode, dcf_in_fwd=self.apply_dcf_in_fwd,
-                              dcf_in_adj=self.apply_dcf_in_adj, maps=self._maps)
+                              dcf_in_adj=self.apply_dcf_in_adj, maps=self._maps,
+                              beta=_kb_beta(self.kb_width, self.osf))
         C, S0, S1 = self._im_shape
         return NufftPlan(S=(S0, S1), G=(_grid_size(S0, self.osf), _grid_size(S1, self.osf)),
                          W=self.kb_width, C=C, k_offsets=self._k_offsets, traj_flat=self._traj_flat,
                          dcf_flat=self._dcf_flat, dcf_mode=self.dcf_mode,
                          dcf_in_fwd=self.apply_dcf_in_fwd, dcf_in_adj=self.apply_dcf_in_adj,
-                         maps=self._maps, like=like, strip_cols=strip_cols)
+                         maps=self._maps, like=like, strip_cols=strip_cols,
+                         beta=_kb_beta(self.kb_width, self.osf))
 
     def _plan(self, like: int) -> NufftPlan:
         """The plan for ``like`` images per frame.  With ``autotune`` and ``strip_cols = 0`` every

@@ -165,6 +165,35 @@ def test_ragged_frames_parity(strip_cols):
     assert rel_l2(xa.cpu().numpy(), ora.adjoint(yr)) <= TOL
 
 
+@pytest.mark.parametrize("strip_cols", [8, 16])
+def test_odd_image_sizes_with_fractional_oversampling(strip_cols):
+    """osf 1.25 on a 25 x 51 image: grids of ceil(osf S) = 32 x 64 points (G / S is not osf, and the
+    KB beta must follow osf as ptnufft.py:125 computes it) and odd sizes, so S - S/2 and S/2 image
+    rows sit on the two sides of the grid origin in both passes."""
+    S0, S1, C, W, osf = 25, 51, 3, 6, 1.25
+    rng = np.random.default_rng(21)
+    Ks = [700, 450]
+    off = np.concatenate([[0], np.cumsum(Ks)]).astype(np.int64)
+    traj = rng.uniform(-math.pi, math.pi, size=(2, int(off[-1]))).astype(np.float32)
+    dcf = rng.uniform(0.2, 1.5, size=int(off[-1])).astype(np.float32)
+    maps = crand((C, S0, S1), 22)
+    x = crand((len(Ks), S0, S1), 23)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", dcf_mode="balanced", kb_width=W, osf=osf, strip_cols=strip_cols)
+    be.prepare((C, S0, S1), torch.from_numpy(traj), torch.from_numpy(dcf), c64(maps), device=DEV, k_offsets=off)
+    assert be.plan.G == (32, 64)
+    ora = KBNufft(maps, traj, off, dcf, W, osf)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(x), y)
+    yr = crand(be.plan.n_k, 24)
+    xa = torch.empty((len(Ks), S0, S1), dtype=torch.complex64, device=DEV)
+    be.plan.adjoint(c64(yr), xa)
+    torch.cuda.synchronize()
+    ef = rel_l2(y.cpu().numpy(), ora.forward(x))
+    ea = rel_l2(xa.cpu().numpy(), ora.adjoint(yr))
+    print(f"[2-D 25x51 osf 1.25, {strip_cols} columns] forward rel={ef:.2e} adjoint rel={ea:.2e}")
+    assert ef <= TOL and ea <= TOL
+
+
 def test_strip_width_autotune_and_unbuilt_widths():
     """The adapter times every strip width built for the plan (W = 6, G = 512: 8 and 16) and keeps
     the fastest; widths the kernels do not cover are refused at plan creation."""

@@ -15,24 +15,28 @@ pytestmark = pytest.mark.gpu
 DEV = torch.device("cuda:0")
 
 
-def problem(S, C, Ks, seed, W=6):
+def problem(S, C, Ks, seed, W=6, osf=2.0):
     rng = np.random.default_rng(seed)
     maps = ((rng.standard_normal((C,) + S) + 1j * rng.standard_normal((C,) + S)) * 0.5).astype(np.complex64)
     off = np.concatenate([[0], np.cumsum(Ks)]).astype(np.int64)
     traj = rng.uniform(-math.pi, math.pi, size=(3, int(off[-1]))).astype(np.float32)
     dcf = rng.uniform(0.2, 1.5, size=int(off[-1])).astype(np.float32)
     x = (rng.standard_normal((len(Ks),) + S) + 1j * rng.standard_normal((len(Ks),) + S)).astype(np.complex64)
-    be = HipNUFFTAdapter(ndim=3, traj_units="rad", kb_width=W, osf=2.0)
+    be = HipNUFFTAdapter(ndim=3, traj_units="rad", kb_width=W, osf=osf)
     be.prepare((C,) + S, torch.from_numpy(traj), torch.from_numpy(dcf), torch.from_numpy(maps).to(DEV),
                device=DEV, k_offsets=off)
-    ora = KBNufft3(maps, traj, off, dcf, W, 2.0)
+    ora = KBNufft3(maps, traj, off, dcf, W, osf)
     return be, ora, x, off
 
 
-@pytest.mark.parametrize("S,C,Ks,W", [((16, 16, 16), 3, [400, 700], 6), ((16, 32, 48), 2, [300, 0, 500], 6),
-                                      ((16, 16, 32), 2, [600], 4)])
-def test_nufft3_forward_adjoint_match_the_oracle(S, C, Ks, W):
-    be, ora, x, off = problem(S, C, Ks, seed=sum(S) + C, W=W)
+@pytest.mark.parametrize("S,C,Ks,W,osf", [((16, 16, 16), 3, [400, 700], 6, 2.0),
+                                          ((16, 32, 48), 2, [300, 0, 500], 6, 2.0),
+                                          ((16, 16, 32), 2, [600], 4, 2.0),
+                                          ((25, 51, 51), 2, [400, 300], 6, 1.25)])
+def test_nufft3_forward_adjoint_match_the_oracle(S, C, Ks, W, osf):
+    """Odd image sizes (osf 1.25: 25 x 51 x 51 on a 32 x 64 x 64 grid) put S - S/2 and S/2 image rows
+    on the two sides of the grid origin, the case the row-pruned FFT passes must get right."""
+    be, ora, x, off = problem(S, C, Ks, seed=sum(S) + C, W=W, osf=osf)
     xg = torch.from_numpy(x).to(DEV).reshape(len(Ks), 1, *S)
     y = be.A(xg)
     torch.cuda.synchronize()
@@ -44,7 +48,7 @@ def test_nufft3_forward_adjoint_match_the_oracle(S, C, Ks, W):
     torch.cuda.synchronize()
     xa_ref = ora.adjoint(yr.astype(np.complex128))
     ea = rel_l2(xa.cpu().numpy().reshape(len(Ks), *S), xa_ref)
-    print(f"[3-D {S} C={C} K={Ks} W={W}] forward rel={ef:.2e} adjoint rel={ea:.2e}")
+    print(f"[3-D {S} C={C} K={Ks} W={W} osf={osf}] forward rel={ef:.2e} adjoint rel={ea:.2e}")
     assert ef <= 1e-5 and ea <= 1e-5
     if 0 in Ks:                                   # an empty frame adjoints to zeros
         b = Ks.index(0)
