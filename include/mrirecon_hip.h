@@ -43,7 +43,7 @@ typedef struct {
   int S0, S1;               /* image size (maps.shape[1:])                            */
   int G0, G1;               /* oversampled grid size                                  */
   int W;                    /* Kaiser-Bessel width in grid cells (2..8)                */
-  double beta;              /* KB shape; <= 0 selects Beatty's formula                 */
+  double beta;              /* KB shape; <= 0: Beatty's formula at osf = G0 / S0       */
   int C;                    /* coils                                                   */
   int B;                    /* frames (batch of trajectories)                          */
   int like;                 /* largest Like-fold used at call time (>= 1)              */
@@ -131,7 +131,7 @@ typedef struct {
   int S0, S1, S2;           /* volume size (maps.shape[1:])                          */
   int G0, G1, G2;           /* grid size per axis (32 ... 1024, 2^a 3^b)               */
   int W;                    /* KB width (2 ... 8)                                      */
-  double beta;              /* KB beta; <= 0 selects Beatty's formula                   */
+  double beta;              /* KB beta; <= 0: Beatty's formula at osf = G0 / S0         */
   int C;                    /* coils                                                   */
   int B;                    /* frames                                                  */
   const int64_t* k_offsets; /* host (B+1): sample offsets of each frame                 */
