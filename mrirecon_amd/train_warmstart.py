@@ -184,8 +184,12 @@ def run_warm(pb: GraspProblem, net, target: Optional[float] = None) -> Dict[str,
     h = pb.history(long=target is not None)
     stop = eng.stop_reasons()[0]
     eng.set_targets(None)
+    # the fallback check may have decided by a lower bound of f(x_adj); the exact value is
+    # evaluated here, outside the timed region, for the report
+    f_adj = float(pb.objective_adj(pb.x_adj)[0])
     return {"seconds": t, "iterations": len(h), "f_final": h[-1]["f_step"] if h else float("nan"),
-            "f_start": float(f_w[0]), "f_adjoint": float(f_a[0]), "accepted": bool(acc[0]),
+            "f_start": float(f_w[0]), "f_adjoint": f_adj, "f_adjoint_used": float(f_a[0]),
+            "decided_by_bound": bool(f_a[0] < f_adj), "accepted": bool(acc[0]),
             "f_path": [r["f_step"] for r in h], "stop": stop,
             "adjoint_ms": ev[0].elapsed_time(ev[1]), "warm_start_ms": ev[1].elapsed_time(ev[2])}
 

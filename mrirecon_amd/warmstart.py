@@ -507,6 +507,37 @@ class ObjectiveEval:
         _lib.check(self.lib.mrh_tv_energy_grad(ctypes.byref(self.tv), x.data_ptr(), None, self.work.data_ptr(),
                                                self.e_tv.data_ptr(), st))
 
+    def launch_adjoint_bound(self, x_adj: torch.Tensor) -> None:
+        """Enqueue, for x_adj = A^H y, TV(x_adj) (into the TV terms) and the lower bound
+        ``self.lb`` of f(x_adj) per problem without a NUFFT: Re<A A^H y, y> = |A^H y|^2, so by
+        Cauchy-Schwarz 1/2 |A x_adj - y|^2 >= 1/2 (|y|^2 - |x_adj|^2)^2 / |y|^2."""
+        st = torch.cuda.current_stream(x_adj.device).cuda_stream
+        if not hasattr(self, "norms"):
+            self.norms = torch.zeros(2 * self.P, dtype=torch.float64, device=self.y.device)   # |y_p|^2, |x_p|^2
+        xf = x_adj.reshape(self.P, -1)
+        n_img = xf.shape[1]
+        for p in range(self.P):
+            a, b = self.k_prob[p], self.k_prob[p + 1]
+            _lib.check(self.lib.mrh_dot_real(self.y[a:].data_ptr(), self.y[a:].data_ptr(), b - a, self.work.data_ptr(),
+                                             self.norms[p:p + 1].data_ptr(), st))
+            _lib.check(self.lib.mrh_dot_real(xf[p].data_ptr(), xf[p].data_ptr(), n_img, self.work.data_ptr(),
+                                             self.norms[self.P + p:self.P + p + 1].data_ptr(), st))
+        _lib.check(self.lib.mrh_tv_energy_grad(ctypes.byref(self.tv), x_adj.data_ptr(), None, self.work.data_ptr(),
+                                               self.e_tv.data_ptr(), st))
+        yy, xx = self.norms[:self.P], self.norms[self.P:]
+        self.lb = 0.5 * (yy - xx) ** 2 / torch.clamp(yy, min=1e-300) + self.e_tv
+
+    def launch_data_masked(self, x: torch.Tensor, active: torch.Tensor) -> None:
+        """Enqueue the data terms 1/2 |A x - y|^2 of the problems with ``active[p]`` != 0 (device
+        int32[P]; the NUFFT skips the other problems' frames, whose data terms are then undefined)."""
+        st = torch.cuda.current_stream(x.device).cuda_stream
+        self.plan.forward_masked(x, self.Ax, active.data_ptr(), self.F)
+        for p in range(self.P):
+            a, b = self.k_prob[p], self.k_prob[p + 1]
+            _lib.check(self.lib.mrh_residual_energy(self.r[a:].data_ptr(), self.Ax[a:].data_ptr(),
+                                                    self.Ax[a:].data_ptr(), self.y[a:].data_ptr(), 0.0, b - a,
+                                                    self.work.data_ptr(), self.e_data[p:p + 1].data_ptr(), st))
+
     def read(self) -> np.ndarray:
         """f per problem of the last launched evaluation (synchronises with its stream)."""
         e = self.e.cpu().numpy()
@@ -590,28 +621,44 @@ class WarmStartChoice:
 
 
 def launch_warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveEval, x_adj: torch.Tensor,
-                      nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None) -> WarmStartChoice:
-    """Network + fallback check with f(x_adj) evaluated on a side stream while the network runs,
-    and the choice per problem made on the device, so nothing waits for the host after the
-    network's input scales are read (the solver's first iteration queues right behind).
+                      nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None,
+                      bound: bool = True) -> WarmStartChoice:
+    """Network + fallback check, the choice per problem made on the device, so nothing waits for
+    the host after the network's input scales are read (the solver's first iteration queues right
+    behind).
 
     ``objective`` and ``objective_adj`` are two evaluators of the same problem (separate
-    buffers).  The network (MFMA-bound) and the adjoint image's objective (a forward NUFFT,
-    memory / LDS-bound) overlap; f(x_warm) runs after both, since the two NUFFTs share the
-    plan's workspace.  Problem p starts from x_warm unless f(x_warm) > f(x_adj); its slice of
-    ``objective.Ax`` is then replaced by A x_adj from ``objective_adj``, so ``objective.Ax``
-    holds A x0 for every problem."""
+    buffers).  Problem p starts from x_warm unless f(x_warm) > f(x_adj); its slice of
+    ``objective.Ax`` is then replaced by A x_adj from ``objective_adj``, so ``objective.Ax`` holds
+    A x0 for every problem.
+
+    ``bound`` (default): while the network runs, a side stream computes TV(x_adj) and the lower
+    bound 1/2 (|y|^2 - |A^H y|^2)^2 / |y|^2 + TV(A^H y) <= f(x_adj) (no NUFFT).  Where
+    f(x_warm) <= (1 - 1e-4) x bound the start is decided without f(x_adj); A x_adj and the exact
+    f(x_adj) are computed (a masked forward NUFFT) only for the other problems, and f_adj reports
+    the bound for the decided ones.  ``bound=False``: f(x_adj) in full on the side stream for
+    every problem, overlapped with the network.  Both make the same choice."""
     main = torch.cuda.current_stream(x_adj.device)
     side = side_stream or torch.cuda.Stream(device=x_adj.device)
     ready = torch.cuda.Event()
     ready.record(main)
     with torch.cuda.stream(side):
         side.wait_event(ready)
-        objective_adj.launch(x_adj)
+        if bound:
+            objective_adj.launch_adjoint_bound(x_adj)
+        else:
+            objective_adj.launch(x_adj)
     x_warm = net.infer(x_adj, nvol=nvol)
     main.wait_stream(side)
     objective.launch(x_warm)
-    f_w, f_a = objective.value_device(), objective_adj.value_device()
+    f_w = objective.value_device()
+    if bound:
+        lb = objective_adj.lb
+        need = f_w > lb * (1.0 - 1e-4)                      # the bound does not decide these
+        objective_adj.launch_data_masked(x_adj, need.to(torch.int32))
+        f_a = torch.where(need, objective_adj.value_device(), lb)
+    else:
+        f_a = objective_adj.value_device()
     acc = f_w <= f_a                                       # device bool per problem
     x0 = x_warm if out is None else out
     P = objective.P
@@ -625,12 +672,14 @@ def launch_warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: Obj
 
 
 def warm_start(net: UNet2DT, objective: ObjectiveEval, objective_adj: ObjectiveEval, x_adj: torch.Tensor,
-               nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None
+               nvol: int = 1, out: Optional[torch.Tensor] = None, side_stream=None, bound: bool = True
                ) -> Tuple[torch.Tensor, np.ndarray, np.ndarray, np.ndarray]:
     """:func:`launch_warm_start` followed by the host read of its choice: returns
-    (x0, accepted[P] bool, f_warm[P], f_adj[P]), the result of
-    ``select_warm_start(objective, x_adj, net.infer(...))``; ``objective.Ax`` holds A x0."""
-    ch = launch_warm_start(net, objective, objective_adj, x_adj, nvol=nvol, out=out, side_stream=side_stream)
+    (x0, accepted[P] bool, f_warm[P], f_adj[P]), the choice of
+    ``select_warm_start(objective, x_adj, net.infer(...))`` (f_adj a lower bound where ``bound``
+    decided without it); ``objective.Ax`` holds A x0."""
+    ch = launch_warm_start(net, objective, objective_adj, x_adj, nvol=nvol, out=out, side_stream=side_stream,
+                           bound=bound)
     acc, f_w, f_a = ch.read()
     return ch.x0, acc, f_w, f_a
 
