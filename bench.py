@@ -688,7 +688,7 @@ def main():
     ap.add_argument("--tol-volumes", type=int, default=200,
                     help="held-out volumes of the time-to-tolerance runs (seeds 600000 + i)")
     ap.add_argument("--workers", type=int, default=16, help="processes generating the held-out phantoms")
-    ap.add_argument("--detail", default=os.path.join("gpurun_out", "bench_detail.json"),
+    ap.add_argument("--detail", default="bench_detail.json",
                     help="file for the full results (per-volume rows, autotuner times, ...)")
     ap.add_argument("--no-batched-tol", action="store_true",
                     help="skip the run of the same volumes as one batched problem set")
