@@ -195,6 +195,36 @@ __device__ __forceinline__ void fft_lds_wave(float2* buf, const float2* tw, int 
   FFTStages<N, 1, INV, 64, true>::run(buf, tw, lane);
 }
 
+// Length-512 transform owned by one wavefront whose lane holds rows lane + 64 r
+// (r < 8) in v, in and out.  With radices 8 x 8 x 8 the first pass's butterfly
+// `lane` reads exactly those rows and the last pass's butterfly `lane` writes
+// them, so the first pass runs on the registers and the last pass leaves its
+// results there: only the middle pass goes through LDS (one write and one read
+// of the row instead of five of each).  The operations are those of fft_lds_wave;
+// where the caller consumes the outputs in registers the compiler may contract
+// the last pass's multiplies into different FMAs, so results can differ from
+// fft_lds_wave's in the last bit (measured: 3e-8 relative on a GRASP adjoint).
+// The caller must not touch `buf` from other wavefronts meanwhile.
+template <bool INV>
+__device__ __forceinline__ void fft512_wave_regs(float2 (&v)[8], float2* buf, const float2* tw, int lane) {
+  constexpr int N = 512;
+  DFT<8, INV>::run(v);                       // pass 1 (NS = 1: no twiddles)
+  fft_sync<true>();                          // earlier reads of buf are done
+#pragma unroll
+  for (int r = 0; r < 8; ++r) buf[PIDX(lane * 8 + r)] = v[r];
+  fft_sync<true>();
+  fft_pass<N, 8, 8, INV, 64, true>(buf, tw, lane);   // pass 2 (NS = 8)
+#pragma unroll
+  for (int r = 0; r < 8; ++r) v[r] = buf[PIDX(lane + r * 64)];   // pass 3 (NS = 64), m = lane
+#pragma unroll
+  for (int r = 1; r < 8; ++r) {
+    float2 w = tw[fft_tw_index(64, r, lane)];
+    if (INV) w.y = -w.y;
+    v[r] = cmul(v[r], w);
+  }
+  DFT<8, INV>::run(v);
+}
+
 // Same transform of a column whose rows [a, b) are zero (the zero padding of a
 // centred, zero-padded image column): the first pass reads only rows outside
 // [a, b), so the caller neither writes nor clears the padding.

@@ -35,6 +35,7 @@
 #include <memory>
 #include <string>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 // ---------------------------------------------------------------- KB window
@@ -439,6 +440,23 @@ __global__ __launch_bounds__(256) void k_fwd_rows(
   __syncthreads();
   const int nstr = G1 / Tc;
   for (int c = 0; c < C; ++c) {
+    if constexpr (G1 == 512) {   // the lane's grid columns are its FFT rows: passes 1 and 3 in registers
+      float2 v[8];
+#pragma unroll
+      for (int q = 0; q < 8; ++q) v[q] = cmul(basev[q], fac[q]);
+      if (c + 1 < C) load_fac(c + 1);
+      fft512_wave_regs<false>(v, buf, stw, lane);
+      if (row_ok) {
+        float2* Tcf = T + (int64_t)(bb * C + c) * nstr * S0 * Tc;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const int pp = lane + 64 * q;
+          const int st = pp / Tc, j = pp - st * Tc;
+          Tcf[((int64_t)st * S0 + ix) * Tc + j] = v[q];
+        }
+      }
+      continue;
+    }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) {
       const int pp = lane + 64 * q;
@@ -899,6 +917,62 @@ __global__ __launch_bounds__(256) void k_adj_rows(
       if (row_ok && pp < S1) mp[q] = mr[pp];
     }
   };
+  if constexpr (G1 == 512) {
+    auto load_t = [&](int c) {
+      const float2* Tcf = T + (int64_t)(bb * C + c) * nstr * S0 * Tc;
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int pp = lane + 64 * q;
+        const int st = pp / Tc, j = pp - st * Tc;
+        tp[q] = row_ok ? Tcf[((int64_t)st * S0 + ix) * Tc + j] : make_float2(0.f, 0.f);
+      }
+    };
+    load_t(0);
+    __syncthreads();
+    // The lane's T columns are its FFT rows and, when S1 / 2 is a multiple of 64, the image
+    // columns it combines are its own output rows too (image column lane + 64 q is grid column
+    // lane + 64 ((q - S1 / 128) mod 8)): passes 1 and 3 stay in registers.
+    if (h1 % 64 == 0 && h1 <= 256) {
+      // only T is fetched a coil ahead; the map row is fetched during the transform
+      auto run = [&](auto shift) {
+        constexpr int SH = decltype(shift)::value;
+        for (int c = 0; c < C; ++c) {
+          float2 v[8], mc[NA];
+          const float2* mr = maps + ((int64_t)c * S0 + ix) * S1;
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            v[q] = tp[q];
+            const int iy = lane + 64 * q;
+            mc[q] = (row_ok && iy < S1) ? mr[iy] : make_float2(0.f, 0.f);
+          }
+          if (c + 1 < C) load_t(c + 1);
+          fft512_wave_regs<true>(v, buf, stw, lane);
+#pragma unroll
+          for (int q = 0; q < NA; ++q) {
+            const int iy = lane + 64 * q;
+            if (row_ok && iy < S1) acc[q] = cadd(acc[q], cmulc(v[(q - SH + 8) & 7], mc[q]));
+          }
+        }
+      };
+      switch (h1 / 64) {
+        case 0: run(std::integral_constant<int, 0>{}); break;
+        case 1: run(std::integral_constant<int, 1>{}); break;
+        case 2: run(std::integral_constant<int, 2>{}); break;
+        case 3: run(std::integral_constant<int, 3>{}); break;
+        default: run(std::integral_constant<int, 4>{}); break;
+      }
+      if (row_ok) {
+        const float dxi = dx[ix];
+        float2* xr = x + ((int64_t)bb * S0 + ix) * S1;
+#pragma unroll
+        for (int q = 0; q < NA; ++q) {
+          const int iy = lane + 64 * q;
+          if (iy < S1) xr[iy] = cscale(acc[q], dxi * dy[iy]);
+        }
+      }
+      return;
+    }
+  }
   load(0);
   __syncthreads();
   for (int c = 0; c < C; ++c) {
