@@ -194,6 +194,40 @@ def test_odd_image_sizes_with_fractional_oversampling(strip_cols):
     assert ef <= TOL and ea <= TOL
 
 
+@pytest.mark.parametrize("S0,S1,osf", [(16, 128, 4.0), (48, 384, 4.0 / 3.0), (32, 512, 1.0)])
+def test_row_transforms_on_registers_every_crop_shift(S0, S1, osf):
+    """512-point grid rows run their first and last FFT passes on registers, and the adjoint crops by
+    a register rotation of S1 / 128 (S1 = 256 is covered by the GRASP tests): image widths 128,
+    384 and 512 on a 512-column grid give rotations 1, 3 and 4, osf 1 leaves no zero padding.
+    At osf 1 the KB transform at the image edge is 2.6e-4 of its centre value (W = 6), so the
+    deapodisation multiplies FP32 rounding there by about 3800: the library matches the FP64
+    oracle to 3e-3 in that case, with or without the register passes, and the bound is 1e-2
+    (a wrong rotation gives errors of order 1)."""
+    C, W = 3, 6
+    rng = np.random.default_rng(31)
+    Ks = [600, 900]
+    off = np.concatenate([[0], np.cumsum(Ks)]).astype(np.int64)
+    traj = rng.uniform(-math.pi, math.pi, size=(2, int(off[-1]))).astype(np.float32)
+    dcf = rng.uniform(0.2, 1.5, size=int(off[-1])).astype(np.float32)
+    maps = crand((C, S0, S1), 32)
+    x = crand((len(Ks), S0, S1), 33)
+    be = HipNUFFTAdapter(ndim=2, traj_units="rad", dcf_mode="balanced", kb_width=W, osf=osf)
+    be.prepare((C, S0, S1), torch.from_numpy(traj), torch.from_numpy(dcf), c64(maps), device=DEV, k_offsets=off)
+    assert be.plan.G[1] == 512
+    ora = KBNufft(maps, traj, off, dcf, W, osf)
+    y = torch.empty(be.plan.n_k, dtype=torch.complex64, device=DEV)
+    be.plan.forward(c64(x), y)
+    yr = crand(be.plan.n_k, 34)
+    xa = torch.empty((len(Ks), S0, S1), dtype=torch.complex64, device=DEV)
+    be.plan.adjoint(c64(yr), xa)
+    torch.cuda.synchronize()
+    ef = rel_l2(y.cpu().numpy(), ora.forward(x))
+    ea = rel_l2(xa.cpu().numpy(), ora.adjoint(yr))
+    print(f"[2-D {S0}x{S1} osf {osf:.3f}, G {be.plan.G}] forward rel={ef:.2e} adjoint rel={ea:.2e}")
+    tol = TOL if osf > 1.0 else 1e-2
+    assert ef <= tol and ea <= tol
+
+
 def test_strip_width_autotune_and_unbuilt_widths():
     """The adapter times every strip width built for the plan (W = 6, G = 512: 8 and 16) and keeps
     the fastest; widths the kernels do not cover are refused at plan creation."""
