@@ -130,25 +130,31 @@ class NufftPlan:
         per coil).  Each is timed on scratch data with HIP events on the current stream
         (one warm-up pair, then the median of ``reps`` pairs); the fastest is kept and
         ``self.tuning`` records every time (ms per pair).  Results do not depend on the
-        choice."""
+        choice.  The inputs are fixed seeded Gaussian image and k-space data, each written to
+        its own output: fed back into each other, A^H A would grow them about tenfold per pair
+        and, over many candidates, to inf, which the adjoint's strips do not spread."""
         t_start = time.perf_counter()
         S0, S1 = self.S
         dev = self.maps.device
-        x = torch.ones((self.B * like, S0, S1), dtype=torch.complex64, device=dev)
-        y = torch.empty(self.n_k * like, dtype=torch.complex64, device=dev)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(1234)
+        x_in = torch.randn((self.B * like, S0, S1), dtype=torch.complex64, device=dev, generator=gen)
+        y_in = torch.randn(self.n_k * like, dtype=torch.complex64, device=dev, generator=gen)
+        x = torch.empty_like(x_in)
+        y = torch.empty_like(y_in)
         cands = sorted({-(-self.C // k) for k in range(1, self.C + 1)} if candidates is None
                        else {int(c) for c in candidates}, reverse=True)
         times: Dict[int, float] = {}
         for c in cands:
             self.set_coil_chunk(c)
-            self.forward(x, y, like)
-            self.adjoint(y, x, like)
+            self.forward(x_in, y, like)
+            self.adjoint(y_in, x, like)
             ts = []
             for _ in range(max(1, reps)):
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
                 e0.record()
-                self.forward(x, y, like)
-                self.adjoint(y, x, like)
+                self.forward(x_in, y, like)
+                self.adjoint(y_in, x, like)
                 e1.record()
                 e1.synchronize()
                 ts.append(e0.elapsed_time(e1))
