@@ -429,14 +429,14 @@ def nufft_rate(wl, dev, reps=5):
 
 def plan_setup(be, seconds):
     """Host-side plan cost of an adapter: prepare() wall time, and per strip width the C-ABI plan
-    creation (strip binning on host threads + uploads) and the coil-chunk autotune."""
+    creation (strip binning on host threads + uploads), and the coil-chunk autotune over all widths."""
     plan = be.plan
     t = plan.tuning or {}
     info = plan.info()
     return {"prepare_ms": round(seconds * 1e3, 1), "samples": int(info["n_samples"]),
             "footprint_entries": int(info["n_entries"]),
             "plan_create_ms_by_width": t.get("plan_create_ms_by_width", {int(info["strip_cols"]): round(plan.create_ms, 2)}),
-            "autotune_ms_by_width": t.get("autotune_ms_by_width")}
+            "autotune_ms": t.get("autotune_ms")}
 
 
 def nufft3_rate(dev, S=128, C=8, spokes=1024, R=256, reps=5):

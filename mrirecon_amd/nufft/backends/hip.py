@@ -28,7 +28,7 @@ import ctypes
 import math
 import time
 from dataclasses import dataclass, field
-from typing import Dict, Literal, Optional, Sequence, Tuple
+from typing import Dict, List, Literal, Optional, Sequence, Tuple
 
 import numpy as np
 import torch
@@ -123,45 +123,15 @@ class NufftPlan:
         _lib.check(self._lib.mrh_nufft_set_coil_chunk(self.handle, int(coils_per_block)))
 
     @torch.no_grad()
-    def autotune(self, reps: int = 3, like: int = 1, candidates: Optional[Sequence[int]] = None) -> Dict[int, float]:
+    def autotune(self, reps: int = 5, like: int = 1, candidates: Optional[Sequence[int]] = None) -> Dict[int, float]:
         """Pick the strip kernels' coils per workgroup by timing forward + adjoint pairs.
 
         Candidates are ceil(C / k) for k = 1..C (one workgroup per (frame, strip) down to one
-        per coil).  Each is timed on scratch data with HIP events on the current stream
-        (one warm-up pair, then the median of ``reps`` pairs); the fastest is kept and
-        ``self.tuning`` records every time (ms per pair).  Results do not depend on the
-        choice.  The inputs are fixed seeded Gaussian image and k-space data, each written to
-        its own output: fed back into each other, A^H A would grow them about tenfold per pair
-        and, over many candidates, to inf, which the adjoint's strips do not spread."""
+        per coil), timed as ``tune_coil_chunks`` describes; the fastest is kept and
+        ``self.tuning`` records every time (ms per pair).  Results do not depend on the choice."""
         t_start = time.perf_counter()
-        S0, S1 = self.S
-        dev = self.maps.device
-        gen = torch.Generator(device=dev)
-        gen.manual_seed(1234)
-        x_in = torch.randn((self.B * like, S0, S1), dtype=torch.complex64, device=dev, generator=gen)
-        y_in = torch.randn(self.n_k * like, dtype=torch.complex64, device=dev, generator=gen)
-        x = torch.empty_like(x_in)
-        y = torch.empty_like(y_in)
-        cands = sorted({-(-self.C // k) for k in range(1, self.C + 1)} if candidates is None
-                       else {int(c) for c in candidates}, reverse=True)
-        times: Dict[int, float] = {}
-        for c in cands:
-            self.set_coil_chunk(c)
-            self.forward(x_in, y, like)
-            self.adjoint(y_in, x, like)
-            ts = []
-            for _ in range(max(1, reps)):
-                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                e0.record()
-                self.forward(x_in, y, like)
-                self.adjoint(y_in, x, like)
-                e1.record()
-                e1.synchronize()
-                ts.append(e0.elapsed_time(e1))
-            times[c] = float(sorted(ts)[len(ts) // 2])
-        best = min(times, key=times.get)
-        self.set_coil_chunk(best)
-        self.tuning = {"coil_chunk": best, "ms_per_pair": times,
+        times = tune_coil_chunks([self], like=like, reps=reps, candidates=candidates)[0]
+        self.tuning = {"coil_chunk": self.coil_chunk, "ms_per_pair": times,
                        "autotune_ms": (time.perf_counter() - t_start) * 1e3}
         return times
 
@@ -227,6 +197,58 @@ class Nufft3Plan:
             self.close()
         except Exception:
             pass
+
+
+@torch.no_grad()
+def tune_coil_chunks(plans: Sequence["NufftPlan"], like: int = 1, reps: int = 5,
+                     candidates: Optional[Sequence[int]] = None) -> List[Dict[int, float]]:
+    """Time every (plan, coils per workgroup) candidate of ``plans`` (plans of one problem that
+    differ in strip width) and leave each plan at its fastest chunk; returns each plan's
+    {chunk: ms per pair}.
+
+    Every candidate runs one warm-up pair, then ``reps`` rounds time one pair of each candidate
+    in turn (HIP events on the current stream) and a candidate's time is its median.  Timed one
+    candidate after another instead, the GPU's clock ramp during the sweep favoured whichever
+    candidates ran last.  The inputs are fixed seeded Gaussian image and k-space data, each
+    written to its own output: fed back into each other, A^H A would grow them about tenfold per
+    pair and, over many candidates, to inf, which the adjoint's strips do not spread."""
+    p0 = plans[0]
+    S0, S1 = p0.S
+    dev = p0.maps.device
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(1234)
+    x_in = torch.randn((p0.B * like, S0, S1), dtype=torch.complex64, device=dev, generator=gen)
+    y_in = torch.randn(p0.n_k * like, dtype=torch.complex64, device=dev, generator=gen)
+    x = torch.empty_like(x_in)
+    y = torch.empty_like(y_in)
+    cand = []
+    for i, p in enumerate(plans):
+        cs = (sorted({-(-p.C // k) for k in range(1, p.C + 1)}, reverse=True) if candidates is None
+              else sorted({int(c) for c in candidates}, reverse=True))
+        cand += [(i, c) for c in cs]
+
+    def pair(i: int, c: int) -> None:
+        plans[i].set_coil_chunk(c)
+        plans[i].forward(x_in, y, like)
+        plans[i].adjoint(y_in, x, like)
+
+    for i, c in cand:
+        pair(i, c)
+    ts: Dict[Tuple[int, int], List[float]] = {ic: [] for ic in cand}
+    for _ in range(max(1, reps)):
+        for i, c in cand:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            pair(i, c)
+            e1.record()
+            e1.synchronize()
+            ts[(i, c)].append(e0.elapsed_time(e1))
+    out: List[Dict[int, float]] = [dict() for _ in plans]
+    for (i, c), v in ts.items():
+        out[i][c] = float(sorted(v)[len(v) // 2])
+    for i, p in enumerate(plans):
+        p.set_coil_chunk(min(out[i], key=out[i].get))
+    return out
 
 
 def _grid_size(S: int, osf: float) -> int:
@@ -382,7 +404,7 @@ class HipNUFFTAdapter:
         else:
             G0 = _grid_size(self._im_shape[1], self.osf)
             widths = [8] + ([16] if G0 <= 512 else []) + ([4] if self.kb_width == 4 else [])
-            best, times, create_ms, tune_ms = None, {}, {}, {}
+            qs, create_ms = [], {}
             t_setup = time.perf_counter()
             for tc in widths:
                 try:
@@ -391,21 +413,21 @@ class HipNUFFTAdapter:
                     if tc == widths[0]:           # the default width: the plan itself is invalid
                         raise
                     continue                      # a width this grid cannot use (G1 % tc, KB terms)
-                t = q.autotune(like=like)
-                times[tc] = t
+                qs.append((tc, q))
                 create_ms[tc] = round(q.create_ms, 2)
-                tune_ms[tc] = round(q.tuning["autotune_ms"], 2)
-                if best is None or min(t.values()) < min(best.tuning["ms_per_pair"].values()):
-                    if best is not None:
-                        best.close()
-                    best = q
-                else:
+            t_tune = time.perf_counter()
+            tuned = tune_coil_chunks([q for _, q in qs], like=like)
+            tune_ms = round((time.perf_counter() - t_tune) * 1e3, 2)
+            times = {tc: t for (tc, _), t in zip(qs, tuned)}
+            k = min(range(len(qs)), key=lambda j: min(tuned[j].values()))
+            p = qs[k][1]
+            for j, (_, q) in enumerate(qs):
+                if j != k:
                     q.close()
-            p = best
-            p.tuning = {"strip_cols": int(p.info()["strip_cols"]), "coil_chunk": p.tuning["coil_chunk"],
-                        "ms_per_pair": p.tuning["ms_per_pair"],
+            p.tuning = {"strip_cols": int(p.info()["strip_cols"]), "coil_chunk": p.coil_chunk,
+                        "ms_per_pair": dict(tuned[k]),
                         "ms_per_pair_by_width": {tc: dict(t) for tc, t in times.items()},
-                        "plan_create_ms_by_width": create_ms, "autotune_ms_by_width": tune_ms,
+                        "plan_create_ms_by_width": create_ms, "autotune_ms": tune_ms,
                         "setup_ms": (time.perf_counter() - t_setup) * 1e3}
         self._plans[like] = p
         return p
