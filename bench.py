@@ -59,16 +59,18 @@ def grid_pipeline_pair_bytes(wl, G):
     return tot
 
 
-def load_traffic(kernel, strip_cols):
-    """Measured HBM bytes per launch from the committed rocprofv3 PMC summary, for the strip width the
-    plan uses (the autotuner picks 8 or 16 columns), if present."""
+def load_traffic(kernel, strip_cols, coil_chunk):
+    """Measured HBM bytes per launch from the committed rocprofv3 PMC summary, for the strip width and
+    coils per strip workgroup the plan uses (the strip kernels' traffic depends on both: each coil group
+    re-reads the strip's footprint entries), if present."""
     p = os.path.join(ROOT, "profiles", "nufft_traffic_c2.json")
     if not os.path.exists(p):
         return None
     try:
         with open(p) as f:
             v = json.load(f).get(kernel)
-        return v.get(str(int(strip_cols))) if isinstance(v, dict) else v
+        v = v.get(str(int(strip_cols))) if isinstance(v, dict) else v
+        return v.get(str(int(coil_chunk))) if isinstance(v, dict) else v
     except Exception:
         return None
 
@@ -776,7 +778,7 @@ def main():
     kb = kernel_bytes(wl, info)
     dom = max(stage_ms, key=stage_ms.get)
     achieved = kb[dom] / (stage_ms[dom] * 1e-3) / 1e9
-    traffic = load_traffic(dom, info["strip_cols"])
+    traffic = load_traffic(dom, info["strip_cols"], info["coil_chunk"])
     roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "algorithmic_bytes": int(kb[dom]), "avg_ms": round(stage_ms[dom], 4)}
