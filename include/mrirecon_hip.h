@@ -294,7 +294,9 @@ int mrh_cg_residual(mrh_cg* eng, const void* Ax, const void* y, void* r, void* s
  * (doubling from t = 1, bisection zoom, lo end accepted after ls_max_iter bisections):
  * f'(t) = b + c t + Re<g_TV(x + t d), d>, with TV values and slopes at seven candidate
  * steps per image pass; the sequential search is replayed over the evaluated steps, so it
- * accepts the same t as the reference's one-trial-at-a-time search. */
+ * accepts the same t as the reference's one-trial-at-a-time search.
+ * y may be null: Ax then holds the residual r = Ax - y (as mrh_cg_residual and mrh_cg_accept
+ * leave it), which gives the same a and b bits without reading Ax and y. */
 int mrh_cg_linesearch(mrh_cg* eng, const void* x, const void* d, const void* g, const void* Ax,
                       const void* Ad, const void* y, void* stream);
 

@@ -452,6 +452,12 @@ struct mrh_cg {
     if ((e)->fp32) hipLaunchKernelGGL(K<float>, __VA_ARGS__);                          \
     else hipLaunchKernelGGL(K<double>, __VA_ARGS__);                                   \
   } while (0)
+// Same with a second (bool) template argument.
+#define LAUNCH_R2(e, K, B, ...)                                                        \
+  do {                                                                                 \
+    if ((e)->fp32) hipLaunchKernelGGL((K<float, B>), __VA_ARGS__);                     \
+    else hipLaunchKernelGGL((K<double, B>), __VA_ARGS__);                              \
+  } while (0)
 
 // g += lambda L x; block partial: E.
 __global__ __launch_bounds__(256) void k_gl_grad(const float2* __restrict__ x, float2* __restrict__ g, GLDev gl,
@@ -580,8 +586,10 @@ __global__ __launch_bounds__(RT) void k_cg_residual(const float2* __restrict__ A
   }
 }
 
-// Pass 1 of the line search: a = |Ax - y|^2, b = Re<Ax - y, Ad>, c = |Ad|^2 per problem.
-template <typename R>
+// Pass 1 of the line search: a = |Ax - y|^2, b = Re<Ax - y, Ad>, c = |Ad|^2 per problem.  RES: Ax
+// already holds the residual Ax - y (as k_cg_residual and k_accept_k leave it, with the same FP32
+// subtraction), so y is not read and the sums are the same bits.
+template <typename R, bool RES>
 __global__ __launch_bounds__(RT) void k_ls_kspace(const float2* __restrict__ Ax, const float2* __restrict__ Ad,
                                                   const float2* __restrict__ y, const int64_t* __restrict__ kprob,
                                                   const ProbState* __restrict__ st, double* part) {
@@ -590,8 +598,13 @@ __global__ __launch_bounds__(RT) void k_ls_kspace(const float2* __restrict__ Ax,
   if (st[p].status == MRH_CG_ACTIVE) {
     const int64_t k0 = kprob[p], k1 = kprob[p + 1];
     for (int64_t i = k0 + blockIdx.x * (int64_t)RT + threadIdx.x; i < k1; i += (int64_t)gridDim.x * RT) {
-      const float2 ax = Ax[i], yy = y[i], ad = Ad[i];
-      const float rx = ax.x - yy.x, ry = ax.y - yy.y;
+      const float2 ax = Ax[i], ad = Ad[i];
+      float rx = ax.x, ry = ax.y;
+      if (!RES) {
+        const float2 yy = y[i];
+        rx = ax.x - yy.x;
+        ry = ax.y - yy.y;
+      }
       v[0] += (R)rx * rx + (R)ry * ry;
       v[1] += (R)rx * ad.x + (R)ry * ad.y;
       v[2] += (R)ad.x * ad.x + (R)ad.y * ad.y;
@@ -1562,10 +1575,15 @@ extern "C" int mrh_cg_residual(mrh_cg* e, const void* Ax, const void* y, void* r
 
 extern "C" int mrh_cg_linesearch(mrh_cg* e, const void* x, const void* d, const void* g, const void* Ax,
                                  const void* Ad, const void* y, void* stream) {
-  MRH_REQUIRE(e && x && d && g && Ax && Ad && y, MRH_EINVAL, "cg_linesearch: null argument");
+  MRH_REQUIRE(e && x && d && g && Ax && Ad, MRH_EINVAL, "cg_linesearch: null argument");
   hipStream_t st = (hipStream_t)stream;
-  LAUNCH_R(e, k_ls_kspace, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
-                     (const float2*)y, e->d_kprob, e->d_st, e->d_part_k);
+  if (y) {
+    LAUNCH_R2(e, k_ls_kspace, false, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
+              (const float2*)y, e->d_kprob, e->d_st, e->d_part_k);
+  } else {                                 // Ax holds the residual r = Ax - y
+    LAUNCH_R2(e, k_ls_kspace, true, dim3(e->nb_k, e->P), dim3(RT), 0, st, (const float2*)Ax, (const float2*)Ad,
+              (const float2*)nullptr, e->d_kprob, e->d_st, e->d_part_k);
+  }
   MRH_CHECK_LAUNCH();
   const double gl_lam = e->gl.lam;
   const double* part_q = e->gl.lam != 0.f ? e->d_part_q : nullptr;

@@ -160,8 +160,9 @@ class NLCGEngine:
     def step(self) -> None:
         """One NLCG iteration for every active problem (no host synchronisation)."""
         lib, e, s = self._lib, self.handle, self._st()
+        # r = Ax - y is current (prime and every accept leave it), so the line search reads r alone
         _lib.check(lib.mrh_cg_linesearch(e, self.x.data_ptr(), self.d.data_ptr(), self.g.data_ptr(),
-                                         self.Ax.data_ptr(), self.Ad.data_ptr(), self.y.data_ptr(), s))
+                                         self.r.data_ptr(), self.Ad.data_ptr(), None, s))
         _lib.check(lib.mrh_cg_accept(e, self.x.data_ptr(), self.d.data_ptr(), self.Ax.data_ptr(),
                                      self.Ad.data_ptr(), self.y.data_ptr(), self.r.data_ptr(), s))
         if self.skip_stopped:
