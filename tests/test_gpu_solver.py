@@ -377,6 +377,53 @@ def test_graph_replay_equals_eager(ls_name):
     assert abs(f_graph - f_eager) <= 1e-6 * abs(f_eager)
 
 
+class _AxYLineSearchEngine(NLCGEngine):
+    """The engine with the line search reading Ax and y (the entry point's other input form)."""
+
+    calls = 0
+
+    def step(self) -> None:
+        lib = self._lib
+        real = lib.mrh_cg_linesearch
+
+        def with_ax_y(eng, x, d, g, r, Ad, y, st):
+            assert y is None                      # the engine's own call passes the residual
+            type(self).calls += 1
+            return real(eng, x, d, g, self.Ax.data_ptr(), Ad, self.y.data_ptr(), st)
+
+        lib.mrh_cg_linesearch = with_ax_y
+        try:
+            super().step()
+        finally:
+            lib.mrh_cg_linesearch = real
+
+
+@pytest.mark.parametrize("ls_name", ["armijo", "wolfe"])
+def test_line_search_from_residual_equals_from_ax_and_y(ls_name):
+    """mrh_cg_linesearch with y null reads the residual r that prime and accept leave (the engine's
+    form); with Ax and y it forms r itself: the same FP32 subtraction, so both trajectories agree bit
+    for bit (steps, objectives and the final image)."""
+    wl = small_dynamic(seed=33)
+    be, ora = prepare(wl)
+    regm, _ = regs_for(wl)
+    x0 = c64(ora.adjoint(wl.y))
+    runs = []
+    _AxYLineSearchEngine.calls = 0
+    for cls in (NLCGEngine, _AxYLineSearchEngine):
+        eng = cls(be.plan, c64(wl.y), problems=1, frames=wl.B, regm=regm, direction="prplus", ls_name=ls_name,
+                  c2=0.1)
+        eng.prime(x0)
+        eng.run(8, use_graph=False)
+        torch.cuda.synchronize()
+        runs.append((eng.history()[0], eng.x.clone()))
+    (h_r, x_r), (h_a, x_a) = runs
+    assert _AxYLineSearchEngine.calls >= 8
+    assert len(h_r) == len(h_a) == 8
+    for a, b in zip(h_r, h_a):
+        assert a["step"] == b["step"] and a["f"] == b["f"] and a["f_step"] == b["f_step"]
+    assert torch.equal(x_r, x_a)
+
+
 def test_rel_tol_stop_rule():
     wl = small_dynamic(seed=41)
     eng, hist, ref, _ = run_pair(wl, 60, rel_tol=1e-4)
